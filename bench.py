@@ -1,0 +1,155 @@
+"""Benchmark: CSD voxels/s on one MI355X (the dwi2fod csd hot path).
+
+Workload: the whole-brain-size single-shell phantom (128x128x72, ellipsoidal mask with 646,128
+voxels, 8 b=0 + 64 b=3000 volumes, lmax 8, 300 constraint directions, reference defaults).
+A step is one solve of every masked voxel with the signals already resident in HBM.
+
+Prints one JSON line with the throughput, the roofline position of the solver kernel (FP64
+algorithmic FLOPs of the reference algorithm, counted from per-voxel iteration statistics the
+kernel accumulates) and a CPU baseline: the oracle (this project's C restatement of the
+reference algorithm) on the host's cores over a bounded sample of the same voxels.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "dwi2fod voxels/s on 1 MI355X: plain vs BF16+NN warm start at ≤1e-4 SH err"
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
+    """Algorithmic FLOPs of the reference CSD for the counted work.
+    stats = (voxels, solve iterations, sum of |neg| over solves, converged voxels)."""
+    v, solves, ksum, conv = [float(x) for x in stats]
+    n, N, R, nr = float(n_sh), float(n_dw), float(n_hr), float(n_init)
+    set_ = v * (2 * nr * N + 2 * n * N)
+    gemv = (solves + conv) * 2 * R * n
+    syrk = ksum * n * (n + 1)
+    llt = solves * (n ** 3 / 3.0 + 2 * n * n)
+    return dict(total=set_ + gemv + syrk + llt, set=set_, amplitudes=gemv, normal_matrix=syrk, solve=llt)
+
+
+def load_traffic(kernel_tag):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), when a profile summary has been recorded for this kernel."""
+    path = os.path.join(ROOT, "profiles", kernel_tag + "_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(ph, gpu_fod, sample):
+    from oracle import orc
+    o = orc.CsdOracle(ph.grad, ph.response)
+    sample = min(sample, ph.nvox)
+    sample -= sample % 128
+    sub = np.ascontiguousarray(ph.dw[:sample])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    fod, iters = o.run_volume(sub, (128, sample // 128, 1), nthreads=threads)
+    dt = time.perf_counter() - t0
+    ref = fod.astype(np.float64)
+    g = gpu_fod[:sample].astype(np.float64)
+    rel = np.linalg.norm(g - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
+    return dict(value=sample / dt, unit="voxels/s", cores=threads, kind="port",
+                sample="first %d masked voxels of the same phantom, oracle C restatement (FP64, "
+                       "reference x-line scheduling), %.1f s wall" % (sample, dt)), rel
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the CSD voxel solve shards by voxel slabs with no "
+              "exchange, and this build runs and reports a single MI355X (--gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mrtrix3_amd import csd, phantom
+
+    dev = torch.device("cuda:0")
+    print("[bench] generating single_shell_brain phantom", file=sys.stderr, flush=True)
+    ph = phantom.single_shell_brain()
+    shared = csd.CSDShared(ph.grad, ph.response)
+    solver = csd.CSD(shared)
+    dw = torch.from_numpy(ph.dw).to(dev)
+    fod = torch.empty((ph.nvox, shared.n_sh), dtype=torch.float32, device=dev)
+    stats = torch.zeros(4, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+
+    print("[bench] warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        solver.run(dw, fod=fod, iters=False, nneg=False)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream(dev)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    print("[bench] timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        solver.run(dw, fod=fod, iters=False, nneg=False, stats=stats if i == 0 else None)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    st = stats.cpu().numpy()
+    fl = csd_flops(shared.n_sh, shared.n_init, shared.n_dw, shared.n_hr, st)
+    achieved = fl["total"] / (kern_ms * 1e-3) / 1e12
+    traffic = load_traffic("csd_persistent_kernel_single_shell_brain")
+
+    out = {
+        "metric": METRIC,
+        "value": ph.nvox * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single_shell_brain 128x128x72 ellipsoid mask, %d voxels, 8 b=0 + 64 "
+                               "b=3000, lmax 8, 300 constraint directions" % ph.nvox,
+                   "form": "plain GPU CSD: FP64 projection and solve, cold start, persistent kernel",
+                   "solves_per_voxel": float(st[1]) / max(1, st[0]),
+                   "mean_neg": float(st[2]) / max(1, st[1]),
+                   "converged_frac": float(st[3]) / max(1, st[0])},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": "csd_persistent_kernel<45>", "kernel_ms": kern_ms,
+                     "flop_per_launch": fl["total"],
+                     "note": "FP64-bound; peak is the MI355X FP64 rate (vector = matrix)"},
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline on %d voxels" % args.cpu_sample, file=sys.stderr, flush=True)
+        base, rel = cpu_baseline(ph, fod.cpu().numpy(), args.cpu_sample)
+        out["cpu_baseline"] = base
+        out["parity_on_cpu_sample"] = {"max_rel_sh_err": float(rel.max()),
+                                       "p999_rel_sh_err": float(np.quantile(rel, 0.999)),
+                                       "frac_le_1e-4": float(np.mean(rel <= 1e-4))}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+"""Build the native parts of mrtrix3_amd in-tree.
+
+Products (all under mrtrix3_amd/):
+  lib/libmrcsd.so   HIP kernels for gfx950 + the C ABI of include/*.h + the C++ host set-up code
+  bin/dwi2fod       the command-line drop-in (C++), linked against lib/libmrcsd.so
+The CPU oracle under oracle/ is test infrastructure and is built by its own Makefile.
+
+Every HIP translation unit is compiled with `-Xclang -target-feature -Xclang -packed-fp32-ops`
+disabled (see HIPFLAGS): packed FP32 instructions are avoided on this target.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I" + CSRC, "-I" + os.path.join(REPO, "include")]
+
+LIB_SOURCES = [
+    "kernels/csd_solve.hip",
+    "host/sh.cpp",
+    "host/dense.cpp",
+    "host/gradient.cpp",
+    "host/csd.cpp",
+    "capi_common.cpp",
+    "capi_csd.cpp",
+]
+
+
+def _gen_dirs_inc():
+    src = os.path.join(PKG, "data", "electrostatic_repulsion_300.txt")
+    dst = os.path.join(CSRC, "host", "dirs300.inc")
+    vals = []
+    for line in open(src):
+        line = line.split("#")[0].strip()
+        if line:
+            vals.extend(line.split())
+    if len(vals) != 600:
+        raise RuntimeError("expected 300 [az el] pairs in %s" % src)
+    text = ("// Generated from mrtrix3_amd/data/electrostatic_repulsion_300.txt by mrtrix3_amd/build.py.\n"
+            "// [azimuth elevation] pairs, MRtrix3 src/dwi/directions/predefined.cpp:92-393 (MPL-2.0).\n")
+    text += "".join("%s, %s,\n" % (vals[i], vals[i + 1]) for i in range(0, 600, 2))
+    if not os.path.exists(dst) or open(dst).read() != text:
+        open(dst, "w").write(text)
+
+
+def _compile(rel):
+    src = os.path.join(CSRC, rel)
+    obj = os.path.join(OBJ, rel.replace("/", "_") + ".o")
+    lang = ["-x", "hip"] if rel.endswith(".hip") else []
+    cmd = [HIPCC] + HIPFLAGS + lang + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return obj
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    _gen_dirs_inc()
+    jobs = min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile, LIB_SOURCES))
+    so = os.path.join(LIB, "libmrcsd.so")
+    tmp = so + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, so)
+    cli = os.path.join(CSRC, "dwi2fod.cpp")
+    if os.path.exists(cli):
+        exe = os.path.join(BIN, "dwi2fod")
+        cmd = [HIPCC] + HIPFLAGS + [cli, "-o", exe + ".tmp", "-L" + LIB, "-lmrcsd",
+                                    "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+        os.replace(exe + ".tmp", exe)
+    if verbose:
+        print("built", so)
+    return so
+
+
+def build_oracle():
+    r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed\n%s%s" % (r.stdout, r.stderr))
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    if "--oracle" in sys.argv:
+        build_oracle()

@@ -1,0 +1,41 @@
+// Device-side problem description and launch parameters of the batched CSD solver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mrcsd {
+
+// All matrices FP64 in device memory (see csd_engine.cpp for how they are laid out).
+struct CsdDeviceProblem {
+  const double* MtM;     // [n_sh][n_sh] symmetric, norm lambda on the diagonal
+  const double* HR;      // [n_hr][n_sh] constraint rows (row gathers for the normal matrix)
+  const double* HRt;     // [n_sh][n_hr_pad] transposed, zero padded (coalesced amplitude GEMV)
+  const double* M;       // [n_dw][n_sh] forward convolution (Mt_b = M^T b)
+  const double* rconvT;  // [n_dw][n_init] initialisation matrix, transposed
+  const int* dwis;       // [n_dw] volume index of each selected DW signal
+  double thr;            // scaled threshold
+  int n_dw, n_sh, n_init, n_hr, n_hr_pad, niter;
+};
+
+struct CsdLaunch {
+  const float* dw;       // [nvox][ld_dw]
+  int ld_dw;
+  const uint8_t* valid;  // [nvox] or null
+  size_t nvox;
+  float* fod;            // [nvox][n_sh]
+  uint16_t* iters;       // [nvox] or null
+  uint16_t* nneg;        // [nvox] or null
+  unsigned long long* stats;    // [4] or null
+  const float* warm_F0;  // [nvox][n_sh] or null
+  unsigned long long* negmask;  // [nvox][n_hr_pad / 64] or null: neg set of the last iterate()
+  unsigned long long* counter;  // work counter (device scalar)
+  int max_groups;        // 0 = fill the device
+};
+
+hipError_t csd_launch(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream);
+bool csd_supported_nsh(int n_sh);
+int csd_threads_for(int n_sh);
+
+}  // namespace mrcsd

@@ -1,0 +1,289 @@
+// Batched constrained spherical deconvolution on gfx950 (MI355X).
+//
+// One workgroup (NT threads = NSH rounded up to whole wavefronts) solves one voxel at a time and
+// pulls the next voxel from a global work counter when it finishes, so voxels that converge after
+// different numbers of iterations retire and refill without a relaunch (persistent kernel).
+//
+// Per voxel this is CSD::set + the dwi2fod iterate loop (src/dwi/sdeconv/csd.h:236-269,
+// cmd/dwi2fod.cpp:128-173), all in FP64:
+//   set      : Mt_b = M^T b and F = [rconv b; 0]                     (thread i owns coefficient i)
+//   iterate  : a = HR_trans F over the constraint directions, neg = {r : a_r < threshold}
+//              (ballots build a bitmask); unchanged neg set => converged.
+//              W = Mt_M + sum_{r in neg} h_r h_r^T is kept in registers (thread i owns row i)
+//              and updated only by the directions that entered or left neg since the last
+//              solve (its rows live in LDS between iterations); then W F = Mt_b is solved by
+//              Gauss-Jordan elimination with diagonal pivots: thread i eliminates column k from
+//              its own row for every k != i, reading the pivot row k from LDS.  Because the
+//              trailing block of a symmetric matrix stays symmetric under elimination, pivot row
+//              k equals column k there, which every thread writes in one LDS store.  After the
+//              sweep F_i = b_i / a_ii, so no triangular factor is stored and no backward pass runs.
+// The reference solves with Cholesky LL^T (Eigen::LLT).  Gauss-Jordan without pivoting is
+// backward stable for symmetric positive definite W; the solutions agree to rounding, not bit
+// for bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "csd_solve.h"
+
+namespace mrcsd {
+namespace {
+
+constexpr int kMaxDirWords = 8;   // up to 512 constraint directions
+constexpr int kMaxDW = 256;       // up to 256 DW volumes per voxel
+constexpr int kMaxQ = 8;          // constraint directions per thread (n_hr_pad / NT)
+
+// A value the whole wavefront agrees on, moved to scalar registers so that loops over it are
+// scalar loops.
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(x & 0xffffffffull));
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Broadcast lane `src`'s double to the whole wavefront (src must be wave-uniform).
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(u & 0xffffffffull), src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The workgroup is a single wavefront, so LDS hand-offs only need the compiler to keep program
+// order and wait for the LDS queue; __syncthreads() provides both (the barrier itself is elided
+// for one-wave workgroups).
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+template <int NSH>
+__global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
+                                                            int ld_dw, const uint8_t* __restrict__ valid,
+                                                            size_t nvox, float* __restrict__ fod,
+                                                            uint16_t* __restrict__ iters_out,
+                                                            uint16_t* __restrict__ nneg_out,
+                                                            unsigned long long* __restrict__ stats,
+                                                            const float* __restrict__ warm_F0,
+                                                            unsigned long long* __restrict__ negmask,
+                                                            unsigned long long* __restrict__ counter) {
+  static_assert(NSH <= 64, "one wavefront owns one row per lane");
+  constexpr int NT = 64;
+  __shared__ double s_b[kMaxDW];
+  __shared__ double s_F[NT];
+  __shared__ double s_h[NT];
+  __shared__ double s_col[2][NT];
+  __shared__ double s_W[NSH * NSH];  // row i of the current normal matrix at [i*NSH, i*NSH+NSH)
+  __shared__ unsigned long long s_words[kMaxDirWords];
+
+  const int tid = threadIdx.x;
+  const int N = P.n_dw;
+  const int R = P.n_hr;
+  const int Rpad = P.n_hr_pad;
+  const int nq = Rpad / NT;   // host guarantees Rpad % 64 == 0 and Rpad / 64 <= kMaxDirWords
+  const bool row_ok = tid < NSH;
+  double* const wrow = s_W + (row_ok ? tid : NSH - 1) * NSH;   // rows >= NSH only read
+
+  unsigned long long st_vox = 0, st_solves = 0, st_k = 0, st_conv = 0;
+
+  for (;;) {
+    unsigned long long vox = 0;
+    if (tid == 0) vox = atomicAdd(counter, 1ull);
+    vox = uniform64(vox);
+    if (vox >= nvox) break;
+
+    // ---- gather the selected DW signals (dwi2fod.cpp:156-173) ----
+    bool bad = valid && !valid[vox];
+    const float* src = dw + vox * (size_t)ld_dw;
+    for (int r = tid; r < N; r += NT) {
+      const float v = src[P.dwis[r]];
+      if (!isfinite(v)) bad = true;
+      s_b[r] = v < 0.0f ? 0.0 : (double)v;
+    }
+    if (__ballot(bad)) {
+      if (row_ok) fod[vox * NSH + tid] = 0.0f;
+      if (tid == 0) {
+        if (iters_out) iters_out[vox] = 0xFFFF;
+        if (nneg_out) nneg_out[vox] = 0;
+      }
+      if (negmask && tid < nq) negmask[vox * nq + tid] = 0ull;
+      continue;
+    }
+    wave_sync();
+
+    // ---- CSD::set: Mt_b and the initial F (csd.h:236-243) ----
+    double mtb = 0.0, f = 0.0;
+    if (row_ok) {
+      for (int r = 0; r < N; ++r) mtb = fma(P.M[r * NSH + tid], s_b[r], mtb);
+      if (warm_F0) {
+        f = (double)warm_F0[vox * NSH + tid];
+      } else if (tid < P.n_init) {
+        for (int r = 0; r < N; ++r) f = fma(P.rconvT[r * P.n_init + tid], s_b[r], f);
+      }
+    }
+    if (row_ok) {
+      const double* mrow = P.MtM + tid * NSH;
+#pragma unroll
+      for (int j = 0; j < NSH; ++j) wrow[j] = mrow[j];
+    }
+
+    unsigned long long oldw[kMaxDirWords];
+#pragma unroll
+    for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = 0ull;
+    bool have_old = false;   // old_neg = [-1] (csd.h:240)
+    int n = 0;
+    int last_k = 0;
+
+    for (n = 0; n < P.niter; ++n) {
+      // ---- amplitudes and the negative set (csd.h:246-250) ----
+      s_F[tid] = f;
+      wave_sync();
+      {
+        double a[kMaxQ];
+#pragma unroll
+        for (int q = 0; q < kMaxQ; ++q) a[q] = 0.0;
+        const double* hp = P.HRt + tid;
+#pragma unroll 5
+        for (int j = 0; j < NSH; ++j) {
+          const double fj = s_F[j];
+#pragma unroll
+          for (int q = 0; q < kMaxQ; ++q)
+            if (q < nq) a[q] = fma(hp[q * NT], fj, a[q]);
+          hp += Rpad;
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxQ; ++q) {
+          if (q < nq) {
+            const int r = tid + q * NT;
+            const unsigned long long ball = __ballot(r < R && a[q] < P.thr);
+            if (tid == 0) s_words[q] = ball;
+          }
+        }
+      }
+      wave_sync();
+      bool same = have_old;
+      int k = 0;
+      unsigned long long neww[kMaxDirWords];
+#pragma unroll
+      for (int w = 0; w < kMaxDirWords; ++w) {
+        neww[w] = w < nq ? uniform64(s_words[w]) : 0ull;
+        k += __popcll(neww[w]);
+        same = same && (neww[w] == oldw[w]);
+      }
+      if (same) break;   // converged (csd.h:252-253)
+
+      // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
+      double A[NSH];
+#pragma unroll
+      for (int j = 0; j < NSH; ++j) A[j] = row_ok ? wrow[j] : 0.0;
+#pragma unroll
+      for (int w = 0; w < kMaxDirWords; ++w) {
+        unsigned long long delta = have_old ? (neww[w] ^ oldw[w]) : neww[w];
+        while (delta) {
+          const int bit = __ffsll((long long)delta) - 1;
+          delta &= delta - 1;
+          const int r = w * 64 + bit;
+          const double hrow = row_ok ? P.HR[(size_t)r * NSH + tid] : 0.0;
+          wave_sync();   // earlier readers of s_h are done
+          s_h[tid] = hrow;
+          wave_sync();
+          const double hi = ((neww[w] >> bit) & 1ull) ? hrow : -hrow;
+#pragma unroll
+          for (int j = 0; j < NSH; ++j) A[j] = fma(hi, s_h[j], A[j]);
+        }
+      }
+      if (row_ok) {
+#pragma unroll
+        for (int j = 0; j < NSH; ++j) wrow[j] = A[j];
+      }
+#pragma unroll
+      for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = neww[w];
+      have_old = true;
+      last_k = k;
+      ++st_solves;
+      st_k += (unsigned long long)k;
+
+      // ---- Gauss-Jordan sweep on [W | Mt_b] (thread i owns row i) ----
+      double rhs = mtb;
+#pragma unroll
+      for (int kk = 0; kk < NSH; ++kk) {
+        const int buf = kk & 1;
+        s_col[buf][tid] = A[kk];                    // column kk == pivot row kk past the diagonal
+        const double d = lane_bcast(A[kk], kk);
+        const double bk = lane_bcast(rhs, kk);
+        wave_sync();
+        const bool other = tid != kk;
+        const double c = other ? A[kk] / d : 0.0;
+#pragma unroll
+        for (int j = kk + 1; j < NSH; ++j) A[j] = fma(-c, s_col[buf][j], A[j]);
+        rhs = fma(-c, bk, rhs);
+      }
+      double di = 1.0;
+#pragma unroll
+      for (int j = 0; j < NSH; ++j) di = (tid == j) ? A[j] : di;
+      f = row_ok ? rhs / di : 0.0;
+    }
+
+    if (row_ok) fod[vox * NSH + tid] = (float)f;
+    if (tid == 0) {
+      if (iters_out) iters_out[vox] = (uint16_t)n;
+      if (nneg_out) nneg_out[vox] = (uint16_t)last_k;
+    }
+    if (negmask) {
+      unsigned long long mine = 0ull;
+#pragma unroll
+      for (int w = 0; w < kMaxDirWords; ++w) mine = (tid == w) ? oldw[w] : mine;
+      if (tid < nq) negmask[vox * nq + tid] = mine;
+    }
+    ++st_vox;
+    if (n < P.niter) ++st_conv;
+    wave_sync();
+  }
+
+  if (stats && tid == 0) {
+    atomicAdd(stats + 0, st_vox);
+    atomicAdd(stats + 1, st_solves);
+    atomicAdd(stats + 2, st_k);
+    atomicAdd(stats + 3, st_conv);
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned long long* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+}
+
+template <int NSH>
+hipError_t launch_impl(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream) {
+  constexpr int NT = 64;
+  auto kern = csd_persistent_kernel<NSH>;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  int dev = 0, ncu = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  size_t grid = (size_t)ncu * (size_t)per_cu;
+  if (L.max_groups > 0 && grid > (size_t)L.max_groups) grid = (size_t)L.max_groups;
+  if (grid > L.nvox) grid = L.nvox ? L.nvox : 1;
+  reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
+  kern<<<dim3((unsigned)grid), dim3(NT), 0, stream>>>(P, L.dw, L.ld_dw, L.valid, L.nvox, L.fod, L.iters,
+                                                     L.nneg, L.stats, L.warm_F0, L.negmask, L.counter);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t csd_launch(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream) {
+  switch (P.n_sh) {
+    case 6:  return launch_impl<6>(P, L, stream);
+    case 15: return launch_impl<15>(P, L, stream);
+    case 28: return launch_impl<28>(P, L, stream);
+    case 45: return launch_impl<45>(P, L, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool csd_supported_nsh(int n_sh) {
+  return n_sh == 6 || n_sh == 15 || n_sh == 28 || n_sh == 45;
+}
+
+int csd_threads_for(int) { return 64; }
+
+}  // namespace mrcsd

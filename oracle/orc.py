@@ -1,0 +1,284 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liborc*.so) — TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, by __graft_entry__.smoke() as the checker, and by bench.py's CPU baseline leg.
+The product (mrtrix3_amd/) never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+_libs = {}
+
+
+def build():
+    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+class CsdShared(C.Structure):
+    _fields_ = [("n_dw", C.c_int), ("n_sh", C.c_int), ("n_init", C.c_int), ("n_hr", C.c_int),
+                ("lmax", C.c_int), ("lmax_response", C.c_int), ("lmax_data", C.c_int),
+                ("niter", C.c_int), ("threshold", C.c_double), ("norm_lambda", C.c_double),
+                ("rconv", C.POINTER(C.c_double)), ("M", C.POINTER(C.c_double)),
+                ("MtM", C.POINTER(C.c_double)), ("HR_trans", C.POINTER(C.c_double))]
+
+
+class IclsProblem(C.Structure):
+    _fields_ = [("nmeas", C.c_int), ("n", C.c_int), ("ncon", C.c_int), ("num_eq", C.c_int),
+                ("max_niter", C.c_int), ("lambda_min_norm", C.c_double), ("tol", C.c_double),
+                ("chol", C.POINTER(C.c_double)), ("b2d", C.POINTER(C.c_double)),
+                ("B", C.POINTER(C.c_double)), ("t", C.POINTER(C.c_double)),
+                ("H", C.POINTER(C.c_double))]
+
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+_SIG = {
+    "orc_nforl": (C.c_int, [C.c_int]),
+    "orc_lforn": (C.c_int, [C.c_int]),
+    "orc_plm_sph": (None, [D, C.c_int, C.c_int, C.c_double]),
+    "orc_sh_transform": (None, [D, C.c_int, C.c_int, D]),
+    "orc_zsh2rh": (None, [D, C.c_int, D]),
+    "orc_pinv": (C.c_int, [D, C.c_int, C.c_int, D]),
+    "orc_llt_solve": (C.c_int, [D, C.c_int, D, D]),
+    "orc_dw_scheme_normalise": (C.c_int, [D, C.c_int, C.c_int]),
+    "orc_gen_direction_matrix": (None, [D, I, C.c_int, D]),
+    "orc_shells_cluster": (C.c_int, [D, C.c_int, C.c_double, C.c_double, I]),
+    "orc_csd_select_volumes": (C.c_int, [D, C.c_int, C.c_double, C.c_double, I, I]),
+    "orc_csd_init": (C.c_int, [D, C.c_int, D, C.c_int, D, C.c_int, C.c_int, D, C.c_int, C.c_double,
+                               C.c_double, C.c_double, C.c_int, C.POINTER(C.POINTER(CsdShared)),
+                               C.c_char_p, C.c_int]),
+    "orc_csd_free": (None, [C.POINTER(CsdShared)]),
+    "orc_csd_solve": (C.c_int, [C.POINTER(CsdShared), D, D, I, I, I]),
+    "orc_csd_solve_from": (C.c_int, [C.POINTER(CsdShared), D, D, D, I, I, I]),
+    "orc_csd_run_volume": (C.c_int, [C.POINTER(CsdShared), C.c_void_p, C.c_int, I, C.c_void_p, C.c_int,
+                                     C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "orc_icls_problem_create": (C.c_int, [D, C.c_int, C.c_int, D, C.c_int, D, C.c_int, C.c_double,
+                                          C.c_double, C.c_int, C.c_double, C.c_int,
+                                          C.POINTER(C.POINTER(IclsProblem)), C.c_char_p, C.c_int]),
+    "orc_icls_problem_free": (None, [C.POINTER(IclsProblem)]),
+    "orc_icls_solve": (C.c_int, [C.POINTER(IclsProblem), D, D, C.c_void_p]),
+    "orc_icls_run_batch": (C.c_int, [C.POINTER(IclsProblem), C.c_void_p, C.c_size_t, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_void_p]),
+    "orc_msmt_build": (C.c_int, [D, C.c_int, I, C.c_int, D, I, C.c_int, I, D, C.c_int, D, D, I, I]),
+    "orc_real_name": (C.c_char_p, []),
+}
+
+
+def lib(long_double=False):
+    key = "ld" if long_double else "d"
+    if key in _libs:
+        return _libs[key]
+    path = os.path.join(BUILD, "liborc_ld.so" if long_double else "liborc.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    for name, (res, args) in _SIG.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _libs[key] = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(D)
+
+
+def _arr(x, dtype=np.float64):
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def sh_transform(dirs, lmax, long_double=False):
+    dirs = _arr(dirs)
+    n = lib().orc_nforl(lmax)
+    out = np.empty((dirs.shape[0], n))
+    lib(long_double).orc_sh_transform(_d(dirs), dirs.shape[0], lmax, _d(out))
+    return out
+
+
+def plm(lmax, m, x):
+    out = np.zeros(lmax + 1)
+    lib().orc_plm_sph(_d(out), lmax, m, float(x))
+    return out
+
+
+def zsh2rh(z):
+    z = _arr(z)
+    out = np.empty_like(z)
+    lib().orc_zsh2rh(_d(z), z.size, _d(out))
+    return out
+
+
+def pinv(M):
+    M = _arr(M)
+    out = np.empty((M.shape[1], M.shape[0]))
+    lib().orc_pinv(_d(M), M.shape[0], M.shape[1], _d(out))
+    return out
+
+
+def lforn(n):
+    return lib().orc_lforn(n)
+
+
+def normalise_grad(grad, scaling_mode=0):
+    g = _arr(grad).copy()
+    scaled = lib().orc_dw_scheme_normalise(_d(g), g.shape[0], scaling_mode)
+    return g, bool(scaled)
+
+
+def direction_matrix(grad, vols):
+    grad = _arr(grad)
+    v = _arr(vols, np.int32)
+    out = np.empty((v.size, 2))
+    lib().orc_gen_direction_matrix(_d(grad), v.ctypes.data_as(I), v.size, _d(out))
+    return out
+
+
+def shells_cluster(b, bzero=10.0, eps=80.0):
+    b = _arr(b)
+    cl = np.zeros(b.size, dtype=np.int32)
+    n = lib().orc_shells_cluster(_d(b), b.size, bzero, eps, cl.ctypes.data_as(I))
+    return n, cl
+
+
+def csd_select_volumes(grad, bzero=10.0, eps=80.0):
+    g = _arr(grad)
+    vols = np.zeros(g.shape[0], dtype=np.int32)
+    nv = C.c_int(0)
+    rc = lib().orc_csd_select_volumes(_d(g), g.shape[0], bzero, eps, vols.ctypes.data_as(I), C.byref(nv))
+    if rc:
+        raise ValueError("shell selection failed (%d)" % rc)
+    return vols[:nv.value].copy()
+
+
+def load_dirs300():
+    path = os.path.join(os.path.dirname(HERE), "mrtrix3_amd", "data", "electrostatic_repulsion_300.txt")
+    return np.loadtxt(path)
+
+
+class CsdOracle:
+    """CSD::Shared::init + per-voxel CSD solves, FP64 (or long double) on the CPU."""
+
+    def __init__(self, grad, response, lmax=0, filter=None, directions=None, neg_lambda=1.0,
+                 norm_lambda=1.0, threshold=0.0, niter=50, long_double=False, bzero=10.0, eps=80.0):
+        self.L = lib(long_double)
+        g, _ = normalise_grad(grad)
+        self.grad = g
+        self.vols = csd_select_volumes(g, bzero, eps)
+        dw_dirs = direction_matrix(g, self.vols)
+        hr = _arr(directions) if directions is not None else load_dirs300()
+        resp = _arr(np.ravel(response))
+        filt = _arr(filter) if filter is not None else None
+        err = C.create_string_buffer(256)
+        ptr = C.POINTER(CsdShared)()
+        rc = self.L.orc_csd_init(_d(dw_dirs), dw_dirs.shape[0], _d(hr), hr.shape[0], _d(resp), resp.size,
+                                 int(lmax), _d(filt) if filt is not None else None,
+                                 filt.size if filt is not None else 0, neg_lambda, norm_lambda, threshold,
+                                 int(niter), C.byref(ptr), err, 256)
+        if rc:
+            raise ValueError(err.value.decode())
+        self.p = ptr
+        s = ptr.contents
+        self.n_dw, self.n_sh, self.n_init, self.n_hr = s.n_dw, s.n_sh, s.n_init, s.n_hr
+        self.lmax, self.lmax_response, self.lmax_data = s.lmax, s.lmax_response, s.lmax_data
+        self.niter, self.threshold, self.norm_lambda = s.niter, s.threshold, s.norm_lambda
+        self.rconv = np.ctypeslib.as_array(s.rconv, (self.n_init, self.n_dw)).copy()
+        self.M = np.ctypeslib.as_array(s.M, (self.n_dw, self.n_sh)).copy()
+        self.MtM = np.ctypeslib.as_array(s.MtM, (self.n_sh, self.n_sh)).copy()
+        self.HR_trans = np.ctypeslib.as_array(s.HR_trans, (self.n_hr, self.n_sh)).copy()
+
+    def solve(self, b, F0=None):
+        """One voxel (selected, clamped signal). Returns (F, n, neg_list, trace_k)."""
+        b = _arr(b)
+        F = np.empty(self.n_sh)
+        neg = np.empty(self.n_hr, dtype=np.int32)
+        nneg = C.c_int(0)
+        trace = np.zeros(max(self.niter, 1), dtype=np.int32)
+        if F0 is None:
+            n = self.L.orc_csd_solve(self.p, _d(b), _d(F), neg.ctypes.data_as(I), C.byref(nneg),
+                                     trace.ctypes.data_as(I))
+        else:
+            f0 = _arr(F0)
+            n = self.L.orc_csd_solve_from(self.p, _d(b), _d(f0), _d(F), neg.ctypes.data_as(I),
+                                          C.byref(nneg), trace.ctypes.data_as(I))
+        return F, n, neg[:nneg.value].copy(), trace[:min(n, self.niter)].copy()
+
+    def run_volume(self, dw, shape=None, mask=None, nthreads=0):
+        """dw [nvox, nvol] float32 (x fastest).  Returns (fod float32 [nvox, n_sh], iters int16)."""
+        dw = _arr(dw, np.float32)
+        nvox, nvol = dw.shape
+        if shape is None:
+            shape = (nvox, 1, 1)
+        nx, ny, nz = shape
+        assert nx * ny * nz == nvox
+        fod = np.empty((nvox, self.n_sh), dtype=np.float32)
+        iters = np.empty(nvox, dtype=np.int16)
+        m = _arr(mask, np.uint8) if mask is not None else None
+        self.L.orc_csd_run_volume(self.p, dw.ctypes.data, nvol, self.vols.ctypes.data_as(I),
+                                  m.ctypes.data if m is not None else None, nx, ny, nz, int(nthreads),
+                                  fod.ctypes.data, iters.ctypes.data)
+        return fod, iters
+
+    def __del__(self):
+        p = getattr(self, "p", None)
+        if p:
+            self.L.orc_csd_free(p)
+            self.p = None
+
+
+class IclsOracle:
+    def __init__(self, H, A, t=None, num_eq=0, sol_reg=0.0, con_reg=0.0, max_iter=0, tol=0.0,
+                 standard_form=False, long_double=False):
+        self.L = lib(long_double)
+        H = _arr(H)
+        A = _arr(A)
+        tt = _arr(t) if t is not None and np.size(t) else None
+        err = C.create_string_buffer(256)
+        ptr = C.POINTER(IclsProblem)()
+        n = H.shape[1]
+        rc = self.L.orc_icls_problem_create(_d(H), H.shape[0], n, _d(A), A.shape[0],
+                                            _d(tt) if tt is not None else None, int(num_eq), sol_reg,
+                                            con_reg, int(max_iter), tol, int(standard_form),
+                                            C.byref(ptr), err, 256)
+        if rc:
+            raise ValueError(err.value.decode())
+        self.p = ptr
+        s = ptr.contents
+        self.n, self.ncon, self.nmeas, self.max_niter = s.n, s.ncon, s.nmeas, s.max_niter
+
+    def matrices(self):
+        s = self.p.contents
+        return dict(chol=np.ctypeslib.as_array(s.chol, (self.n, self.n)).copy(),
+                    b2d=np.ctypeslib.as_array(s.b2d, (self.nmeas, self.n)).copy(),
+                    B=np.ctypeslib.as_array(s.B, (self.ncon, self.n)).copy(),
+                    t=(np.ctypeslib.as_array(s.t, (self.ncon,)).copy() if s.t else None))
+
+    def solve(self, b):
+        b = _arr(b)
+        x = np.empty(self.n)
+        act = np.zeros(self.ncon, dtype=np.uint8)
+        it = self.L.orc_icls_solve(self.p, _d(b), _d(x), act.ctypes.data)
+        return x, it, act.astype(bool)
+
+    def run_batch(self, b, valid=None, nthreads=0):
+        b = _arr(b, np.float32)
+        nvox = b.shape[0]
+        x = np.empty((nvox, self.n), dtype=np.float32)
+        it = np.empty(nvox, dtype=np.int16)
+        v = _arr(valid, np.uint8) if valid is not None else None
+        self.L.orc_icls_run_batch(self.p, b.ctypes.data, nvox, v.ctypes.data if v is not None else None,
+                                  int(nthreads), x.ctypes.data, it.ctypes.data)
+        return x, it
+
+    def __del__(self):
+        p = getattr(self, "p", None)
+        if p:
+            self.L.orc_icls_problem_free(p)
+            self.p = None
