@@ -1,0 +1,76 @@
+/*
+ * mricls — batched inequality-constrained least squares and multi-tissue CSD set-up on the GPU
+ * (part of libmrcsd.so; conventions as in mrcsd.h: row-major doubles, status codes,
+ * mrcsd_last_error() for the message).
+ *
+ * Replaces, for a batch of right-hand sides:
+ *   - Math::ICLS::Problem / Solver          (core/math/constrained_least_squares.h:73-365)
+ *   - MSMT_CSD::Shared and the per-voxel MSMT_CSD solve
+ *                                           (src/dwi/sdeconv/msmt_csd.h:47-275)
+ *   - dwi2fod's MSMT_Processor              (cmd/dwi2fod.cpp:181-231)
+ */
+#ifndef MRICLS_H
+#define MRICLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mrcsd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mricls_ctx mricls_ctx;
+
+/* Problem(H, A, t, num_eq, sol_reg, con_reg, max_iter, tol, standard_form)
+ * (constrained_least_squares.h:91-152).  H [nmeas][n] (or [n][n] with H^T H in the lower
+ * triangle when standard_form), A [ncon][n] with the num_eq equality rows last, t [ncon] or NULL
+ * (zeros).  Limits: n <= 64, nmeas <= 512, ncon <= 512, at most 64 simultaneously active
+ * constraints (a voxel exceeding it reports iters 0xFFFE). */
+int mricls_create(const double* H, int nmeas, int n, const double* A, int ncon, const double* t,
+                  int num_eq, double sol_reg, double con_reg, int max_iter, double tol,
+                  int standard_form, mricls_ctx** out);
+void mricls_destroy(mricls_ctx* ctx);
+/* Dimensions and the effective max_niter (10 n when 0 was given, :105). */
+int mricls_dims(const mricls_ctx* ctx, int* nmeas, int* n, int* ncon, int* max_niter);
+
+/* Solver::operator()(x, b) (:237-357) for nvox right-hand sides.  Device pointers:
+ *   b       [nvox][ld_b] float, the first nmeas entries of each row
+ *   valid   [nvox] uint8 or NULL: 0 -> x = 0 and iters = 0xFFFF
+ *   x       [nvox][n] float
+ *   iters   [nvox] uint16 or NULL: the returned niter (0xFFFE: active set over capacity)
+ *   nactive [nvox] uint16 or NULL: size of the final active set
+ *   stats   uint64[8] or NULL, accumulated: {voxels, outer iterations, inner solves,
+ *           sum k, sum k^2, sum k^3 over inner solves, sum k over constraint updates,
+ *           voxels over capacity}
+ * Asynchronous on `stream`. */
+int mricls_solve(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
+                 float* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream);
+/* Host pointers (copies in, solves, copies out; synchronous). */
+int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
+                      float* x, uint16_t* iters, uint16_t* nactive);
+
+typedef struct mrmsmt_shared mrmsmt_shared;
+
+/* MSMT_CSD::Shared(header) + parse_cmdline_options() + set_responses() + init().
+ * grad [nvol][4] raw table; responses: tissue t is a [resp_rows[t]][resp_cols[t]] row-major block,
+ * blocks concatenated; lmax [n_lmax] (the -lmax list) or NULL = from the responses capped at 8
+ * (msmt_csd.h:102-114; n_lmax != ntissue is the reference's error);
+ * directions NULL = built-in 300; norm_lambda / neg_lambda < 0 = defaults (1e-10). */
+int mrmsmt_shared_create(const double* grad, int nvol, const double* responses,
+                         const int* resp_rows, const int* resp_cols, int ntissue, const int* lmax,
+                         int n_lmax, const double* directions, int n_directions,
+                         double norm_lambda, double neg_lambda, mrmsmt_shared** out);
+void mrmsmt_shared_destroy(mrmsmt_shared* s);
+/* nshell, nparams (sum of NforL(lmax_t)), ncon (constraint rows), lmax_out[ntissue]. */
+int mrmsmt_shared_info(const mrmsmt_shared* s, int* nshell, int* nparams, int* ncon, int* lmax_out);
+/* which: 0 C [nvol][nparams], 1 A [ncon][nparams] */
+int mrmsmt_shared_matrix(const mrmsmt_shared* s, int which, double* out, size_t capacity);
+/* The GPU solver of its ICLS problem. */
+int mrmsmt_icls_create(const mrmsmt_shared* s, mricls_ctx** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

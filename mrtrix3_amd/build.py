@@ -29,12 +29,16 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
 
 LIB_SOURCES = [
     "kernels/csd_solve.hip",
+    "kernels/icls_solve.hip",
     "host/sh.cpp",
     "host/dense.cpp",
     "host/gradient.cpp",
     "host/csd.cpp",
+    "host/icls.cpp",
+    "host/msmt.cpp",
     "capi_common.cpp",
     "capi_csd.cpp",
+    "capi_icls.cpp",
 ]
 
 
@@ -84,7 +88,8 @@ def build(verbose=False):
     cli = os.path.join(CSRC, "dwi2fod.cpp")
     if os.path.exists(cli):
         exe = os.path.join(BIN, "dwi2fod")
-        cmd = [HIPCC] + HIPFLAGS + [cli, "-o", exe + ".tmp", "-L" + LIB, "-lmrcsd",
+        cmd = [HIPCC] + HIPFLAGS + [cli, os.path.join(CSRC, "host", "mif.cpp"), "-o", exe + ".tmp",
+                                    "-L" + LIB, "-lmrcsd",
                                     "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

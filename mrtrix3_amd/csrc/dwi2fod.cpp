@@ -1,0 +1,367 @@
+// dwi2fod — command-line drop-in for MRtrix3's dwi2fod (cmd/dwi2fod.cpp) running the voxel fits
+// on the GPU through the C ABI of libmrcsd.so (include/mrcsd.h, include/mricls.h).
+//
+//   dwi2fod csd dwi.mif response.txt fod.mif [options]
+//   dwi2fod msmt_csd dwi.mif wm.txt wm.mif gm.txt gm.mif csf.txt csf.mif [options]
+//
+// Options kept from the reference: -mask, -lmax, -directions, -filter, -neg_lambda,
+// -norm_lambda, -threshold, -niter, -grad, -bvalue_scaling, -predicted_signal, -force, -quiet,
+// -info, -nthreads (accepted; the solve runs on the GPU).  Per-voxel semantics follow
+// dwi2fod.cpp:128-231: masked-out voxels are written as zeros, CSD voxels with a non-finite
+// signal give zeros, negative CSD signals are clamped, MSMT uses the raw signal.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/mrcsd.h"
+#include "../../include/mricls.h"
+#include "host/mif.h"
+
+namespace {
+
+bool g_quiet = false, g_info = false;
+
+void console(const std::string& s) {
+  if (!g_quiet) std::fprintf(stderr, "dwi2fod: %s\n", s.c_str());
+}
+void info(const std::string& s) {
+  if (g_info) std::fprintf(stderr, "dwi2fod: [INFO] %s\n", s.c_str());
+}
+
+std::vector<std::vector<double>> parse_rows(const std::string& text) {
+  std::vector<std::vector<double>> rows;
+  std::istringstream in(text);
+  std::string line;
+  while (std::getline(in, line)) {
+    const auto hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    for (char& c : line)
+      if (c == ',' || c == ';' || c == '\t' || c == '\r') c = ' ';
+    std::istringstream ls(line);
+    std::vector<double> r;
+    std::string tok;
+    while (ls >> tok) r.push_back(std::stod(tok));
+    if (!r.empty()) rows.push_back(r);
+  }
+  return rows;
+}
+
+std::vector<std::vector<double>> load_rows(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("cannot open file \"" + path + "\"");
+  std::stringstream ss;
+  ss << f.rdbuf();
+  auto r = parse_rows(ss.str());
+  if (r.empty()) throw std::runtime_error("no data in file \"" + path + "\"");
+  for (auto& row : r)
+    if (row.size() != r[0].size()) throw std::runtime_error("uneven rows in file \"" + path + "\"");
+  return r;
+}
+
+std::vector<double> flatten(const std::vector<std::vector<double>>& r) {
+  std::vector<double> out;
+  for (auto& row : r) out.insert(out.end(), row.begin(), row.end());
+  return out;
+}
+
+std::vector<int> parse_ints(const std::string& s) {
+  std::vector<int> out;
+  std::string cur;
+  for (char c : s + ",") {
+    if (c == ',') {
+      if (!cur.empty()) out.push_back(std::stoi(cur));
+      cur.clear();
+    } else {
+      cur += c;
+    }
+  }
+  return out;
+}
+
+std::string scheme_text(const std::vector<double>& g, int nvol) {
+  std::ostringstream o;
+  o.precision(17);
+  for (int r = 0; r < nvol; ++r) {
+    if (r) o << "\n";
+    o << g[size_t(r) * 4] << "," << g[size_t(r) * 4 + 1] << "," << g[size_t(r) * 4 + 2] << "," << g[size_t(r) * 4 + 3];
+  }
+  return o.str();
+}
+
+void check(int rc) {
+  if (rc != MRCSD_OK) throw std::runtime_error(mrcsd_last_error());
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Args {
+  std::vector<std::string> pos;
+  std::string mask, directions, filter, grad, predicted;
+  std::vector<int> lmax;
+  bool has_neg = false, has_norm = false, has_thr = false, has_niter = false, force = false;
+  double neg = 0, norm = 0, thr = 0;
+  int niter = 50, bscale = 0;
+};
+
+Args parse_args(int argc, char** argv) {
+  Args a;
+  for (int i = 1; i < argc; ++i) {
+    std::string s = argv[i];
+    auto next = [&]() -> std::string {
+      if (i + 1 >= argc) throw std::runtime_error("no value supplied for option \"" + s + "\"");
+      return argv[++i];
+    };
+    if (s.size() > 1 && s[0] == '-' && !std::isdigit((unsigned char)s[1])) {
+      std::string o = s.substr(s[1] == '-' ? 2 : 1);
+      if (o == "mask") a.mask = next();
+      else if (o == "lmax") a.lmax = parse_ints(next());
+      else if (o == "directions") a.directions = next();
+      else if (o == "filter") a.filter = next();
+      else if (o == "grad") a.grad = next();
+      else if (o == "predicted_signal") a.predicted = next();
+      else if (o == "neg_lambda") { a.has_neg = true; a.neg = std::stod(next()); }
+      else if (o == "norm_lambda") { a.has_norm = true; a.norm = std::stod(next()); }
+      else if (o == "threshold") { a.has_thr = true; a.thr = std::stod(next()); }
+      else if (o == "niter") { a.has_niter = true; a.niter = std::stoi(next()); }
+      else if (o == "bvalue_scaling") {
+        const std::string v = next();
+        a.bscale = (v == "yes" || v == "true" || v == "1") ? 1 : 2;
+      }
+      else if (o == "nthreads") next();
+      else if (o == "force") a.force = true;
+      else if (o == "quiet") g_quiet = true;
+      else if (o == "info") g_info = true;
+      else throw std::runtime_error("unknown option \"" + s + "\"");
+    } else {
+      a.pos.push_back(s);
+    }
+  }
+  return a;
+}
+
+bool exists(const std::string& p) {
+  struct stat st;
+  return ::stat(p.c_str(), &st) == 0;
+}
+
+// Gathers the voxels to solve: returns their canonical (x-fastest) indices and packs their
+// signals as rows [nsel][nvol] (the DWI buffer is in canonical order with axis 0 fastest).
+void gather(const mrcsd::MifImage& dwi, const std::vector<float>* mask, std::vector<int64_t>& sel,
+            std::vector<float>& rows) {
+  const int64_t nspatial = dwi.h.dim[0] * dwi.h.dim[1] * dwi.h.dim[2];
+  const int64_t nvol = dwi.h.dim[3];
+  sel.clear();
+  for (int64_t v = 0; v < nspatial; ++v)
+    if (!mask || (*mask)[size_t(v)] != 0.0f) sel.push_back(v);
+  rows.resize(sel.size() * size_t(nvol));
+  for (size_t i = 0; i < sel.size(); ++i)
+    for (int64_t c = 0; c < nvol; ++c) rows[i * size_t(nvol) + size_t(c)] = dwi.data[size_t(c * nspatial + sel[i])];
+}
+
+mrcsd::MifHeader output_header(const mrcsd::MifHeader& in, int64_t nvol_out, const std::vector<double>& grad,
+                               int nvol) {
+  mrcsd::MifHeader h = in;
+  h.dim = {in.dim[0], in.dim[1], in.dim[2], nvol_out};
+  h.vox.resize(3);
+  h.vox.push_back(1.0);
+  h.erase("dw_scheme");
+  h.set("prior_dw_scheme", scheme_text(grad, nvol));
+  for (const char* k : {"pe_scheme", "PhaseEncodingDirection", "TotalReadoutTime"}) h.erase(k);
+  return h;
+}
+
+void scatter_write(const std::string& path, const mrcsd::MifHeader& h, const std::vector<int64_t>& sel,
+                   const float* rows, int ncoef, int ld, int off) {
+  const int64_t nspatial = h.dim[0] * h.dim[1] * h.dim[2];
+  std::vector<float> out(size_t(nspatial * ncoef), 0.0f);
+  for (size_t i = 0; i < sel.size(); ++i)
+    for (int c = 0; c < ncoef; ++c) out[size_t(c * nspatial + sel[i])] = rows[i * size_t(ld) + size_t(off + c)];
+  mrcsd::write_mif(path, h, out, true);
+}
+
+int run(int argc, char** argv) {
+  Args a = parse_args(argc, argv);
+  if (a.pos.size() < 2) throw std::runtime_error("usage: dwi2fod algorithm dwi response odf [response odf ...]");
+  const std::string algo = a.pos[0];
+  if (algo != "csd" && algo != "msmt_csd")
+    throw std::runtime_error("algorithm must be one of: csd,msmt_csd");
+  console("reading image \"" + a.pos[1] + "\"");
+  mrcsd::MifImage dwi = mrcsd::read_mif(a.pos[1]);
+  if (dwi.h.dim.size() != 4) throw std::runtime_error("input DWI must be a 4-D image");
+  const int nvol = int(dwi.h.dim[3]);
+
+  std::vector<double> grad;
+  if (!a.grad.empty()) grad = flatten(load_rows(a.grad));
+  else grad = flatten(parse_rows(dwi.h.get("dw_scheme")));
+  if (grad.empty()) throw std::runtime_error("no valid diffusion gradient table found");
+  if (grad.size() != size_t(nvol) * 4)
+    throw std::runtime_error("number of studies in base image (" + std::to_string(nvol) +
+                             ") does not match number of rows in diffusion gradient table (" +
+                             std::to_string(grad.size() / 4) + ")");
+
+  std::vector<float> maskv;
+  if (!a.mask.empty()) {
+    mrcsd::MifImage m = mrcsd::read_mif(a.mask);
+    if (m.h.dim.size() < 3 || m.h.dim[0] != dwi.h.dim[0] || m.h.dim[1] != dwi.h.dim[1] || m.h.dim[2] != dwi.h.dim[2])
+      throw std::runtime_error("dimension mismatch between \"" + a.pos[1] + "\" and mask \"" + a.mask + "\"");
+    maskv.assign(m.data.begin(), m.data.begin() + dwi.h.dim[0] * dwi.h.dim[1] * dwi.h.dim[2]);
+  }
+  std::vector<int64_t> sel;
+  std::vector<float> rows;
+  gather(dwi, a.mask.empty() ? nullptr : &maskv, sel, rows);
+  const size_t nsel = sel.size();
+
+  if (algo == "csd") {
+    if (a.pos.size() != 4)
+      throw std::runtime_error("CSD algorithm expects a single input response function and single output FOD image");
+    if (exists(a.pos[3]) && !a.force) throw std::runtime_error("output image \"" + a.pos[3] + "\" already exists (use -force option to force overwrite)");
+    std::vector<double> response;
+    try {
+      auto r = load_rows(a.pos[2]);
+      if (r.size() != 1 && r[0].size() != 1) throw std::runtime_error("response file contains matrix, not vector");
+      response = flatten(r);
+    } catch (const std::exception& e) {
+      throw std::runtime_error(std::string(e.what()) + "\nCSD algorithm expects second argument to be the input response function file");
+    }
+    mrcsd_options opt;
+    mrcsd_options_default(&opt);
+    if (a.lmax.size() > 1) throw std::runtime_error("CSD algorithm expects a single lmax to be specified");
+    if (!a.lmax.empty()) opt.lmax = a.lmax[0];
+    std::vector<double> filt, dirs;
+    if (!a.filter.empty()) { filt = flatten(load_rows(a.filter)); opt.filter = filt.data(); opt.n_filter = int(filt.size()); }
+    if (!a.directions.empty()) {
+      auto d = load_rows(a.directions);
+      if (d[0].size() != 2) throw std::runtime_error("direction matrix should have 2 columns: [ azimuth elevation ]");
+      dirs = flatten(d);
+      opt.directions = dirs.data();
+      opt.n_directions = int(d.size());
+    }
+    if (a.has_neg) opt.neg_lambda = a.neg;
+    if (a.has_norm) opt.norm_lambda = a.norm;
+    if (a.has_thr) opt.threshold = a.thr;
+    if (a.has_niter) opt.niter = a.niter;
+    opt.bvalue_scaling = a.bscale;
+    mrcsd_shared* shared = nullptr;
+    check(mrcsd_shared_create(grad.data(), nvol, response.data(), int(response.size()), &opt, &shared));
+    mrcsd_shared_info si;
+    check(mrcsd_shared_info_get(shared, &si));
+    std::vector<double> gused(size_t(nvol) * 4);
+    check(mrcsd_shared_matrix(shared, 5, gused.data(), gused.size()));
+    mrcsd_ctx* ctx = nullptr;
+    check(mrcsd_create_from_shared(shared, &ctx));
+    console("performing constrained spherical deconvolution on " + std::to_string(nsel) + " voxels (GPU)");
+    std::vector<float> fod(nsel * size_t(si.n_sh));
+    std::vector<uint16_t> iters(nsel);
+    check(mrcsd_run_host(ctx, rows.data(), nvol, nullptr, nsel, fod.data(), iters.data(), nullptr));
+    size_t nonconv = 0;
+    for (size_t i = 0; i < nsel; ++i)
+      if (iters[i] != 0xFFFF && si.niter && iters[i] >= si.niter) ++nonconv;
+    if (nonconv) info(std::to_string(nonconv) + " voxels did not reach full convergence");
+    mrcsd::MifHeader h = output_header(dwi.h, si.n_sh, gused, nvol);
+    scatter_write(a.pos[3], h, sel, fod.data(), si.n_sh, si.n_sh, 0);
+    mrcsd_destroy(ctx);
+    mrcsd_shared_destroy(shared);
+    return 0;
+  }
+
+  // msmt_csd
+  if (a.pos.size() % 2)
+    throw std::runtime_error("MSMT_CSD algorithm expects pairs of (input response function & output FOD image) to be provided");
+  const int nt = int(a.pos.size() - 2) / 2;
+  std::vector<double> resp;
+  std::vector<int> rrows, rcols;
+  for (int t = 0; t < nt; ++t) {
+    const std::string& f = a.pos[size_t(2 + 2 * t)];
+    std::vector<std::vector<double>> r;
+    try {
+      r = load_rows(f);
+    } catch (const std::exception& e) {
+      throw std::runtime_error("File \"" + f + "\" is not a valid response function file\n"
+                               "MSMT_CSD algorithm expects the first file in each argument pair to be an input response function file");
+    }
+    rrows.push_back(int(r.size()));
+    rcols.push_back(int(r[0].size()));
+    auto fl = flatten(r);
+    resp.insert(resp.end(), fl.begin(), fl.end());
+    if (exists(a.pos[size_t(3 + 2 * t)]) && !a.force)
+      throw std::runtime_error("output image \"" + a.pos[size_t(3 + 2 * t)] + "\" already exists (use -force option to force overwrite)");
+  }
+  std::vector<double> dirs;
+  int ndirs = 0;
+  if (!a.directions.empty()) {
+    auto d = load_rows(a.directions);
+    dirs = flatten(d);
+    ndirs = int(d.size());
+  }
+  mrmsmt_shared* ms = nullptr;
+  check(mrmsmt_shared_create(grad.data(), nvol, resp.data(), rrows.data(), rcols.data(), nt,
+                             a.lmax.empty() ? nullptr : a.lmax.data(), int(a.lmax.size()),
+                             dirs.empty() ? nullptr : dirs.data(), ndirs,
+                             a.has_norm ? a.norm : -1.0, a.has_neg ? a.neg : -1.0, &ms));
+  int nshell = 0, nparams = 0, ncon = 0;
+  std::vector<int> lm(static_cast<size_t>(nt));
+  check(mrmsmt_shared_info(ms, &nshell, &nparams, &ncon, lm.data()));
+  mricls_ctx* ic = nullptr;
+  check(mrmsmt_icls_create(ms, &ic));
+  console("performing MSMT CSD (" + std::to_string(nshell) + " shell" + (nshell > 1 ? "s" : "") + ", " +
+          std::to_string(nt) + " tissue" + (nt > 1 ? "s" : "") + ") on " + std::to_string(nsel) + " voxels (GPU)");
+  std::vector<float> x(nsel * size_t(nparams));
+  std::vector<uint16_t> iters(nsel);
+  check(mricls_solve_host(ic, rows.data(), nvol, nullptr, nsel, x.data(), iters.data(), nullptr));
+  std::vector<double> gused = grad;   // stash the table as interpreted (normalised) by the set-up
+  {
+    for (int r = 0; r < nvol; ++r) {
+      double* g = &gused[size_t(r) * 4];
+      const double sq = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+      if (sq > 0) { const double n = std::sqrt(sq); g[0] /= n; g[1] /= n; g[2] /= n; }
+    }
+  }
+  int off = 0;
+  for (int t = 0; t < nt; ++t) {
+    const int ncoef = (lm[size_t(t)] + 1) * (lm[size_t(t)] + 2) / 2;
+    mrcsd::MifHeader h = output_header(dwi.h, ncoef, gused, nvol);
+    scatter_write(a.pos[size_t(3 + 2 * t)], h, sel, x.data(), ncoef, nparams, off);
+    off += ncoef;
+  }
+  if (!a.predicted.empty()) {
+    // predicted signal H x (dwi2fod.cpp:216-220), H = the MSMT problem matrix
+    std::vector<double> Cm(size_t(nvol) * nparams);
+    check(mrmsmt_shared_matrix(ms, 0, Cm.data(), Cm.size()));
+    std::vector<float> pred(nsel * size_t(nvol));
+    for (size_t i = 0; i < nsel; ++i)
+      for (int r = 0; r < nvol; ++r) {
+        double s = 0.0;
+        for (int j = 0; j < nparams; ++j) s += Cm[size_t(r) * nparams + j] * x[i * size_t(nparams) + j];
+        pred[i * size_t(nvol) + r] = float(s);
+      }
+    mrcsd::MifHeader h = dwi.h;
+    scatter_write(a.predicted, h, sel, pred.data(), nvol, nvol, 0);
+  }
+  mricls_destroy(ic);
+  mrmsmt_shared_destroy(ms);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  try {
+    return run(argc, argv);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "dwi2fod: [ERROR] %s\n", e.what());
+    return 1;
+  }
+}

@@ -1,0 +1,41 @@
+// Host-side set-up of the inequality-constrained least-squares problem
+// (Math::ICLS::Problem, core/math/constrained_least_squares.h:73-197).  The per-voxel solver
+// (Solver::operator(), :237-357) runs on the GPU, see kernels/icls_solve.hip.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "dense.h"
+
+namespace MR {
+namespace Math {
+namespace ICLS {
+
+using mrcsd::Matrix;
+
+class Problem {
+ public:
+  Problem() = default;
+  // :91-152.  Equality rows (if any) are the last num_equalities rows of the constraint matrix;
+  // an empty t means zeros.  max_iterations 0 means 10 * ncols.
+  Problem(const Matrix& problem_matrix, const Matrix& constraint_matrix,
+          const std::vector<double>& constraint_vector = {}, size_t num_equalities = 0,
+          double solution_min_norm_regularisation = 0.0,
+          double constraint_min_norm_regularisation = 0.0, size_t max_iterations = 0,
+          double tolerance = 0.0, bool problem_in_standard_form = false);
+
+  size_t num_parameters() const { return size_t(H.cols); }
+  size_t num_measurements() const { return size_t(b2d.rows); }
+  size_t num_constraints() const { return size_t(B.rows); }
+  size_t num_equalities() const { return num_eq; }
+
+  Matrix H, chol_HtH, B, b2d;
+  std::vector<double> t;
+  double lambda_min_norm = 0.0, tol = 0.0;
+  size_t max_niter = 0, num_eq = 0;
+};
+
+}  // namespace ICLS
+}  // namespace Math
+}  // namespace MR

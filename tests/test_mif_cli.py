@@ -1,0 +1,61 @@
+"""The .mif format helpers and the dwi2fod drop-in's argument handling (CPU only: these paths
+fail before any GPU work)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mrtrix3_amd import mif
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "mrtrix3_amd", "bin", "dwi2fod")
+
+
+def test_mif_roundtrip_layouts(tmp_path):
+    rng = np.random.default_rng(0)
+    data = rng.standard_normal((5, 4, 3, 7)).astype(np.float32)
+    for vc in (True, False):
+        p = str(tmp_path / ("a%d.mif" % vc))
+        mif.write(p, data, vox=[2.0, 2.0, 2.5], keyval=[("dw_scheme", "0,0,1,0\n1,0,0,1000")],
+                  volume_contiguous=vc)
+        back, h = mif.read(p)
+        assert np.array_equal(back, data)
+        assert h["dim"] == [5, 4, 3, 7]
+        assert [k for k, _ in h["keyval"]].count("dw_scheme") == 2
+        raw = open(p, "rb").read()
+        off = int(h["file"].split()[1])
+        assert off % 4 == 0 and raw[:13] == b"mrtrix image\n"
+
+
+def test_mif_reads_flipped_and_integer_layout(tmp_path):
+    # hand-written header: int16 big-endian, axis 0 stored reversed, axis 1 fastest
+    data = np.arange(2 * 3, dtype=np.int16).reshape(2, 3)
+    p = str(tmp_path / "b.mif")
+    head = b"mrtrix image\ndim: 2,3\nvox: 1,1\nlayout: -1,+0\ndatatype: Int16BE\nscaling: 1,2\nfile: . 96\nEND\n"
+    filed = np.ascontiguousarray(np.flip(data, 0)).astype(">i2")   # C order (x slowest) = y fastest
+    with open(p, "wb") as f:
+        f.write(head + b"\0" * (96 - len(head)) + filed.tobytes())
+    back, h = mif.read(p)
+    assert np.array_equal(back, 1 + 2 * data.astype(np.float32))
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="dwi2fod not built")
+@pytest.mark.parametrize("args,msg", [
+    (["csd"], "usage"),
+    (["foo", "x.mif", "r.txt", "o.mif"], "algorithm must be one of"),
+    (["csd", "/nonexistent.mif", "r.txt", "o.mif"], "failed to open image"),
+])
+def test_cli_argument_errors(args, msg):
+    r = subprocess.run([EXE] + args, capture_output=True, text=True)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="dwi2fod not built")
+def test_cli_csd_wrong_pair_count(tmp_path):
+    p = str(tmp_path / "d.mif")
+    mif.write(p, np.ones((2, 2, 2, 7), dtype=np.float32), keyval=[("dw_scheme", "\n".join(["0,0,1,1000"] * 7))])
+    r = subprocess.run([EXE, "csd", p, "r.txt", "o.mif", "x.mif"], capture_output=True, text=True)
+    assert r.returncode == 1 and "expects a single input response function" in r.stderr
+    r = subprocess.run([EXE, "msmt_csd", p, "r.txt", "o.mif", "x.mif"], capture_output=True, text=True)
+    assert r.returncode == 1 and "expects pairs" in r.stderr
