@@ -31,6 +31,7 @@ namespace {
 constexpr int kMaxDirWords = 8;   // up to 512 constraint directions
 constexpr int kMaxDW = 256;       // up to 256 DW volumes per voxel
 constexpr int kMaxQ = 8;          // constraint directions per thread (n_hr_pad / NT)
+constexpr int kChunk = 8;         // normal-matrix update rows staged per LDS hand-off
 
 // A value the whole wavefront agrees on, moved to scalar registers so that loops over it are
 // scalar loops.
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
   constexpr int NT = 64;
   __shared__ double s_b[kMaxDW];
   __shared__ double s_F[NT];
-  __shared__ double s_h[NT];
+  __shared__ double s_h[kChunk * NT];   // staged constraint rows of one update chunk
   __shared__ double s_col[2][NT];
   __shared__ double s_W[NSH * NSH];  // row i of the current normal matrix at [i*NSH, i*NSH+NSH)
   __shared__ unsigned long long s_words[kMaxDirWords];
@@ -169,23 +170,63 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
       if (same) break;   // converged (csd.h:252-253)
 
       // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
+      // Directions that entered (+) or left (-) neg are processed in chunks of kChunk: each lane
+      // stages its own coefficient of every row of the chunk in LDS, then adds the rank-kChunk
+      // update to its row of W.
       double A[NSH];
 #pragma unroll
       for (int j = 0; j < NSH; ++j) A[j] = row_ok ? wrow[j] : 0.0;
+      {
+        int w = 0;
+        unsigned long long delta = have_old ? (neww[0] ^ oldw[0]) : neww[0];
+        for (;;) {
+          // next chunk of changed directions (uniform scalar scan of the bitmasks)
+          int cnt = 0;
+          int rows_c[kChunk];
+          double sgn_c[kChunk];
 #pragma unroll
-      for (int w = 0; w < kMaxDirWords; ++w) {
-        unsigned long long delta = have_old ? (neww[w] ^ oldw[w]) : neww[w];
-        while (delta) {
-          const int bit = __ffsll((long long)delta) - 1;
-          delta &= delta - 1;
-          const int r = w * 64 + bit;
-          const double hrow = row_ok ? P.HR[(size_t)r * NSH + tid] : 0.0;
+          for (int c = 0; c < kChunk; ++c) {
+            while (delta == 0ull && w + 1 < nq) {
+              ++w;
+              unsigned long long nw = 0ull, ow = 0ull;
+#pragma unroll
+              for (int q = 0; q < kMaxDirWords; ++q) {
+                nw = (q == w) ? neww[q] : nw;
+                ow = (q == w) ? oldw[q] : ow;
+              }
+              delta = have_old ? (nw ^ ow) : nw;
+            }
+            rows_c[c] = 0;
+            sgn_c[c] = 0.0;
+            if (delta) {
+              const int bit = __ffsll((long long)delta) - 1;
+              delta &= delta - 1;
+              unsigned long long nw = 0ull;
+#pragma unroll
+              for (int q = 0; q < kMaxDirWords; ++q) nw = (q == w) ? neww[q] : nw;
+              rows_c[c] = w * 64 + bit;
+              sgn_c[c] = ((nw >> bit) & 1ull) ? 1.0 : -1.0;
+              ++cnt;
+            }
+          }
+          if (cnt == 0) break;
+          double hown[kChunk];
+#pragma unroll
+          for (int c = 0; c < kChunk; ++c)
+            hown[c] = (c < cnt) ? P.HR[(size_t)rows_c[c] * NSH + (row_ok ? tid : 0)] : 0.0;
           wave_sync();   // earlier readers of s_h are done
-          s_h[tid] = hrow;
-          wave_sync();
-          const double hi = ((neww[w] >> bit) & 1ull) ? hrow : -hrow;
 #pragma unroll
-          for (int j = 0; j < NSH; ++j) A[j] = fma(hi, s_h[j], A[j]);
+          for (int c = 0; c < kChunk; ++c) s_h[c * NT + tid] = row_ok ? hown[c] : 0.0;
+          wave_sync();
+#pragma unroll
+          for (int c = 0; c < kChunk; ++c) {
+            if (c < cnt) {
+              const double hi = sgn_c[c] * hown[c];
+#pragma unroll
+              for (int j = 0; j < NSH; ++j) A[j] = fma(hi, s_h[c * NT + j], A[j]);
+            }
+          }
+          if (cnt < kChunk) break;
         }
       }
       if (row_ok) {
