@@ -9,9 +9,9 @@
 //   set      : Mt_b = M^T b and F = [rconv b; 0]                     (thread i owns coefficient i)
 //   iterate  : a = HR_trans F over the constraint directions, neg = {r : a_r < threshold}
 //              (ballots build a bitmask); unchanged neg set => converged.
-//              W = Mt_M + sum_{r in neg} h_r h_r^T is kept in registers (thread i owns row i)
-//              and updated only by the directions that entered or left neg since the last
-//              solve (its rows live in LDS between iterations); then W F = Mt_b is solved by
+//              W = Mt_M + sum_{r in neg} h_r h_r^T (thread i owns row i while solving; the
+//              packed lower triangle lives in LDS between iterations) is updated only by the
+//              directions that entered or left neg since the last solve; then W F = Mt_b is solved by
 //              Gauss-Jordan elimination with diagonal pivots: thread i eliminates column k from
 //              its own row for every k != i, reading the pivot row k from LDS.  Because the
 //              trailing block of a symmetric matrix stays symmetric under elimination, pivot row
@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "csd_solve.h"
 
 namespace mrcsd {
@@ -32,6 +34,7 @@ constexpr int kMaxDirWords = 8;   // up to 512 constraint directions
 constexpr int kMaxDW = 256;       // up to 256 DW volumes per voxel
 constexpr int kMaxQ = 8;          // constraint directions per thread (n_hr_pad / NT)
 constexpr int kChunk = 8;         // normal-matrix update rows staged per LDS hand-off
+constexpr int kColStride = 66;    // doubles per pivot-column buffer (16-byte aligned)
 
 // A value the whole wavefront agrees on, moved to scalar registers so that loops over it are
 // scalar loops.
@@ -54,8 +57,78 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 // for one-wave workgroups).
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
+// The lane index as a value the compiler cannot treat as loop-invariant: per-column masks and
+// addresses derived from it are recomputed where they are used instead of being hoisted out of
+// the persistent voxel loop (where 45 of them would stay live in registers).
+__device__ __forceinline__ int opaque_lane() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// Row `i` of a symmetric matrix whose lower triangle is packed row by row (element (r,c), c <= r,
+// at r(r+1)/2 + c): the lower part comes from row i, the upper part from column i of later rows.
 template <int NSH>
-__global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
+__device__ __forceinline__ void load_sym_row(const double* W, double (&A)[NSH]) {
+  const int t = opaque_lane();
+  const int tt = t < NSH ? t : NSH - 1;
+  const double* lo = W + tt * (tt + 1) / 2;
+  const double* up = W + tt;
+#pragma unroll
+  for (int j = 0; j < NSH; ++j) {
+    const double a = lo[j <= tt ? j : 0];
+    const double b = up[j * (j + 1) / 2];
+    A[j] = (j <= tt) ? a : b;
+    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);   // bounds the loads in flight
+  }
+}
+
+// Stores the lower part of row i (columns 0..i) into the packed triangle.
+template <int NSH>
+__device__ __forceinline__ void store_lower_row(double* W, const double (&A)[NSH]) {
+  const int t = opaque_lane();
+  if (t < NSH) {
+    double* lo = W + t * (t + 1) / 2;
+#pragma unroll
+    for (int j = 0; j < NSH; ++j)
+      if (j <= t) lo[j] = A[j];
+  }
+}
+
+// One Gauss-Jordan step on [A | rhs], thread i owning row i: eliminates column KK from every
+// row but KK.  The pivot row past the diagonal equals column KK (symmetric trailing block), read
+// from `cur`; the next step's column is updated first and published to `nxt` (look-ahead) so its
+// LDS round trip overlaps the remaining FMAs.
+template <int NSH, int KK>
+__device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double* col0, double* col1, int tid) {
+  double* const cur = (KK & 1) ? col1 : col0;
+  double* const nxt = (KK & 1) ? col0 : col1;
+  const double d = lane_bcast(A[KK], KK);
+  const double bk = lane_bcast(rhs, KK);
+  const double c = (tid != KK) ? A[KK] / d : 0.0;
+  if constexpr (KK + 1 < NSH) {
+    A[KK + 1] = fma(-c, cur[KK + 1], A[KK + 1]);
+    nxt[tid] = A[KK + 1];
+  }
+#pragma unroll
+  for (int j = KK + 2; j < NSH; ++j) {
+    A[j] = fma(-c, cur[j], A[j]);
+    if (((j - KK) & 7) == 7) __builtin_amdgcn_sched_barrier(0);   // bounds the loads in flight
+  }
+  rhs = fma(-c, bk, rhs);
+  wave_sync();
+}
+
+template <int NSH, int... KK>
+__device__ __forceinline__ void gj_sweep(double (&A)[NSH], double& rhs, double* col0, double* col1, int tid,
+                                         std::integer_sequence<int, KK...>) {
+  (gj_step<NSH, KK>(A, rhs, col0, col1, tid), ...);
+}
+
+// amdgpu_waves_per_eu(3): caps the kernel at 168 VGPRs so three wavefronts share each SIMD
+// (the LDS footprint, ~12.8 KB per wavefront at NSH = 45, allows twelve per CU).
+template <int NSH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
                                                             int ld_dw, const uint8_t* __restrict__ valid,
                                                             size_t nvox, float* __restrict__ fod,
                                                             uint16_t* __restrict__ iters_out,
@@ -66,11 +139,13 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
                                                             unsigned long long* __restrict__ counter) {
   static_assert(NSH <= 64, "one wavefront owns one row per lane");
   constexpr int NT = 64;
-  __shared__ double s_b[kMaxDW];
+  constexpr int NTRI = NSH * (NSH + 1) / 2;
+  constexpr int COLS = 2 * kColStride;                       // two pivot-column buffers
+  constexpr int HBUF = kChunk * NSH > COLS ? kChunk * NSH : COLS;
+  __shared__ float s_b[kMaxDW];
   __shared__ double s_F[NT];
-  __shared__ double s_h[kChunk * NT];   // staged constraint rows of one update chunk
-  __shared__ double s_col[2][NT];
-  __shared__ double s_W[NSH * NSH];  // row i of the current normal matrix at [i*NSH, i*NSH+NSH)
+  __shared__ __attribute__((aligned(16))) double s_h[HBUF];  // update rows, then pivot columns
+  __shared__ double s_W[NTRI];        // packed lower triangle of the normal matrix
   __shared__ unsigned long long s_words[kMaxDirWords];
 
   const int tid = threadIdx.x;
@@ -79,7 +154,6 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
   const int Rpad = P.n_hr_pad;
   const int nq = Rpad / NT;   // host guarantees Rpad % 64 == 0 and Rpad / 64 <= kMaxDirWords
   const bool row_ok = tid < NSH;
-  double* const wrow = s_W + (row_ok ? tid : NSH - 1) * NSH;   // rows >= NSH only read
 
   unsigned long long st_vox = 0, st_solves = 0, st_k = 0, st_conv = 0;
 
@@ -95,7 +169,7 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
     for (int r = tid; r < N; r += NT) {
       const float v = src[P.dwis[r]];
       if (!isfinite(v)) bad = true;
-      s_b[r] = v < 0.0f ? 0.0 : (double)v;
+      s_b[r] = v < 0.0f ? 0.0f : v;
     }
     if (__ballot(bad)) {
       if (row_ok) fod[vox * NSH + tid] = 0.0f;
@@ -111,17 +185,19 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
     // ---- CSD::set: Mt_b and the initial F (csd.h:236-243) ----
     double mtb = 0.0, f = 0.0;
     if (row_ok) {
-      for (int r = 0; r < N; ++r) mtb = fma(P.M[r * NSH + tid], s_b[r], mtb);
+      for (int r = 0; r < N; ++r) mtb = fma(P.M[r * NSH + tid], (double)s_b[r], mtb);
       if (warm_F0) {
         f = (double)warm_F0[vox * NSH + tid];
       } else if (tid < P.n_init) {
-        for (int r = 0; r < N; ++r) f = fma(P.rconvT[r * P.n_init + tid], s_b[r], f);
+        for (int r = 0; r < N; ++r) f = fma(P.rconvT[r * P.n_init + tid], (double)s_b[r], f);
       }
     }
-    if (row_ok) {
-      const double* mrow = P.MtM + tid * NSH;
+    {
+      double A0[NSH];
+      const double* mrow = P.MtM + (row_ok ? tid : 0) * NSH;
 #pragma unroll
-      for (int j = 0; j < NSH; ++j) wrow[j] = mrow[j];
+      for (int j = 0; j < NSH; ++j) A0[j] = mrow[j];
+      store_lower_row<NSH>(s_W, A0);
     }
 
     unsigned long long oldw[kMaxDirWords];
@@ -140,13 +216,14 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
 #pragma unroll
         for (int q = 0; q < kMaxQ; ++q) a[q] = 0.0;
         const double* hp = P.HRt + tid;
-#pragma unroll 5
+#pragma unroll 3
         for (int j = 0; j < NSH; ++j) {
           const double fj = s_F[j];
 #pragma unroll
           for (int q = 0; q < kMaxQ; ++q)
             if (q < nq) a[q] = fma(hp[q * NT], fj, a[q]);
           hp += Rpad;
+          __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int q = 0; q < kMaxQ; ++q) {
@@ -170,17 +247,15 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
       if (same) break;   // converged (csd.h:252-253)
 
       // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
-      // Directions that entered (+) or left (-) neg are processed in chunks of kChunk: each lane
-      // stages its own coefficient of every row of the chunk in LDS, then adds the rank-kChunk
-      // update to its row of W.
+      // Row i of the symmetric W is read from the packed lower triangle (element (i,j) sits at
+      // max(i,j)(max(i,j)+1)/2 + min(i,j)).  Directions that entered (+) or left (-) neg are
+      // applied in chunks of kChunk rows staged in LDS.
       double A[NSH];
-#pragma unroll
-      for (int j = 0; j < NSH; ++j) A[j] = row_ok ? wrow[j] : 0.0;
+      load_sym_row<NSH>(s_W, A);
       {
         int w = 0;
         unsigned long long delta = have_old ? (neww[0] ^ oldw[0]) : neww[0];
         for (;;) {
-          // next chunk of changed directions (uniform scalar scan of the bitmasks)
           int cnt = 0;
           int rows_c[kChunk];
           double sgn_c[kChunk];
@@ -215,24 +290,26 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
           for (int c = 0; c < kChunk; ++c)
             hown[c] = (c < cnt) ? P.HR[(size_t)rows_c[c] * NSH + (row_ok ? tid : 0)] : 0.0;
           wave_sync();   // earlier readers of s_h are done
+          if (row_ok) {
 #pragma unroll
-          for (int c = 0; c < kChunk; ++c) s_h[c * NT + tid] = row_ok ? hown[c] : 0.0;
+            for (int c = 0; c < kChunk; ++c) s_h[c * NSH + tid] = hown[c];
+          }
           wave_sync();
 #pragma unroll
           for (int c = 0; c < kChunk; ++c) {
             if (c < cnt) {
               const double hi = sgn_c[c] * hown[c];
 #pragma unroll
-              for (int j = 0; j < NSH; ++j) A[j] = fma(hi, s_h[c * NT + j], A[j]);
+              for (int j = 0; j < NSH; ++j) {
+                A[j] = fma(hi, s_h[c * NSH + j], A[j]);
+                if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+              }
             }
           }
           if (cnt < kChunk) break;
         }
       }
-      if (row_ok) {
-#pragma unroll
-        for (int j = 0; j < NSH; ++j) wrow[j] = A[j];
-      }
+      store_lower_row<NSH>(s_W, A);
 #pragma unroll
       for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = neww[w];
       have_old = true;
@@ -241,20 +318,17 @@ __global__ __launch_bounds__(64) void csd_persistent_kernel(CsdDeviceProblem P, 
       st_k += (unsigned long long)k;
 
       // ---- Gauss-Jordan sweep on [W | Mt_b] (thread i owns row i) ----
+      // Step kk eliminates column kk from every other row.  Pivot row kk equals column kk past
+      // the diagonal, so each lane publishes its A[kk] as column kk.  The next column is
+      // updated first and published before the rest of the row (look-ahead), so its LDS
+      // round trip overlaps the remaining FMAs.
+      double* const col0 = s_h;
+      double* const col1 = s_h + kColStride;
       double rhs = mtb;
-#pragma unroll
-      for (int kk = 0; kk < NSH; ++kk) {
-        const int buf = kk & 1;
-        s_col[buf][tid] = A[kk];                    // column kk == pivot row kk past the diagonal
-        const double d = lane_bcast(A[kk], kk);
-        const double bk = lane_bcast(rhs, kk);
-        wave_sync();
-        const bool other = tid != kk;
-        const double c = other ? A[kk] / d : 0.0;
-#pragma unroll
-        for (int j = kk + 1; j < NSH; ++j) A[j] = fma(-c, s_col[buf][j], A[j]);
-        rhs = fma(-c, bk, rhs);
-      }
+      wave_sync();   // s_h (update rows) is free again
+      col0[tid] = A[0];
+      wave_sync();
+      gj_sweep<NSH>(A, rhs, col0, col1, tid, std::make_integer_sequence<int, NSH>{});
       double di = 1.0;
 #pragma unroll
       for (int j = 0; j < NSH; ++j) di = (tid == j) ? A[j] : di;

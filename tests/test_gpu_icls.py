@@ -52,7 +52,9 @@ def test_icls_golden_on_gpu(torch_cuda, name, kw, ref):
     o = orc.IclsOracle(**kw)
     xo, ito, acto = o.solve(b[0].astype(np.float64))       # the oracle on the same float32 input
     for r in range(3):
-        assert it[r] == ito
+        # with tol = 0 the reference takes one extra no-op outer iteration whenever rounding leaves
+        # an active constraint at c = -1e-17 instead of +1e-17, so niter may differ by one
+        assert abs(int(it[r]) - int(ito)) <= 1 and na[r] == acto.sum()
         # the float32 measurement rounding moves the answer by ~1e-7; the reference bar is 1e-6
         assert is_approx(x[r].astype(np.float64), ref, 2e-6), np.linalg.norm(x[r] - ref) / np.linalg.norm(ref)
         assert is_approx(x[r].astype(np.float64), xo, 1e-6)
@@ -73,10 +75,12 @@ def test_icls_random_vs_oracle(torch_cuda):
     torch.cuda.synchronize()
     x = x.cpu().numpy()
     it = it.cpu().numpy().astype(np.int64)
+    na = na.cpu().numpy().astype(np.int64)
     bad = 0
     for v in range(500):
-        xo, ito, _ = o.solve(B[v].astype(np.float64))
-        if not (np.linalg.norm(x[v] - xo) <= 1e-5 * max(np.linalg.norm(xo), 1e-12) and it[v] == ito):
+        xo, ito, acto = o.solve(B[v].astype(np.float64))
+        if not (np.linalg.norm(x[v] - xo) <= 1e-5 * max(np.linalg.norm(xo), 1e-12)
+                and abs(it[v] - ito) <= 1 and na[v] == acto.sum()):
             bad += 1
     say("[gpu icls] mismatches %d / 500" % bad)
     assert bad <= 1
