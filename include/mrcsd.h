@@ -97,9 +97,16 @@ int mrcsd_create(const mrcsd_params* p, mrcsd_ctx** out);
 int mrcsd_create_from_shared(const mrcsd_shared* s, mrcsd_ctx** out);
 void mrcsd_destroy(mrcsd_ctx* ctx);
 
-/* Optional warm start: per-voxel initial SH coefficients replacing rconv*b (csd.h:238). */
+/* Optional warm start (see mrwarm.h; no counterpart in the reference).  Each voxel first runs
+ * at most `budget` iterate() calls from the proposal: F0 replaces rconv*b (csd.h:238) and/or the
+ * proposed negative set replaces the first amplitude evaluation.  A voxel whose negative set
+ * reaches a fixed point there is accepted (for threshold 0 that is the unique minimiser, i.e. the
+ * plain solver's answer); any other voxel is solved again from rconv*b with the full niter budget,
+ * which reproduces the reference trajectory.  Ignored when niter is 0. */
 typedef struct mrcsd_warm {
-  const float* F0;         /* device [nvox][n_sh] */
+  const float* F0;         /* device [nvox][n_sh] or NULL */
+  const uint64_t* neg;     /* device [nvox][(n_hr+63)/64] proposed negative sets, or NULL */
+  int budget;              /* iterate() calls allowed from the proposal (>= 2 to certify) */
 } mrcsd_warm;
 
 /* Optional per-voxel outputs beyond the FOD. */
@@ -108,8 +115,9 @@ typedef struct mrcsd_outputs {
                               0xFFFF for skipped voxels */
   uint16_t* nneg;          /* [nvox] |neg| of the last iterate() */
   uint64_t* negmask;       /* [nvox][(n_hr+63)/64] bitmask of that neg set (bit r of word r/64) */
-  uint64_t* stats;         /* uint64[4], accumulated: {voxels solved, solve iterations,
-                              sum of |neg| over solves, converged voxels} */
+  uint64_t* stats;         /* uint64[6], accumulated: {voxels solved, solve iterations,
+                              sum of |neg| over solves, converged voxels, warm-start fallbacks,
+                              solves spent in warm-start passes} */
 } mrcsd_outputs;
 
 /* Solve nvox voxels.  Device pointers:

@@ -37,7 +37,7 @@ class Params(C.Structure):
 
 
 class Warm(C.Structure):
-    _fields_ = [("F0", C.c_void_p)]
+    _fields_ = [("F0", C.c_void_p), ("neg", C.c_void_p), ("budget", C.c_int)]
 
 
 class Outputs(C.Structure):

@@ -30,6 +30,7 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
 LIB_SOURCES = [
     "kernels/csd_solve.hip",
     "kernels/icls_solve.hip",
+    "kernels/mlp.hip",
     "host/sh.cpp",
     "host/dense.cpp",
     "host/gradient.cpp",
@@ -39,6 +40,7 @@ LIB_SOURCES = [
     "capi_common.cpp",
     "capi_csd.cpp",
     "capi_icls.cpp",
+    "capi_warm.cpp",
 ]
 
 

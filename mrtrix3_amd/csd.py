@@ -72,9 +72,9 @@ class CSDShared:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.load().mrcsd_shared_destroy(h)
-            self._h = None
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.mrcsd_shared_destroy(h)
+        self._h = None
 
 
 def _is_torch(x):
@@ -116,7 +116,7 @@ class CSD:
         return obj
 
     def run(self, dw, valid=None, iters=True, nneg=True, stats=None, warm_F0=None, fod=None,
-            negmask=False):
+            negmask=False, warm_neg=None, warm_budget=6):
         """Solve every row of dw ([nvox, nvol] float32).  Returns (fod, iters, nneg), plus the
         neg-set bitmask [nvox, words] when negmask=True (device path only)."""
         lib = _lib.load()
@@ -132,9 +132,14 @@ class CSD:
             if valid is not None:
                 assert valid.dtype == torch.uint8 and valid.is_cuda and valid.numel() == nvox
             warm = None
-            if warm_F0 is not None:
-                assert warm_F0.is_cuda and warm_F0.dtype == torch.float32 and warm_F0.shape == fod.shape
-                warm = _lib.Warm(warm_F0.data_ptr())
+            if warm_F0 is not None or warm_neg is not None:
+                if warm_F0 is not None:
+                    assert warm_F0.is_cuda and warm_F0.dtype == torch.float32 and warm_F0.shape == fod.shape
+                if warm_neg is not None:
+                    assert warm_neg.is_cuda and warm_neg.dtype == torch.int64
+                    assert warm_neg.shape == (nvox, (self.n_hr + 63) // 64)
+                warm = _lib.Warm(warm_F0.data_ptr() if warm_F0 is not None else None,
+                                 warm_neg.data_ptr() if warm_neg is not None else None, int(warm_budget))
             nm = None
             if negmask:
                 nm = torch.empty((nvox, (self.n_hr + 63) // 64), dtype=torch.int64, device=dev)
@@ -164,6 +169,6 @@ class CSD:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.load().mrcsd_destroy(h)
-            self._h = None
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.mrcsd_destroy(h)
+        self._h = None

@@ -121,7 +121,7 @@ int mrcsd_create(const mrcsd_params* p, mrcsd_ctx** out) {
       return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_create: null argument");
     if (!mrcsd::csd_supported_nsh(p->n_sh))
       return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_create: n_sh = " + std::to_string(p->n_sh) +
-                                            " not supported (lmax 2, 4, 6 or 8)");
+                                            " not supported (lmax 2 to 12)");
     if (p->n_dw <= 0 || p->n_dw > 256) return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_create: n_dw must be in [1, 256]");
     if (p->n_hr <= 0 || p->n_hr > 512) return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_create: n_hr must be in [1, 512]");
     if (p->n_init <= 0 || p->n_init > p->n_sh) return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_create: bad n_init");
@@ -131,10 +131,11 @@ int mrcsd_create(const mrcsd_params* p, mrcsd_ctx** out) {
     auto ctx = std::make_unique<mrcsd_ctx>();
     MRCSD_HIP(hipGetDevice(&ctx->device));
     const int n = p->n_sh, R = p->n_hr, N = p->n_dw, ni = p->n_init;
-    const int Rpad = (R + 63) / 64 * 64;
+    const int NT = mrcsd::csd_threads_for(n);
+    const int Rpad = (R + NT - 1) / NT * NT;
     std::vector<double> HRt(size_t(n) * Rpad, 0.0), rconvT(size_t(N) * ni);
     for (int r = 0; r < R; ++r)
-      for (int j = 0; j < n; ++j) HRt[size_t(j) * Rpad + r] = p->HR_trans[size_t(r) * n + j];
+      for (int j = 0; j < n; ++j) HRt[(size_t(r / NT) * n + j) * NT + r % NT] = p->HR_trans[size_t(r) * n + j];
     for (int i = 0; i < ni; ++i)
       for (int c = 0; c < N; ++c) rconvT[size_t(c) * ni + i] = p->rconv[size_t(i) * N + c];
     ctx->MtM.upload(p->MtM, size_t(n) * n);
@@ -200,6 +201,10 @@ int mrcsd_run(mrcsd_ctx* ctx, const float* dw, int ld_dw, const uint8_t* valid, 
       L.stats = reinterpret_cast<unsigned long long*>(out->stats);
     }
     L.warm_F0 = warm ? warm->F0 : nullptr;
+    L.warm_neg = warm ? reinterpret_cast<const unsigned long long*>(warm->neg) : nullptr;
+    L.warm_budget = warm ? warm->budget : 0;
+    if (warm && (warm->F0 || warm->neg) && (warm->budget < 1 || warm->budget > 65534))
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_run: warm-start budget must be in [1, 65534]");
     L.counter = ctx->counter.get();
     L.max_groups = 0;
     MRCSD_HIP(mrcsd::csd_launch(ctx->P, L, static_cast<hipStream_t>(stream)));

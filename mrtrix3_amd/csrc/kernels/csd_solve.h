@@ -11,7 +11,8 @@ namespace mrcsd {
 struct CsdDeviceProblem {
   const double* MtM;     // [n_sh][n_sh] symmetric, norm lambda on the diagonal
   const double* HR;      // [n_hr][n_sh] constraint rows (row gathers for the normal matrix)
-  const double* HRt;     // [n_sh][n_hr_pad] transposed, zero padded (coalesced amplitude GEMV)
+  const double* HRt;     // [n_hr_pad / NT][n_sh][NT]: direction r = b * NT + t of block b at (b, j, t),
+                         // zero padded (NT = csd_threads_for(n_sh)); coalesced amplitude GEMV
   const double* M;       // [n_dw][n_sh] forward convolution (Mt_b = M^T b)
   const double* rconvT;  // [n_dw][n_init] initialisation matrix, transposed
   const int* dwis;       // [n_dw] volume index of each selected DW signal
@@ -27,8 +28,10 @@ struct CsdLaunch {
   float* fod;            // [nvox][n_sh]
   uint16_t* iters;       // [nvox] or null
   uint16_t* nneg;        // [nvox] or null
-  unsigned long long* stats;    // [4] or null
+  unsigned long long* stats;    // [6] or null
   const float* warm_F0;  // [nvox][n_sh] or null
+  const unsigned long long* warm_neg;   // [nvox][n_hr_pad / 64] proposed negative sets, or null
+  int warm_budget;       // iterate() calls allowed from the warm start before the plain restart
   unsigned long long* negmask;  // [nvox][n_hr_pad / 64] or null: neg set of the last iterate()
   unsigned long long* counter;  // work counter (device scalar)
   int max_groups;        // 0 = fill the device

@@ -1,6 +1,7 @@
 // Batched constrained spherical deconvolution on gfx950 (MI355X).
 //
-// One workgroup (NT threads = NSH rounded up to whole wavefronts) solves one voxel at a time and
+// One workgroup (NT = 64 threads for up to 64 SH coefficients, 128 for lmax 10-12) solves one
+// voxel at a time and
 // pulls the next voxel from a global work counter when it finishes, so voxels that converge after
 // different numbers of iterations retire and refill without a relaunch (persistent kernel).
 //
@@ -32,9 +33,11 @@ namespace {
 
 constexpr int kMaxDirWords = 8;   // up to 512 constraint directions
 constexpr int kMaxDW = 256;       // up to 256 DW volumes per voxel
-constexpr int kMaxQ = 8;          // constraint directions per thread (n_hr_pad / NT)
 constexpr int kChunk = 8;         // normal-matrix update rows staged per LDS hand-off
-constexpr int kColStride = 66;    // doubles per pivot-column buffer (16-byte aligned)
+// doubles per pivot-column buffer: NT column entries, then (workgroups wider than one wavefront)
+// the NT right-hand sides, so that the pivot's right-hand side is read from LDS.
+template <int NT>
+constexpr int col_stride() { return NT == 64 ? NT + 2 : 2 * NT; }
 
 // A value the whole wavefront agrees on, moved to scalar registers so that loops over it are
 // scalar loops.
@@ -66,80 +69,172 @@ __device__ __forceinline__ int opaque_lane() {
   return t;
 }
 
+// An LDS index the compiler cannot relate to its neighbours.  Loads from different chunks of a row
+// then cannot be merged into one vectorised batch issued up front (which would keep the whole
+// row in flight and spill), while loads within a chunk still merge into 16-byte reads.
+__device__ __forceinline__ int opaque_index(int i) {
+  asm volatile("" : "+s"(i));
+  return i;
+}
+
+constexpr int kPipe = 8;   // elements per software-pipelined load chunk
+
+// The hot loops below are software pipelined by hand: the loads of chunk k+1 are issued before the
+// FMAs of chunk k (double buffering), and sched_barrier keeps the compiler from moving them back
+// next to their uses.  Left alone, the scheduler places every load directly before its FMA and
+// each FMA then waits for a full LDS or L2 round trip.
+
 // Row `i` of a symmetric matrix whose lower triangle is packed row by row (element (r,c), c <= r,
 // at r(r+1)/2 + c): the lower part comes from row i, the upper part from column i of later rows.
 template <int NSH>
 __device__ __forceinline__ void load_sym_row(const double* W, double (&A)[NSH]) {
   const int t = opaque_lane();
   const int tt = t < NSH ? t : NSH - 1;
-  const double* lo = W + tt * (tt + 1) / 2;
-  const double* up = W + tt;
+  const double* const lo = W + tt * (tt + 1) / 2;
+  const double* const up = W + tt;
+  double buf[2][kPipe];
 #pragma unroll
-  for (int j = 0; j < NSH; ++j) {
-    const double a = lo[j <= tt ? j : 0];
-    const double b = up[j * (j + 1) / 2];
-    A[j] = (j <= tt) ? a : b;
-    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);   // bounds the loads in flight
+  for (int i = 0; i < kPipe; ++i)
+    if (i < NSH) buf[0][i] = *((i <= tt) ? lo + i : up + i * (i + 1) / 2);
+#pragma unroll
+  for (int k = 0; k * kPipe < NSH; ++k) {
+    const int jn = (k + 1) * kPipe;
+    if (jn < NSH) {
+#pragma unroll
+      for (int i = 0; i < kPipe; ++i) {
+        const int j = jn + i;
+        if (j < NSH) buf[(k + 1) & 1][i] = *((j <= tt) ? lo + j : up + j * (j + 1) / 2);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < kPipe; ++i)
+      if (k * kPipe + i < NSH) A[k * kPipe + i] = buf[k & 1][i];
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-// Stores the lower part of row i (columns 0..i) into the packed triangle.
+// Stores the lower part of row i (columns 0..i) into the packed triangle.  Branch free: the other
+// columns, and threads without a row, write to their own slot of `junk` instead.
 template <int NSH>
-__device__ __forceinline__ void store_lower_row(double* W, const double (&A)[NSH]) {
+__device__ __forceinline__ void store_lower_row(double* W, double* junk, const double (&A)[NSH]) {
   const int t = opaque_lane();
-  if (t < NSH) {
-    double* lo = W + t * (t + 1) / 2;
+  const int tt = t < NSH ? t : -1;
+  double* const lo = W + (t < NSH ? t * (t + 1) / 2 : 0);
+  double* const jk = junk + t;
 #pragma unroll
-    for (int j = 0; j < NSH; ++j)
-      if (j <= t) lo[j] = A[j];
+  for (int j = 0; j < NSH; ++j) *((j <= tt) ? lo + j : jk) = A[j];
+}
+
+// A[j] = fma(s, v[j], A[j]) for j in [0, NSH), v in LDS.
+template <int NSH>
+__device__ __forceinline__ void axpy_lds(double (&A)[NSH], double s, const double* v) {
+  double buf[2][kPipe];
+  {
+    const double* p = v + opaque_index(0);
+#pragma unroll
+    for (int i = 0; i < kPipe; ++i)
+      if (i < NSH) buf[0][i] = p[i];
+  }
+#pragma unroll
+  for (int k = 0; k * kPipe < NSH; ++k) {
+    const int j = k * kPipe;
+    if (j + kPipe < NSH) {
+      const double* p = v + opaque_index(j + kPipe);
+#pragma unroll
+      for (int i = 0; i < kPipe; ++i)
+        if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < kPipe; ++i)
+      if (j + i < NSH) A[j + i] = fma(s, buf[k & 1][i], A[j + i]);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 // One Gauss-Jordan step on [A | rhs], thread i owning row i: eliminates column KK from every
 // row but KK.  The pivot row past the diagonal equals column KK (symmetric trailing block), read
 // from `cur`; the next step's column is updated first and published to `nxt` (look-ahead) so its
-// LDS round trip overlaps the remaining FMAs.
-template <int NSH, int KK>
-__device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double* col0, double* col1, int tid) {
+// LDS round trip overlaps the remaining FMAs.  The first chunk of the pivot column is in flight
+// during the division.  A step has no branches (the divide runs on every lane and is masked;
+// every lane publishes): a conditional block per step lets the compiler sink each step's FMAs
+// into the next one, which keeps all the loaded pivot columns live and spills.  The pivot row's
+// diagonal is taken into `di` at its own step, so columns already eliminated are dead afterwards
+// and the live part of the row shrinks as the sweep proceeds.
+template <int NSH, int NT, int KK>
+__device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& di, double* col0, double* col1,
+                                        int tid) {
   double* const cur = (KK & 1) ? col1 : col0;
   double* const nxt = (KK & 1) ? col0 : col1;
-  const double d = lane_bcast(A[KK], KK);
-  const double bk = lane_bcast(rhs, KK);
-  const double c = (tid != KK) ? A[KK] / d : 0.0;
-  if constexpr (KK + 1 < NSH) {
-    A[KK + 1] = fma(-c, cur[KK + 1], A[KK + 1]);
-    nxt[tid] = A[KK + 1];
+  constexpr int J0 = KK + 1;
+  double d, bk;
+  if constexpr (NT == 64) {
+    d = lane_bcast(A[KK], KK);
+    bk = lane_bcast(rhs, KK);
+  } else {
+    d = cur[KK];    // published by lane KK in the previous step (or before step 0)
+    bk = cur[NT + KK];
   }
+  double buf[2][kPipe];
+  if constexpr (J0 < NSH) {
+    const double* p = cur + opaque_index(J0);
 #pragma unroll
-  for (int j = KK + 2; j < NSH; ++j) {
-    A[j] = fma(-c, cur[j], A[j]);
-    if (((j - KK) & 7) == 7) __builtin_amdgcn_sched_barrier(0);   // bounds the loads in flight
+    for (int i = 0; i < kPipe; ++i)
+      if (J0 + i < NSH) buf[0][i] = p[i];
   }
-  rhs = fma(-c, bk, rhs);
+  const double q = A[KK] / d;
+  const double c = (tid != KK) ? q : 0.0;
+  di = (tid == KK) ? A[KK] : di;
+  const double rn = fma(-c, bk, rhs);
+#pragma unroll
+  for (int k = 0; J0 + k * kPipe < NSH; ++k) {
+    const int j = J0 + k * kPipe;
+    if (j + kPipe < NSH) {
+      const double* p = cur + opaque_index(j + kPipe);
+#pragma unroll
+      for (int i = 0; i < kPipe; ++i)
+        if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < kPipe; ++i)
+      if (j + i < NSH) A[j + i] = fma(-c, buf[k & 1][i], A[j + i]);
+    if (k == 0) {
+      nxt[tid] = A[J0];
+      if constexpr (NT != 64) nxt[NT + tid] = rn;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  rhs = rn;
   wave_sync();
 }
 
-template <int NSH, int... KK>
-__device__ __forceinline__ void gj_sweep(double (&A)[NSH], double& rhs, double* col0, double* col1, int tid,
-                                         std::integer_sequence<int, KK...>) {
-  (gj_step<NSH, KK>(A, rhs, col0, col1, tid), ...);
+template <int NSH, int NT, int... KK>
+__device__ __forceinline__ void gj_sweep(double (&A)[NSH], double& rhs, double& di, double* col0, double* col1,
+                                         int tid, std::integer_sequence<int, KK...>) {
+  (gj_step<NSH, NT, KK>(A, rhs, di, col0, col1, tid), ...);
 }
 
-// amdgpu_waves_per_eu(3): caps the kernel at 168 VGPRs so three wavefronts share each SIMD
-// (the LDS footprint, ~12.8 KB per wavefront at NSH = 45, allows twelve per CU).
-template <int NSH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
+// amdgpu_waves_per_eu: caps the 64-thread kernel at 168 VGPRs so three wavefronts share each SIMD
+// (the LDS footprint, ~12.8 KB per wavefront at NSH = 45, allows twelve per CU); the 128-thread
+// kernel keeps 91-wide rows in registers and runs two wavefronts per SIMD.
+template <int NSH, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3 : 2))) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
                                                             int ld_dw, const uint8_t* __restrict__ valid,
                                                             size_t nvox, float* __restrict__ fod,
                                                             uint16_t* __restrict__ iters_out,
                                                             uint16_t* __restrict__ nneg_out,
                                                             unsigned long long* __restrict__ stats,
                                                             const float* __restrict__ warm_F0,
+                                                            const unsigned long long* __restrict__ warm_neg,
+                                                            int warm_budget,
                                                             unsigned long long* __restrict__ negmask,
                                                             unsigned long long* __restrict__ counter) {
-  static_assert(NSH <= 64, "one wavefront owns one row per lane");
-  constexpr int NT = 64;
+  static_assert(NSH <= NT && (NT == 64 || NT == 128), "one thread owns one row");
+  constexpr int NWAVE = NT / 64;
   constexpr int NTRI = NSH * (NSH + 1) / 2;
+  constexpr int kColStride = col_stride<NT>();
   constexpr int COLS = 2 * kColStride;                       // two pivot-column buffers
   constexpr int HBUF = kChunk * NSH > COLS ? kChunk * NSH : COLS;
   __shared__ float s_b[kMaxDW];
@@ -147,20 +242,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void cs
   __shared__ __attribute__((aligned(16))) double s_h[HBUF];  // update rows, then pivot columns
   __shared__ double s_W[NTRI];        // packed lower triangle of the normal matrix
   __shared__ unsigned long long s_words[kMaxDirWords];
+  __shared__ unsigned long long s_vox;
+  __shared__ int s_flag[NWAVE];
 
   const int tid = threadIdx.x;
   const int N = P.n_dw;
   const int R = P.n_hr;
   const int Rpad = P.n_hr_pad;
-  const int nq = Rpad / NT;   // host guarantees Rpad % 64 == 0 and Rpad / 64 <= kMaxDirWords
+  const int nq = Rpad / NT;   // host guarantees Rpad % NT == 0 and Rpad / 64 <= kMaxDirWords
+  const int nwords = nq * NWAVE;          // bitmask words in use (word w: directions 64w..64w+63)
+  const int nwio = (R + 63) / 64;         // words per voxel in the caller's mask arrays
+  const int wave = tid >> 6;
   const bool row_ok = tid < NSH;
 
-  unsigned long long st_vox = 0, st_solves = 0, st_k = 0, st_conv = 0;
+  unsigned long long st_vox = 0, st_solves = 0, st_k = 0, st_conv = 0, st_fallback = 0, st_warm_solves = 0;
+  const bool warm = (warm_F0 != nullptr || warm_neg != nullptr) && P.niter > 0;
 
   for (;;) {
     unsigned long long vox = 0;
-    if (tid == 0) vox = atomicAdd(counter, 1ull);
-    vox = uniform64(vox);
+    if constexpr (NT == 64) {
+      if (tid == 0) vox = atomicAdd(counter, 1ull);
+      vox = uniform64(vox);
+    } else {
+      wave_sync();
+      if (tid == 0) s_vox = atomicAdd(counter, 1ull);
+      wave_sync();
+      vox = uniform64(s_vox);
+    }
     if (vox >= nvox) break;
 
     // ---- gather the selected DW signals (dwi2fod.cpp:156-173) ----
@@ -171,168 +279,226 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void cs
       if (!isfinite(v)) bad = true;
       s_b[r] = v < 0.0f ? 0.0f : v;
     }
-    if (__ballot(bad)) {
+    bool any_bad;
+    if constexpr (NT == 64) {
+      any_bad = __ballot(bad) != 0ull;
+    } else {
+      const unsigned long long bb = __ballot(bad);
+      if ((tid & 63) == 0) s_flag[wave] = bb != 0ull;
+      wave_sync();
+      any_bad = false;
+#pragma unroll
+      for (int w = 0; w < NWAVE; ++w) any_bad = any_bad || s_flag[w];
+    }
+    if (any_bad) {
       if (row_ok) fod[vox * NSH + tid] = 0.0f;
       if (tid == 0) {
         if (iters_out) iters_out[vox] = 0xFFFF;
         if (nneg_out) nneg_out[vox] = 0;
       }
-      if (negmask && tid < nq) negmask[vox * nq + tid] = 0ull;
+      if (negmask && tid < nwio) negmask[vox * nwio + tid] = 0ull;
       continue;
     }
     wave_sync();
 
     // ---- CSD::set: Mt_b and the initial F (csd.h:236-243) ----
-    double mtb = 0.0, f = 0.0;
-    if (row_ok) {
-      for (int r = 0; r < N; ++r) mtb = fma(P.M[r * NSH + tid], (double)s_b[r], mtb);
-      if (warm_F0) {
-        f = (double)warm_F0[vox * NSH + tid];
-      } else if (tid < P.n_init) {
-        for (int r = 0; r < N; ++r) f = fma(P.rconvT[r * P.n_init + tid], (double)s_b[r], f);
-      }
-    }
+    // Chunks of kPipe volumes: all loads of a chunk are issued before its FMAs (volumes past N
+    // are clamped reads with a zero weight, which leaves the sums unchanged).
+    double mtb = 0.0, f_plain = 0.0;
     {
-      double A0[NSH];
-      const double* mrow = P.MtM + (row_ok ? tid : 0) * NSH;
+      const int ni = P.n_init;
+      const int rt = row_ok ? tid : 0;
+      const int it = tid < ni ? tid : 0;
+      for (int r0 = 0; r0 < N; r0 += kPipe) {
+        double m[kPipe], g[kPipe], bv[kPipe];
 #pragma unroll
-      for (int j = 0; j < NSH; ++j) A0[j] = mrow[j];
-      store_lower_row<NSH>(s_W, A0);
+        for (int i = 0; i < kPipe; ++i) {
+          const int r = r0 + i < N ? r0 + i : N - 1;
+          m[i] = P.M[r * NSH + rt];
+          g[i] = P.rconvT[r * ni + it];
+          bv[i] = r0 + i < N ? (double)s_b[r] : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < kPipe; ++i) {
+          mtb = fma(m[i], bv[i], mtb);
+          f_plain = fma(g[i], bv[i], f_plain);
+        }
+      }
+      if (!row_ok) mtb = 0.0;
+      if (tid >= ni) f_plain = 0.0;
     }
 
     unsigned long long oldw[kMaxDirWords];
-#pragma unroll
-    for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = 0ull;
-    bool have_old = false;   // old_neg = [-1] (csd.h:240)
     int n = 0;
     int last_k = 0;
-
-    for (n = 0; n < P.niter; ++n) {
-      // ---- amplitudes and the negative set (csd.h:246-250) ----
-      s_F[tid] = f;
-      wave_sync();
+    bool converged = false;
+    double f = 0.0;
+    // pass 0: warm start (proposed F0 and/or negative set), at most warm_budget iterate() calls;
+    // pass 1: the plain initialisation with the full niter budget (the reference trajectory)
+    for (int pass = warm ? 0 : 1; pass < 2 && !converged; ++pass) {
+      const bool proposal = pass == 0 && warm_neg != nullptr;
+      const int limit = pass == 0 ? warm_budget : P.niter;
+      f = (pass == 0 && warm_F0 && row_ok) ? (double)warm_F0[vox * NSH + tid] : f_plain;
       {
-        double a[kMaxQ];
+        double A0[NSH];
+        const double* mrow = P.MtM + (row_ok ? tid : 0) * NSH;
 #pragma unroll
-        for (int q = 0; q < kMaxQ; ++q) a[q] = 0.0;
-        const double* hp = P.HRt + tid;
-#pragma unroll 3
-        for (int j = 0; j < NSH; ++j) {
-          const double fj = s_F[j];
+        for (int j = 0; j < NSH; ++j) A0[j] = mrow[j];
+        wave_sync();
+        store_lower_row<NSH>(s_W, s_F, A0);
+      }
 #pragma unroll
-          for (int q = 0; q < kMaxQ; ++q)
-            if (q < nq) a[q] = fma(hp[q * NT], fj, a[q]);
-          hp += Rpad;
-          __builtin_amdgcn_sched_barrier(0);
-        }
+      for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = 0ull;
+      bool have_old = false;   // old_neg = [-1] (csd.h:240)
+      last_k = 0;
+
+      for (n = 0; n < limit; ++n) {
+        unsigned long long neww[kMaxDirWords];
+        bool same = have_old;
+        int k = 0;
+        if (proposal && n == 0) {
+          // the proposed negative set replaces the first amplitude evaluation
 #pragma unroll
-        for (int q = 0; q < kMaxQ; ++q) {
-          if (q < nq) {
-            const int r = tid + q * NT;
-            const unsigned long long ball = __ballot(r < R && a[q] < P.thr);
-            if (tid == 0) s_words[q] = ball;
+          for (int w = 0; w < kMaxDirWords; ++w) {
+            const int left = R - 64 * w;   // only bits of existing directions are honoured
+            const unsigned long long valid_bits = left >= 64 ? ~0ull : (left > 0 ? (1ull << left) - 1ull : 0ull);
+            neww[w] = w < nwio ? (uniform64(warm_neg[vox * nwio + w]) & valid_bits) : 0ull;
+            k += __popcll(neww[w]);
           }
-        }
-      }
-      wave_sync();
-      bool same = have_old;
-      int k = 0;
-      unsigned long long neww[kMaxDirWords];
+          same = false;
+        } else {
+          // ---- amplitudes and the negative set (csd.h:246-250) ----
+          // One block of NT directions at a time (thread t: direction b*NT + t): NSH independent
+          // loads per block, accumulated in coefficient order, then one ballot per wavefront.
+          s_F[tid] = f;
+          wave_sync();
+          for (int b = 0; b < nq; ++b) {
+            const double* hb = P.HRt + (size_t)b * (NSH * NT) + tid;
+            double a = 0.0;
+            double hbuf[2][kPipe], fbuf[2][kPipe];
 #pragma unroll
-      for (int w = 0; w < kMaxDirWords; ++w) {
-        neww[w] = w < nq ? uniform64(s_words[w]) : 0ull;
-        k += __popcll(neww[w]);
-        same = same && (neww[w] == oldw[w]);
-      }
-      if (same) break;   // converged (csd.h:252-253)
-
-      // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
-      // Row i of the symmetric W is read from the packed lower triangle (element (i,j) sits at
-      // max(i,j)(max(i,j)+1)/2 + min(i,j)).  Directions that entered (+) or left (-) neg are
-      // applied in chunks of kChunk rows staged in LDS.
-      double A[NSH];
-      load_sym_row<NSH>(s_W, A);
-      {
-        int w = 0;
-        unsigned long long delta = have_old ? (neww[0] ^ oldw[0]) : neww[0];
-        for (;;) {
-          int cnt = 0;
-          int rows_c[kChunk];
-          double sgn_c[kChunk];
-#pragma unroll
-          for (int c = 0; c < kChunk; ++c) {
-            while (delta == 0ull && w + 1 < nq) {
-              ++w;
-              unsigned long long nw = 0ull, ow = 0ull;
-#pragma unroll
-              for (int q = 0; q < kMaxDirWords; ++q) {
-                nw = (q == w) ? neww[q] : nw;
-                ow = (q == w) ? oldw[q] : ow;
+            for (int i = 0; i < kPipe; ++i) {
+              if (i < NSH) {
+                hbuf[0][i] = hb[i * NT];
+                fbuf[0][i] = s_F[i];
               }
-              delta = have_old ? (nw ^ ow) : nw;
             }
-            rows_c[c] = 0;
-            sgn_c[c] = 0.0;
-            if (delta) {
-              const int bit = __ffsll((long long)delta) - 1;
-              delta &= delta - 1;
-              unsigned long long nw = 0ull;
 #pragma unroll
-              for (int q = 0; q < kMaxDirWords; ++q) nw = (q == w) ? neww[q] : nw;
-              rows_c[c] = w * 64 + bit;
-              sgn_c[c] = ((nw >> bit) & 1ull) ? 1.0 : -1.0;
-              ++cnt;
+            for (int kc = 0; kc * kPipe < NSH; ++kc) {
+              const int j = kc * kPipe;
+              if (j + kPipe < NSH) {
+#pragma unroll
+                for (int i = 0; i < kPipe; ++i) {
+                  if (j + kPipe + i < NSH) {
+                    hbuf[(kc + 1) & 1][i] = hb[(j + kPipe + i) * NT];
+                    fbuf[(kc + 1) & 1][i] = s_F[j + kPipe + i];
+                  }
+                }
+              }
+              __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+              for (int i = 0; i < kPipe; ++i)
+                if (j + i < NSH) a = fma(hbuf[kc & 1][i], fbuf[kc & 1][i], a);
+              __builtin_amdgcn_sched_barrier(0);
             }
-          }
-          if (cnt == 0) break;
-          double hown[kChunk];
-#pragma unroll
-          for (int c = 0; c < kChunk; ++c)
-            hown[c] = (c < cnt) ? P.HR[(size_t)rows_c[c] * NSH + (row_ok ? tid : 0)] : 0.0;
-          wave_sync();   // earlier readers of s_h are done
-          if (row_ok) {
-#pragma unroll
-            for (int c = 0; c < kChunk; ++c) s_h[c * NSH + tid] = hown[c];
+            const int r = tid + b * NT;
+            const unsigned long long ball = __ballot(r < R && a < P.thr);
+            if ((tid & 63) == 0) s_words[b * NWAVE + wave] = ball;
           }
           wave_sync();
 #pragma unroll
-          for (int c = 0; c < kChunk; ++c) {
-            if (c < cnt) {
-              const double hi = sgn_c[c] * hown[c];
+          for (int w = 0; w < kMaxDirWords; ++w) {
+            neww[w] = w < nwords ? uniform64(s_words[w]) : 0ull;
+            k += __popcll(neww[w]);
+            same = same && (neww[w] == oldw[w]);
+          }
+        }
+        if (same) { converged = true; break; }   // (csd.h:252-253)
+        // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
+        // Row i of the symmetric W is read from the packed lower triangle (element (i,j) sits at
+        // max(i,j)(max(i,j)+1)/2 + min(i,j)).  Directions that entered (+) or left (-) neg are
+        // applied in chunks of kChunk rows staged in LDS.
+        double A[NSH];
+        load_sym_row<NSH>(s_W, A);
+        {
+          int w = 0;
+          unsigned long long delta = have_old ? (neww[0] ^ oldw[0]) : neww[0];
+          for (;;) {
+            int cnt = 0;
+            int rows_c[kChunk];
+            double sgn_c[kChunk];
 #pragma unroll
-              for (int j = 0; j < NSH; ++j) {
-                A[j] = fma(hi, s_h[c * NSH + j], A[j]);
-                if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+            for (int c = 0; c < kChunk; ++c) {
+              while (delta == 0ull && w + 1 < nwords) {
+                ++w;
+                unsigned long long nw = 0ull, ow = 0ull;
+#pragma unroll
+                for (int q = 0; q < kMaxDirWords; ++q) {
+                  nw = (q == w) ? neww[q] : nw;
+                  ow = (q == w) ? oldw[q] : ow;
+                }
+                delta = have_old ? (nw ^ ow) : nw;
+              }
+              rows_c[c] = 0;
+              sgn_c[c] = 0.0;
+              if (delta) {
+                const int bit = __ffsll((long long)delta) - 1;
+                delta &= delta - 1;
+                unsigned long long nw = 0ull;
+#pragma unroll
+                for (int q = 0; q < kMaxDirWords; ++q) nw = (q == w) ? neww[q] : nw;
+                rows_c[c] = w * 64 + bit;
+                sgn_c[c] = ((nw >> bit) & 1ull) ? 1.0 : -1.0;
+                ++cnt;
               }
             }
+            if (cnt == 0) break;
+            double hown[kChunk];
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c)
+              hown[c] = (c < cnt) ? P.HR[(size_t)rows_c[c] * NSH + (row_ok ? tid : 0)] : 0.0;
+            wave_sync();   // earlier readers of s_h are done
+            if (row_ok) {
+#pragma unroll
+              for (int c = 0; c < kChunk; ++c) s_h[c * NSH + tid] = hown[c];
+            }
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c) hown[c] *= sgn_c[c];   // this row's signed weights
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < kChunk; ++c)
+              if (c < cnt) axpy_lds<NSH>(A, hown[c], s_h + c * NSH);
+            if (cnt < kChunk) break;
           }
-          if (cnt < kChunk) break;
         }
-      }
-      store_lower_row<NSH>(s_W, A);
+        store_lower_row<NSH>(s_W, s_F, A);
 #pragma unroll
-      for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = neww[w];
-      have_old = true;
-      last_k = k;
-      ++st_solves;
-      st_k += (unsigned long long)k;
+        for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = neww[w];
+        have_old = true;
+        last_k = k;
+        ++st_solves;
+        st_k += (unsigned long long)k;
 
-      // ---- Gauss-Jordan sweep on [W | Mt_b] (thread i owns row i) ----
-      // Step kk eliminates column kk from every other row.  Pivot row kk equals column kk past
-      // the diagonal, so each lane publishes its A[kk] as column kk.  The next column is
-      // updated first and published before the rest of the row (look-ahead), so its LDS
-      // round trip overlaps the remaining FMAs.
-      double* const col0 = s_h;
-      double* const col1 = s_h + kColStride;
-      double rhs = mtb;
-      wave_sync();   // s_h (update rows) is free again
-      col0[tid] = A[0];
-      wave_sync();
-      gj_sweep<NSH>(A, rhs, col0, col1, tid, std::make_integer_sequence<int, NSH>{});
-      double di = 1.0;
-#pragma unroll
-      for (int j = 0; j < NSH; ++j) di = (tid == j) ? A[j] : di;
-      f = row_ok ? rhs / di : 0.0;
+        // ---- Gauss-Jordan sweep on [W | Mt_b] (thread i owns row i) ----
+        // Step kk eliminates column kk from every other row.  Pivot row kk equals column kk past
+        // the diagonal, so each lane publishes its A[kk] as column kk.  The next column is
+        // updated first and published before the rest of the row (look-ahead), so its LDS
+        // round trip overlaps the remaining FMAs.
+        double* const col0 = s_h;
+        double* const col1 = s_h + kColStride;
+        double rhs = mtb;
+        wave_sync();   // s_h (update rows) is free again
+        col0[tid] = A[0];
+        if (NT != 64) col0[NT + tid] = rhs;
+        wave_sync();
+        double di = 1.0;
+        gj_sweep<NSH, NT>(A, rhs, di, col0, col1, tid, std::make_integer_sequence<int, NSH>{});
+        f = row_ok ? rhs / di : 0.0;
+        if (pass == 0) ++st_warm_solves;
+      }
+      if (!converged && pass == 0) ++st_fallback;
+      if (P.niter == 0 && pass == 1) converged = true;   // -niter 0: the linear deconvolution
     }
 
     if (row_ok) fod[vox * NSH + tid] = (float)f;
@@ -344,10 +510,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void cs
       unsigned long long mine = 0ull;
 #pragma unroll
       for (int w = 0; w < kMaxDirWords; ++w) mine = (tid == w) ? oldw[w] : mine;
-      if (tid < nq) negmask[vox * nq + tid] = mine;
+      if (tid < nwio) negmask[vox * nwio + tid] = mine;
     }
     ++st_vox;
-    if (n < P.niter) ++st_conv;
+    if (converged && P.niter) ++st_conv;
     wave_sync();
   }
 
@@ -356,6 +522,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void cs
     atomicAdd(stats + 1, st_solves);
     atomicAdd(stats + 2, st_k);
     atomicAdd(stats + 3, st_conv);
+    atomicAdd(stats + 4, st_fallback);
+    atomicAdd(stats + 5, st_warm_solves);
   }
 }
 
@@ -363,10 +531,9 @@ __global__ void reset_counter_kernel(unsigned long long* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
 }
 
-template <int NSH>
+template <int NSH, int NT>
 hipError_t launch_impl(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream) {
-  constexpr int NT = 64;
-  auto kern = csd_persistent_kernel<NSH>;
+  auto kern = csd_persistent_kernel<NSH, NT>;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0);
   if (e != hipSuccess) return e;
@@ -379,7 +546,8 @@ hipError_t launch_impl(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_
   if (grid > L.nvox) grid = L.nvox ? L.nvox : 1;
   reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
   kern<<<dim3((unsigned)grid), dim3(NT), 0, stream>>>(P, L.dw, L.ld_dw, L.valid, L.nvox, L.fod, L.iters,
-                                                     L.nneg, L.stats, L.warm_F0, L.negmask, L.counter);
+                                                     L.nneg, L.stats, L.warm_F0, L.warm_neg, L.warm_budget,
+                                                     L.negmask, L.counter);
   return hipGetLastError();
 }
 
@@ -387,18 +555,20 @@ hipError_t launch_impl(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_
 
 hipError_t csd_launch(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream) {
   switch (P.n_sh) {
-    case 6:  return launch_impl<6>(P, L, stream);
-    case 15: return launch_impl<15>(P, L, stream);
-    case 28: return launch_impl<28>(P, L, stream);
-    case 45: return launch_impl<45>(P, L, stream);
+    case 6:  return launch_impl<6, 64>(P, L, stream);
+    case 15: return launch_impl<15, 64>(P, L, stream);
+    case 28: return launch_impl<28, 64>(P, L, stream);
+    case 45: return launch_impl<45, 64>(P, L, stream);
+    case 66: return launch_impl<66, 128>(P, L, stream);
+    case 91: return launch_impl<91, 128>(P, L, stream);
     default: return hipErrorInvalidValue;
   }
 }
 
 bool csd_supported_nsh(int n_sh) {
-  return n_sh == 6 || n_sh == 15 || n_sh == 28 || n_sh == 45;
+  return n_sh == 6 || n_sh == 15 || n_sh == 28 || n_sh == 45 || n_sh == 66 || n_sh == 91;
 }
 
-int csd_threads_for(int) { return 64; }
+int csd_threads_for(int n_sh) { return n_sh <= 64 ? 64 : 128; }
 
 }  // namespace mrcsd

@@ -91,9 +91,9 @@ class IclsProblem:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.load().mricls_destroy(h)
-            self._h = None
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.mricls_destroy(h)
+        self._h = None
 
 
 class MSMTShared:
@@ -144,6 +144,6 @@ class MSMTShared:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            _lib.load().mrmsmt_shared_destroy(h)
-            self._h = None
+        if h and _lib is not None and _lib._lib is not None:
+            _lib._lib.mrmsmt_shared_destroy(h)
+        self._h = None

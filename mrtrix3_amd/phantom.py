@@ -172,9 +172,11 @@ class Phantom:
 
 
 def single_shell(name, shape, n_b0, n_dw, bval, seed, mask_semi=None, snr=30.0,
-                 snr_range=None, iso_fraction=0.0, min_angle=30.0, lmax_resp=8):
-    rng = np.random.default_rng(seed)
-    grad = gradient_table([(0, n_b0), (bval, n_dw)], rng)
+                 snr_range=None, iso_fraction=0.0, min_angle=30.0, lmax_resp=8, protocol_seed=None):
+    """protocol_seed fixes the gradient table independently of the voxel data (default: seed)."""
+    grad = gradient_table([(0, n_b0), (bval, n_dw)],
+                          np.random.default_rng(seed if protocol_seed is None else protocol_seed))
+    rng = np.random.default_rng(seed + 7919)
     mask = ellipsoid_mask(shape, mask_semi) if mask_semi is not None else None
     nvox = int(mask.sum()) if mask is not None else int(np.prod(shape))
     count, dirs, frac = random_sticks(nvox, rng, min_angle_deg=min_angle)
@@ -194,25 +196,32 @@ def single_shell(name, shape, n_b0, n_dw, bval, seed, mask_semi=None, snr=30.0,
     return Phantom(name, dw, grad, response, mask, shape, extra)
 
 
-def single_shell_small(seed=1):
+def single_shell_small(seed=1, protocol_seed=1):
     return single_shell("single_shell_small 32x32x16 5xb0+60xb3000 SNR30", (32, 32, 16), 5, 60,
-                        3000, seed, snr=30.0)
+                        3000, seed, snr=30.0, protocol_seed=protocol_seed)
 
 
 BRAIN_SHAPE = (128, 128, 72)
 BRAIN_SEMI = (63.5, 63.5, 38.5)
 
 
-def single_shell_brain(seed=2):
+def single_shell_brain(seed=2, protocol_seed=2):
     return single_shell("single_shell_brain 128x128x72 ellipsoid 8xb0+64xb3000 SNR15-40 15%iso",
                         BRAIN_SHAPE, 8, 64, 3000, seed, mask_semi=BRAIN_SEMI,
-                        snr_range=(15.0, 40.0), iso_fraction=0.15)
+                        snr_range=(15.0, 40.0), iso_fraction=0.15, protocol_seed=protocol_seed)
 
 
-def super_resolved(seed=3):
+def super_resolved(seed=3, protocol_seed=3):
     return single_shell("super_resolved 128x128x72 ellipsoid 5xb0+45xb3000 SNR20 lmax12",
                         BRAIN_SHAPE, 5, 45, 3000, seed, mask_semi=BRAIN_SEMI, snr=20.0,
-                        min_angle=25.0, lmax_resp=12)
+                        min_angle=25.0, lmax_resp=12, protocol_seed=protocol_seed)
+
+
+def super_resolved_small(seed=3, protocol_seed=3, shape=(16, 16, 8)):
+    """The super-resolved protocol (5xb0 + 45xb3000, lmax-12 response, SNR 20, crossings down to
+    25 degrees) on a small unmasked block, for parity tests against the CPU solver."""
+    return single_shell("super_resolved_small %dx%dx%d 5xb0+45xb3000 SNR20 lmax12" % shape, shape, 5, 45,
+                        3000, seed, snr=20.0, min_angle=25.0, lmax_resp=12, protocol_seed=protocol_seed)
 
 
 def multi_shell(seed=4, shape=BRAIN_SHAPE, semi=BRAIN_SEMI):

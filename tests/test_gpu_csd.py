@@ -96,6 +96,24 @@ def test_lower_lmax(torch_cuda, small_phantom, lmax):
     assert ok >= 0.999
 
 
+def test_lmax12_super_resolved(torch_cuda):
+    """91 coefficients from 45 DW volumes (under-determined: l = 10, 12 are set by the constraint
+    and the norm penalty), two-wavefront workgroups."""
+    torch = torch_cuda
+    ph = phantom.super_resolved_small()
+    say("[gpu csd] super_resolved_small lmax12: %d voxels" % ph.nvox)
+    o = orc.CsdOracle(ph.grad, ph.response, lmax=12)
+    solver = csd.CSD(csd.CSDShared(ph.grad, ph.response, lmax=12))
+    assert solver.n_sh == 91
+    fod, it, nn, nm = solver.run(torch.from_numpy(ph.dw).cuda(), negmask=True)
+    torch.cuda.synchronize()
+    F_ref, n_ref, neg_ref = oracle_solve_all(o, ph.dw, range(ph.nvox))
+    _, _, ok = parity_report(fod.cpu().numpy().astype(np.float64), F_ref,
+                             neg_lists(nm.cpu().numpy(), o.n_hr), neg_ref, "lmax12")
+    assert ok >= 0.999
+    assert np.mean(it.cpu().numpy().astype(np.int64) == n_ref) >= 0.999
+
+
 @pytest.mark.parametrize("niter", [0, 1, 3])
 def test_iteration_caps(torch_cuda, small_phantom, niter):
     torch = torch_cuda
@@ -161,12 +179,12 @@ def test_brain_volume_properties(torch_cuda):
     o = orc.CsdOracle(ph.grad, ph.response)
     solver = csd.CSD(csd.CSDShared(ph.grad, ph.response))
     say("[gpu csd] single_shell_brain: solving %d voxels" % ph.nvox)
-    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(6, dtype=torch.int64, device="cuda")
     fod, it, nn, nm = solver.run(torch.from_numpy(ph.dw).cuda(), stats=stats, negmask=True)
     torch.cuda.synchronize()
     st = stats.cpu().numpy()
     it = it.cpu().numpy().astype(np.int64)
-    say("[brain] stats voxels=%d solves=%d sum_k=%d converged=%d" % tuple(st))
+    say("[brain] stats voxels=%d solves=%d sum_k=%d converged=%d" % tuple(st[:4]))
     assert st[0] == ph.nvox
     assert np.isfinite(fod.cpu().numpy()).all()
     assert np.mean(it < 50) >= 0.999
