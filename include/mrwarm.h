@@ -57,6 +57,14 @@ int mrwarm_gradients(mrwarm_net* net, const void* x_bf16, const float* tgt_coef,
 int mrwarm_propose(const float* out, const float* s0, size_t M, int nsh, int nlog, float* F0,
                    uint64_t* negmask, void* stream);
 
+/* Features, forward pass and proposal fused in one kernel (activations stay in on-chip memory):
+ * the same results as mrwarm_features + mrwarm_forward + mrwarm_propose, without the [M][hidden]
+ * intermediates.  dw: [M][ld_dw] DW rows (device), dwis_dev [n_in] and b0_dev [n_b0] volume indices
+ * (device); F0 [M][nsh] (may be NULL), negmask [M][(nlog + 63) / 64] (device).  Needs hidden = 256,
+ * n_in <= 128 and nsh + nlog <= 512 (MRCSD_ERR_ARG otherwise). */
+int mrwarm_infer(mrwarm_net* net, const float* dw, int ld_dw, size_t M, const int* dwis_dev, const int* b0_dev,
+                 int n_b0, float* F0, uint64_t* negmask, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

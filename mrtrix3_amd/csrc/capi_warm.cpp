@@ -19,6 +19,8 @@ struct Layer {
   int fin = 0, fout = 0;
   mrcsd::DeviceBuffer<float> W, b, gW, gb, mW, vW, mb, vb;   // fp32 master, grads, Adam moments
   mrcsd::DeviceBuffer<__bf16> Wb;                          // bf16 copy used by the GEMMs
+  mrcsd::DeviceBuffer<__bf16> WT;                          // [npad][kpad] transposed copy (fused inference)
+  int kpad = 0, npad = 0;
 };
 
 }  // namespace
@@ -249,6 +251,42 @@ int mrwarm_gradients(mrwarm_net* net, const void* x_bf16, const float* tgt_coef,
       o += size_t(L.fout);
     }
     if (loss_out) net->loss.download(loss_out, 2);
+    return MRCSD_OK;
+  });
+}
+
+int mrwarm_infer(mrwarm_net* net, const float* dw, int ld_dw, size_t M, const int* dwis_dev, const int* b0_dev,
+                 int n_b0, float* F0, uint64_t* negmask, void* stream) {
+  return mrcsd::guarded([&]() -> int {
+    if (!net || !dw || !dwis_dev || !negmask || (n_b0 > 0 && !b0_dev))
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrwarm_infer: null argument");
+    if (net->hidden != mrcsd::kInferHidden || net->n_in > mrcsd::kInferMaxIn || net->nsh + net->nlog > 512)
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrwarm_infer: the fused path needs hidden = " +
+                                                    std::to_string(mrcsd::kInferHidden) + ", n_in <= " +
+                                                    std::to_string(mrcsd::kInferMaxIn) + ", nsh + nlog <= 512");
+    if (ld_dw < 1) return mrcsd::fail(MRCSD_ERR_ARG, "mrwarm_infer: ld_dw must be positive");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    mrcsd::InferArgs a{};
+    for (int l = 0; l < 4; ++l) {
+      Layer& L = net->layers[l];
+      const int kpad = (L.fin + 31) / 32 * 32, npad = (L.fout + 15) / 16 * 16;
+      if (L.kpad != kpad || L.npad != npad) {
+        L.WT.alloc(size_t(kpad) * npad);
+        L.kpad = kpad;
+        L.npad = npad;
+      }
+      MRCSD_HIP(mrcsd::transpose_pad(L.Wb.get(), L.fin, L.fout, L.WT.get(), kpad, npad, s));
+      a.WT[l] = L.WT.get();
+      a.ldw[l] = kpad;
+      a.bias[l] = L.b.get();
+    }
+    a.dw = dw; a.ld = ld_dw; a.M = M;
+    a.dwis = dwis_dev; a.n_in = net->n_in; a.k0 = net->layers[0].kpad;
+    a.b0 = b0_dev; a.n_b0 = n_b0;
+    a.nsh = net->nsh; a.nlog = net->nlog;
+    a.F0 = F0;
+    a.negmask = reinterpret_cast<unsigned long long*>(negmask);
+    MRCSD_HIP(mrcsd::infer_propose(a, s));
     return MRCSD_OK;
   });
 }

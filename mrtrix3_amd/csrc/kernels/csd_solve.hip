@@ -33,7 +33,6 @@ namespace {
 
 constexpr int kMaxDirWords = 8;   // up to 512 constraint directions
 constexpr int kMaxDW = 256;       // up to 256 DW volumes per voxel
-constexpr int kChunk = 8;         // normal-matrix update rows staged per LDS hand-off
 // doubles per pivot-column buffer: NT column entries, then (workgroups wider than one wavefront)
 // the NT right-hand sides, so that the pivot's right-hand side is read from LDS.
 template <int NT>
@@ -126,30 +125,118 @@ __device__ __forceinline__ void store_lower_row(double* W, double* junk, const d
   for (int j = 0; j < NSH; ++j) *((j <= tt) ? lo + j : jk) = A[j];
 }
 
-// A[j] = fma(s, v[j], A[j]) for j in [0, NSH), v in LDS.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// The normal matrix as 16x16 tiles (n rounded up to whole tiles).  Lower tiles are numbered
+// row-major over the lower triangle, g = I(I+1)/2 + J with J <= I; a workgroup of NWAVE
+// wavefronts gives tile g to wavefront g % NWAVE.
+constexpr int tile_row(int g) {
+  int I = 0;
+  while ((I + 1) * (I + 2) / 2 <= g) ++I;
+  return I;
+}
+constexpr int tile_col(int g) { return g - tile_row(g) * (tile_row(g) + 1) / 2; }
+
+template <int NSH, int NWAVE, int WAVE>
+struct WaveTiles {
+  static constexpr int NTL = (NSH + 15) / 16;
+  static constexpr int NLOW = NTL * (NTL + 1) / 2;
+  static constexpr int MYT = (NLOW - WAVE + NWAVE - 1) / NWAVE;
+  static constexpr int g(int t) { return WAVE + t * NWAVE; }
+};
+
+// Element (row, col) of tile (I, J) held by lane l in register r of an FP64 16x16x4 MFMA
+// accumulator: col = 16J + l % 16, row = 16I + l / 16 + 4r.  Its index in the packed lower
+// triangle (element (a, b), b <= a, at a(a+1)/2 + b) mirrors it into the lower part on diagonal
+// tiles; `ok` is false for padding and, when storing, for the upper half of diagonal tiles.
 template <int NSH>
-__device__ __forceinline__ void axpy_lds(double (&A)[NSH], double s, const double* v) {
-  double buf[2][kPipe];
-  {
-    const double* p = v + opaque_index(0);
+__device__ __forceinline__ int packed_index(int I, int J, int l, int r, bool store, bool& ok) {
+  const int row = 16 * I + (l >> 4) + 4 * r;
+  const int col = 16 * J + (l & 15);
+  const int a = row > col ? row : col;
+  const int b = row > col ? col : row;
+  ok = row < NSH && col < NSH && (!store || I != J || row >= col);
+  return a * (a + 1) / 2 + b;
+}
+
+// W += sum over the chunk's directions d of s_d h_d h_d^T on this wavefront's lower tiles, where
+// h_d is row d of HR (the constraint direction's SH values) and s_d = +1 (entered the negative
+// set) or -1 (left it).  One v_mfma_f64_16x16x4f64 per tile per four directions: the A operand
+// (lane i + 16c holds s_c h_c[16I + i]) and the B operand (lane j + 16c holds h_c[16J + j]) come
+// from the same loads.  The next chunk's loads are issued before the current chunk's MFMAs.
+// take4(r0..r3, s0..s3) yields the next up-to-four directions (HR rows and signs, unused slots row
+// 0 and sign 0; returns the count) and is evaluated identically by every wavefront of the
+// workgroup.
+template <int NSH, int NWAVE, int WAVE, class Take>
+__device__ __forceinline__ void mfma_update(double* W, double* junk, const double* __restrict__ HR, Take& take4) {
+  using G = WaveTiles<NSH, NWAVE, WAVE>;
+  constexpr int NTL = G::NTL;
+  const int l = opaque_lane() & 63;
+  v4d acc[G::MYT];
 #pragma unroll
-    for (int i = 0; i < kPipe; ++i)
-      if (i < NSH) buf[0][i] = p[i];
+  for (int t = 0; t < G::MYT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      bool ok;
+      const int idx = packed_index<NSH>(tile_row(G::g(t)), tile_col(G::g(t)), l, r, false, ok);
+      const double x = W[ok ? idx : 0];
+      acc[t][r] = ok ? x : 0.0;
+    }
   }
+  auto fetch = [&](int r0, int r1, int r2, int r3, double s0, double s1, double s2, double s3, double (&h)[NTL],
+                   double& s) __attribute__((always_inline)) {
+    const int c = l >> 4;
+    const int row = c == 0 ? r0 : (c == 1 ? r1 : (c == 2 ? r2 : r3));
+    s = c == 0 ? s0 : (c == 1 ? s1 : (c == 2 ? s2 : s3));
+    const double* src = HR + (size_t)row * NSH + (l & 15);
 #pragma unroll
-  for (int k = 0; k * kPipe < NSH; ++k) {
-    const int j = k * kPipe;
-    if (j + kPipe < NSH) {
-      const double* p = v + opaque_index(j + kPipe);
+    for (int T = 0; T < NTL; ++T) {
+      const bool in = 16 * T + (l & 15) < NSH;
+      const double x = src[in ? 16 * T : 0];
+      h[T] = in ? x : 0.0;
+    }
+  };
+  double h[NTL], s = 0.0;
+  int cnt;
+  {
+    int r0, r1, r2, r3;
+    double s0, s1, s2, s3;
+    cnt = take4(r0, r1, r2, r3, s0, s1, s2, s3);
+    fetch(r0, r1, r2, r3, s0, s1, s2, s3, h, s);
+  }
+  while (cnt > 0) {
+    double h2[NTL], s2 = 0.0;
+    int cnt2 = 0;
+    if (cnt == 4) {
+      int r0, r1, r2, r3;
+      double q0, q1, q2, q3;
+      cnt2 = take4(r0, r1, r2, r3, q0, q1, q2, q3);
+      fetch(r0, r1, r2, r3, q0, q1, q2, q3, h2, s2);
+    } else {
 #pragma unroll
-      for (int i = 0; i < kPipe; ++i)
-        if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
+      for (int T = 0; T < NTL; ++T) h2[T] = 0.0;
     }
     __builtin_amdgcn_sched_barrier(0);
+    double a[NTL];
 #pragma unroll
-    for (int i = 0; i < kPipe; ++i)
-      if (j + i < NSH) A[j + i] = fma(s, buf[k & 1][i], A[j + i]);
+    for (int T = 0; T < NTL; ++T) a[T] = s * h[T];
+#pragma unroll
+    for (int t = 0; t < G::MYT; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tile_row(G::g(t))], h[tile_col(G::g(t))], acc[t], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int T = 0; T < NTL; ++T) h[T] = h2[T];
+    s = s2;
+    cnt = cnt2;
+  }
+#pragma unroll
+  for (int t = 0; t < G::MYT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      bool ok;
+      const int idx = packed_index<NSH>(tile_row(G::g(t)), tile_col(G::g(t)), l, r, true, ok);
+      *(ok ? W + idx : junk + l) = acc[t][r];
+    }
   }
 }
 
@@ -236,10 +323,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
   constexpr int NTRI = NSH * (NSH + 1) / 2;
   constexpr int kColStride = col_stride<NT>();
   constexpr int COLS = 2 * kColStride;                       // two pivot-column buffers
-  constexpr int HBUF = kChunk * NSH > COLS ? kChunk * NSH : COLS;
+  constexpr int HBUF = COLS;
   __shared__ float s_b[kMaxDW];
   __shared__ double s_F[NT];
-  __shared__ __attribute__((aligned(16))) double s_h[HBUF];  // update rows, then pivot columns
+  __shared__ __attribute__((aligned(16))) double s_h[HBUF];  // pivot columns of the sweep
   __shared__ double s_W[NTRI];        // packed lower triangle of the normal matrix
   __shared__ unsigned long long s_words[kMaxDirWords];
   __shared__ unsigned long long s_vox;
@@ -416,63 +503,53 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
         }
         if (same) { converged = true; break; }   // (csd.h:252-253)
         // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
-        // Row i of the symmetric W is read from the packed lower triangle (element (i,j) sits at
-        // max(i,j)(max(i,j)+1)/2 + min(i,j)).  Directions that entered (+) or left (-) neg are
-        // applied in chunks of kChunk rows staged in LDS.
-        double A[NSH];
-        load_sym_row<NSH>(s_W, A);
+        // Directions that entered (+) or left (-) neg since the last solve are applied to the
+        // packed lower triangle in LDS as a signed rank-4 update per MFMA (mfma_update).
         {
           int w = 0;
           unsigned long long delta = have_old ? (neww[0] ^ oldw[0]) : neww[0];
-          for (;;) {
+          // next direction of the change (entered: +1, left: -1); 0 when none is left
+          auto take1 = [&](int& cnt, double& sgn) __attribute__((always_inline)) -> int {
+            while (delta == 0ull && w + 1 < nwords) {
+              ++w;
+              unsigned long long nw = 0ull, ow = 0ull;
+#pragma unroll
+              for (int q = 0; q < kMaxDirWords; ++q) {
+                nw = (q == w) ? neww[q] : nw;
+                ow = (q == w) ? oldw[q] : ow;
+              }
+              delta = have_old ? (nw ^ ow) : nw;
+            }
+            sgn = 0.0;
+            if (delta == 0ull) return 0;
+            const int bit = __ffsll((long long)delta) - 1;
+            delta &= delta - 1;
+            unsigned long long nw = 0ull;
+#pragma unroll
+            for (int q = 0; q < kMaxDirWords; ++q) nw = (q == w) ? neww[q] : nw;
+            sgn = ((nw >> bit) & 1ull) ? 1.0 : -1.0;
+            ++cnt;
+            return w * 64 + bit;
+          };
+          auto take4 = [&](int& r0, int& r1, int& r2, int& r3, double& s0, double& s1, double& s2,
+                           double& s3) __attribute__((always_inline)) -> int {
             int cnt = 0;
-            int rows_c[kChunk];
-            double sgn_c[kChunk];
-#pragma unroll
-            for (int c = 0; c < kChunk; ++c) {
-              while (delta == 0ull && w + 1 < nwords) {
-                ++w;
-                unsigned long long nw = 0ull, ow = 0ull;
-#pragma unroll
-                for (int q = 0; q < kMaxDirWords; ++q) {
-                  nw = (q == w) ? neww[q] : nw;
-                  ow = (q == w) ? oldw[q] : ow;
-                }
-                delta = have_old ? (nw ^ ow) : nw;
-              }
-              rows_c[c] = 0;
-              sgn_c[c] = 0.0;
-              if (delta) {
-                const int bit = __ffsll((long long)delta) - 1;
-                delta &= delta - 1;
-                unsigned long long nw = 0ull;
-#pragma unroll
-                for (int q = 0; q < kMaxDirWords; ++q) nw = (q == w) ? neww[q] : nw;
-                rows_c[c] = w * 64 + bit;
-                sgn_c[c] = ((nw >> bit) & 1ull) ? 1.0 : -1.0;
-                ++cnt;
-              }
-            }
-            if (cnt == 0) break;
-            double hown[kChunk];
-#pragma unroll
-            for (int c = 0; c < kChunk; ++c)
-              hown[c] = (c < cnt) ? P.HR[(size_t)rows_c[c] * NSH + (row_ok ? tid : 0)] : 0.0;
-            wave_sync();   // earlier readers of s_h are done
-            if (row_ok) {
-#pragma unroll
-              for (int c = 0; c < kChunk; ++c) s_h[c * NSH + tid] = hown[c];
-            }
-#pragma unroll
-            for (int c = 0; c < kChunk; ++c) hown[c] *= sgn_c[c];   // this row's signed weights
-            wave_sync();
-#pragma unroll
-            for (int c = 0; c < kChunk; ++c)
-              if (c < cnt) axpy_lds<NSH>(A, hown[c], s_h + c * NSH);
-            if (cnt < kChunk) break;
+            r0 = take1(cnt, s0);
+            r1 = take1(cnt, s1);
+            r2 = take1(cnt, s2);
+            r3 = take1(cnt, s3);
+            return cnt;
+          };
+          if constexpr (NWAVE == 1) {
+            mfma_update<NSH, 1, 0>(s_W, s_F, P.HR, take4);
+          } else {
+            if (wave == 0) mfma_update<NSH, NWAVE, 0>(s_W, s_F, P.HR, take4);
+            else mfma_update<NSH, NWAVE, 1>(s_W, s_F, P.HR, take4);
           }
         }
-        store_lower_row<NSH>(s_W, s_F, A);
+        wave_sync();
+        double A[NSH];
+        load_sym_row<NSH>(s_W, A);
 #pragma unroll
         for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = neww[w];
         have_old = true;
@@ -488,7 +565,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
         double* const col0 = s_h;
         double* const col1 = s_h + kColStride;
         double rhs = mtb;
-        wave_sync();   // s_h (update rows) is free again
+        wave_sync();   // earlier readers of the pivot columns are done
         col0[tid] = A[0];
         if (NT != 64) col0[NT + tid] = rhs;
         wave_sync();

@@ -25,6 +25,28 @@ struct GemmArgs {
 };
 
 hipError_t gemm_bf16(const GemmArgs& g, hipStream_t stream);
+
+// Fused inference (features, the four layers, proposal) for networks with kInferHidden hidden units
+// and at most kInferMaxIn inputs (padded to a multiple of 32).
+constexpr int kInferHidden = 256;
+constexpr int kInferMaxIn = 128;
+struct InferArgs {
+  const float* dw;            // [M][ld] DW rows
+  int ld;
+  size_t M;
+  const int* dwis;            // [n_in] input volumes
+  int n_in, k0;               // k0 = n_in rounded up to 32
+  const int* b0;              // [n_b0] b=0 volumes
+  int n_b0;
+  const __bf16* WT[4];        // transposed weights [npad][kpad], layer l row pitch ldw[l]
+  int ldw[4];
+  const float* bias[4];
+  int nsh, nlog;
+  float* F0;                  // [M][nsh] or null
+  unsigned long long* negmask;   // [M][(nlog + 63) / 64]
+};
+hipError_t infer_propose(const InferArgs& a, hipStream_t stream);
+hipError_t transpose_pad(const __bf16* W, int fin, int fout, __bf16* WT, int kpad, int npad, hipStream_t stream);
 hipError_t gelu_backward(const float* pre, const float* dact, __bf16* dpre_bf, float* dpre, size_t n,
                          hipStream_t stream);
 hipError_t to_bf16(const float* x, __bf16* y, size_t n, hipStream_t stream);

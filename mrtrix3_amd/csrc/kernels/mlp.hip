@@ -240,7 +240,177 @@ __global__ void propose_kernel(const float* __restrict__ out, const float* __res
   }
 }
 
+// ---- fused inference: features -> 4 layers -> proposal, one 64-voxel tile per workgroup ----
+// Activations stay in LDS (bf16, two ping-pong buffers); weights are read from L2 as the B operand
+// of v_mfma_f32_16x16x32_bf16 from a transposed [fan_out][fan_in] copy (16-byte loads of eight
+// consecutive k), prefetched one k-step ahead.  The arithmetic is the unfused path's: the same MFMA
+// over k in steps of 32, bias, GELU and rounding to bf16, so both produce identical proposals.
+constexpr int FI_TM = 64;               // voxels per tile
+constexpr int FI_PH = kInferHidden + 8; // LDS row pitch (bf16) of hidden activations
+constexpr int FI_PX = kInferMaxIn + 8;  // LDS row pitch (bf16) of network inputs
+
+// acc = in[64 x K] * W[K x (n0 + 64 wave .. + 63)] for the column blocks that start below N.
+__device__ __forceinline__ void fi_chunk(const __bf16* sIn, int pin, int K, const __bf16* __restrict__ WT,
+                                         int ldw, int N, int n0, int wave, int lane, f32x4 (&acc)[4][4]) {
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  const int nb = n0 + 64 * wave;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero = {};
+  bf16x8 b[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    b[j] = (nb + 16 * j < N) ? *reinterpret_cast<const bf16x8*>(WT + (size_t)(nb + 16 * j + r16) * ldw + kq) : zero;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16x8 bn[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      bn[j] = (k0 + 32 < K && nb + 16 * j < N)
+                  ? *reinterpret_cast<const bf16x8*>(WT + (size_t)(nb + 16 * j + r16) * ldw + k0 + 32 + kq)
+                  : zero;
+    bf16x8 a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(sIn + (16 * i + r16) * pin + k0 + kq);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (nb + 16 * j < N)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = bn[j];
+  }
+}
+
+// hidden layer epilogue: out = bf16(gelu(acc + bias)) into the LDS activation buffer
+__device__ __forceinline__ void fi_store_hidden(const f32x4 (&acc)[4][4], const float* __restrict__ bias, __bf16* sOut,
+                                                int n0, int wave, int lane) {
+  const int cn = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = n0 + 64 * wave + 16 * j + cn;
+    const float bn = bias[n];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sOut[(16 * i + rq + r) * FI_PH + n] = (__bf16)gelu_f(acc[i][j][r] + bn);
+  }
+}
+
+__global__ __launch_bounds__(256) void infer_propose_kernel(InferArgs a) {
+  static_assert(FI_PX <= FI_PH, "the input tile lives in the second activation buffer");
+  __shared__ __attribute__((aligned(16))) __bf16 sHa[FI_TM * FI_PH];
+  __shared__ __attribute__((aligned(16))) __bf16 sHb[FI_TM * FI_PH];
+  __shared__ float sS0[FI_TM];
+  __bf16* const sX = sHb;   // network inputs; dead once the first layer has read them
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const size_t m0 = (size_t)blockIdx.x * FI_TM;
+
+  // features: x = dw[dwis] / s0, s0 = mean of the b=0 volumes (1 when not positive)
+  if (tid < FI_TM) {
+    const size_t m = m0 + tid;
+    float s0 = 1.f;
+    if (m < a.M) {
+      const float* row = a.dw + m * (size_t)a.ld;
+      float s = 0.f;
+      for (int j = 0; j < a.n_b0; ++j) s += row[a.b0[j]];
+      s0 = a.n_b0 ? s / (float)a.n_b0 : 1.f;
+      if (!(s0 > 0.f) || !isfinite(s0)) s0 = 1.f;
+    }
+    sS0[tid] = s0;
+  }
+  __syncthreads();
+  for (int e = tid; e < FI_TM * a.k0; e += 256) {
+    const int row = e / a.k0, i = e % a.k0;
+    const size_t m = m0 + row;
+    float v = 0.f;
+    if (m < a.M && i < a.n_in) {
+      v = a.dw[m * (size_t)a.ld + a.dwis[i]] * (1.f / sS0[row]);
+      if (!isfinite(v)) v = 0.f;
+    }
+    sX[row * FI_PX + i] = (__bf16)v;
+  }
+  __syncthreads();
+
+  f32x4 acc[4][4];
+  fi_chunk(sX, FI_PX, a.k0, a.WT[0], a.ldw[0], kInferHidden, 0, wave, lane, acc);
+  fi_store_hidden(acc, a.bias[0], sHa, 0, wave, lane);
+  __syncthreads();
+  fi_chunk(sHa, FI_PH, kInferHidden, a.WT[1], a.ldw[1], kInferHidden, 0, wave, lane, acc);
+  fi_store_hidden(acc, a.bias[1], sHb, 0, wave, lane);
+  __syncthreads();
+  fi_chunk(sHb, FI_PH, kInferHidden, a.WT[2], a.ldw[2], kInferHidden, 0, wave, lane, acc);
+  fi_store_hidden(acc, a.bias[2], sHa, 0, wave, lane);
+  __syncthreads();
+
+  // output layer: coefficients (n < nsh) -> F0 = out * s0; logits -> one flag byte per direction in
+  // sHb (free now), packed into 64-bit words below
+  unsigned char* flags = reinterpret_cast<unsigned char*>(sHb);
+  const int nout = a.nsh + a.nlog;
+  const int fpitch = (a.nlog + 63) / 64 * 64;
+  const int cn = lane & 15, rq = (lane >> 4) * 4;
+  for (int n0 = 0; n0 < nout; n0 += 4 * 64) {
+    fi_chunk(sHa, FI_PH, kInferHidden, a.WT[3], a.ldw[3], nout, n0, wave, lane, acc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + 64 * wave + 16 * j + cn;
+      if (n >= nout) continue;
+      const float bn = a.bias[3][n];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * i + rq + r;
+          const float v = acc[i][j][r] + bn;
+          if (n < a.nsh) {
+            const size_t m = m0 + row;
+            if (a.F0 && m < a.M) a.F0[m * (size_t)a.nsh + n] = v * sS0[row];
+          } else {
+            flags[row * fpitch + (n - a.nsh)] = v > 0.f ? 1 : 0;
+          }
+        }
+    }
+  }
+  __syncthreads();
+  const int words = (a.nlog + 63) / 64;
+  for (int e = tid; e < FI_TM * words; e += 256) {
+    const int row = e / words, w = e % words;
+    const size_t m = m0 + row;
+    if (m >= a.M) continue;
+    unsigned long long bits = 0ull;
+    for (int b = 0; b < 64; ++b) {
+      const int r = 64 * w + b;
+      if (r < a.nlog && flags[row * fpitch + r]) bits |= 1ull << b;
+    }
+    a.negmask[m * (size_t)words + w] = bits;
+  }
+}
+
+// WT[n][k] = W[k][n] (bf16), zero padded to [npad][kpad]
+__global__ void transpose_pad_kernel(const __bf16* __restrict__ W, int fin, int fout, __bf16* __restrict__ WT,
+                                     int kpad, int npad) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)kpad * npad) return;
+  const int n = (int)(i / kpad), k = (int)(i % kpad);
+  WT[i] = (n < fout && k < fin) ? W[(size_t)k * fout + n] : (__bf16)0.0f;
+}
+
 }  // namespace
+
+hipError_t transpose_pad(const __bf16* W, int fin, int fout, __bf16* WT, int kpad, int npad, hipStream_t stream) {
+  const size_t n = (size_t)kpad * npad;
+  if (!n) return hipSuccess;
+  transpose_pad_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(W, fin, fout, WT, kpad, npad);
+  return hipGetLastError();
+}
+
+hipError_t infer_propose(const InferArgs& a, hipStream_t stream) {
+  if (!a.M) return hipSuccess;
+  if (a.k0 % 32 || a.k0 > kInferMaxIn || a.nsh + a.nlog > 512 || a.nlog > 512) return hipErrorInvalidValue;
+  infer_propose_kernel<<<(unsigned)((a.M + FI_TM - 1) / FI_TM), 256, 0, stream>>>(a);
+  return hipGetLastError();
+}
 
 hipError_t features(const float* dw, int ld, size_t M, const int* dwis, int n_in, const int* b0, int n_b0,
                     __bf16* x, float* s0, hipStream_t stream) {

@@ -37,6 +37,7 @@ def _register():
                                                 _P, _P])
     _lib.register("mrwarm_gradients", C.c_int, [_P, _P, _P, _P, C.c_size_t, C.c_float, C.c_float, _P, _P])
     _lib.register("mrwarm_propose", C.c_int, [_P, _P, C.c_size_t, C.c_int, C.c_int, _P, _P, _P])
+    _lib.register("mrwarm_infer", C.c_int, [_P, _P, C.c_int, C.c_size_t, _P, _P, C.c_int, _P, _P, _P])
     _registered = True
 
 
@@ -111,6 +112,21 @@ class WarmNet:
                                          neg.data_ptr(), _stream(out)))
         return F0, neg
 
+    def fused_ok(self):
+        """Dimensions the fused inference kernel handles (hidden 256, <= 128 inputs, <= 512 outputs)."""
+        return self.hidden == 256 and self.n_in <= 128 and self.nout <= 512
+
+    def infer(self, dw, dwis, b0, want_F0=False):
+        """Features + forward + proposal in one kernel (mrwarm_infer): (F0 or None, neg words)."""
+        import torch
+        M = dw.shape[0]
+        F0 = torch.empty((M, self.nsh), dtype=torch.float32, device=dw.device) if want_F0 else None
+        neg = torch.empty((M, (self.nlog + 63) // 64), dtype=torch.int64, device=dw.device)
+        check(_lib.load().mrwarm_infer(self._h, dw.data_ptr(), dw.stride(0), M, dwis.data_ptr(), b0.data_ptr(),
+                                       b0.shape[0], F0.data_ptr() if want_F0 else None, neg.data_ptr(),
+                                       _stream(dw)))
+        return F0, neg
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and _lib is not None and _lib._lib is not None:
@@ -163,10 +179,16 @@ def train_warm_start(net, solver, dw_train, dwis, b0, epochs=12, batch=4096, lr=
     return history
 
 
-def warm_solve(net, solver, dw, dwis, b0, valid=None, budget=6, stats=None, fod=None, negmask=False):
-    """Features -> network -> proposal -> warm-started CSD (fallback to the plain start)."""
-    x, s0 = net.features(dw, dwis, b0)
-    out = net.forward(x)
-    F0, neg = net.propose(out, s0)
+def warm_solve(net, solver, dw, dwis, b0, valid=None, budget=6, stats=None, fod=None, negmask=False, fused=True):
+    """Network proposal -> warm-started CSD (fallback to the plain start).  The solver starts from the
+    proposed negative set, so the proposed coefficients are not needed; the fused kernel computes the
+    set directly from the DW rows."""
+    if fused and net.fused_ok():
+        _, neg = net.infer(dw, dwis, b0)
+        F0 = None
+    else:
+        x, s0 = net.features(dw, dwis, b0)
+        out = net.forward(x)
+        F0, neg = net.propose(out, s0)
     return solver.run(dw, valid=valid, warm_F0=F0, warm_neg=neg, warm_budget=budget, stats=stats, fod=fod,
                       negmask=negmask)

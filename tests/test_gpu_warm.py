@@ -97,6 +97,27 @@ def test_training_reduces_loss(trained):
     assert total[-1] < 0.8 * total[0] and h[-1][1] < 0.9 * h[0][1]
 
 
+def test_fused_inference_matches_unfused(torch_cuda, trained):
+    """mrwarm_infer (features, layers and proposal in one kernel) against mrwarm_features +
+    mrwarm_forward + mrwarm_propose: the same MFMA sequence and rounding, so the same proposal."""
+    torch = torch_cuda
+    net = trained["net"]
+    ph = phantom.single_shell_small(seed=98)
+    M = 5000                                   # not a multiple of the 64-voxel tile
+    dw = torch.from_numpy(np.ascontiguousarray(ph.dw[:M])).cuda()
+    x, s0 = net.features(dw, trained["dwis"], trained["b0"])
+    F0_ref, neg_ref = net.propose(net.forward(x), s0)
+    F0, neg = net.infer(dw, trained["dwis"], trained["b0"], want_F0=True)
+    torch.cuda.synchronize()
+    same = (neg == neg_ref).all(dim=1).float().mean().item()
+    ferr = ((F0 - F0_ref).abs().max() / F0_ref.abs().max()).item()
+    say("[gpu warm] fused inference: identical proposals %.6f, F0 max rel diff %.3e" % (same, ferr))
+    assert same >= 0.9999 and ferr <= 1e-5
+    _, neg2 = net.infer(dw, trained["dwis"], trained["b0"])   # without F0
+    torch.cuda.synchronize()
+    assert torch.equal(neg2, neg)
+
+
 def test_warm_solve_matches_plain_on_held_out_voxels(torch_cuda, trained):
     torch = torch_cuda
     ph = phantom.single_shell_small(seed=99)   # held out: a different seed of the same protocol
