@@ -46,6 +46,20 @@ __device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// Lane `src`'s 64-bit word as a wave-uniform value (src must be wave-uniform).
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int src) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(v & 0xffffffffull), src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Sum over the wavefront.
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
 // Broadcast lane `src`'s double to the whole wavefront (src must be wave-uniform).
 __device__ __forceinline__ double lane_bcast(double v, int src) {
   const unsigned long long u = __double_as_longlong(v);
@@ -76,7 +90,7 @@ __device__ __forceinline__ int opaque_index(int i) {
   return i;
 }
 
-constexpr int kPipe = 8;   // elements per software-pipelined load chunk
+constexpr int kPipe = 4;   // elements per software-pipelined load chunk
 
 // The hot loops below are software pipelined by hand: the loads of chunk k+1 are issued before the
 // FMAs of chunk k (double buffering), and sched_barrier keeps the compiler from moving them back
@@ -270,9 +284,10 @@ __device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& d
     for (int i = 0; i < kPipe; ++i)
       if (J0 + i < NSH) buf[0][i] = p[i];
   }
+  const int t = opaque_lane();   // per-step lane masks are recomputed, not hoisted (one SGPR pair each)
   const double q = A[KK] / d;
-  const double c = (tid != KK) ? q : 0.0;
-  di = (tid == KK) ? A[KK] : di;
+  const double c = (t != KK) ? q : 0.0;
+  di = (t == KK) ? A[KK] : di;
   const double rn = fma(-c, bk, rhs);
 #pragma unroll
   for (int k = 0; J0 + k * kPipe < NSH; ++k) {
@@ -303,11 +318,12 @@ __device__ __forceinline__ void gj_sweep(double (&A)[NSH], double& rhs, double& 
   (gj_step<NSH, NT, KK>(A, rhs, di, col0, col1, tid), ...);
 }
 
-// amdgpu_waves_per_eu: caps the 64-thread kernel at 168 VGPRs so three wavefronts share each SIMD
-// (the LDS footprint, ~12.8 KB per wavefront at NSH = 45, allows twelve per CU); the 128-thread
-// kernel keeps 91-wide rows in registers and runs two wavefronts per SIMD.
+// amdgpu_waves_per_eu: caps the 64-thread kernel at 128 VGPRs so four wavefronts share each SIMD
+// (the LDS footprint, ~9.9 KB per wavefront at NSH = 45, allows sixteen per CU); the kernel is
+// latency bound, and the fourth wavefront hides more than the few spilled per-voxel values cost.
+// The 128-thread kernel keeps 91-wide rows in registers and runs two wavefronts per SIMD.
 template <int NSH, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3 : 2))) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4 : 2))) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
                                                             int ld_dw, const uint8_t* __restrict__ valid,
                                                             size_t nvox, float* __restrict__ fod,
                                                             uint16_t* __restrict__ iters_out,
@@ -323,13 +339,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
   constexpr int NTRI = NSH * (NSH + 1) / 2;
   constexpr int kColStride = col_stride<NT>();
   constexpr int COLS = 2 * kColStride;                       // two pivot-column buffers
-  constexpr int HBUF = COLS;
-  __shared__ float s_b[kMaxDW];
+  constexpr int HBUF = COLS > kMaxDW / 2 ? COLS : kMaxDW / 2;
   __shared__ double s_F[NT];
-  __shared__ __attribute__((aligned(16))) double s_h[HBUF];  // pivot columns of the sweep
+  __shared__ __attribute__((aligned(16))) double s_h[HBUF];  // pivot columns of the sweep; DW signals
   __shared__ double s_W[NTRI];        // packed lower triangle of the normal matrix
   __shared__ unsigned long long s_words[kMaxDirWords];
   __shared__ unsigned long long s_vox;
+  float* const s_b = reinterpret_cast<float*>(s_h);   // DW signals during set-up (s_h is free then)
   __shared__ int s_flag[NWAVE];
 
   const int tid = threadIdx.x;
@@ -342,7 +358,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
   const int wave = tid >> 6;
   const bool row_ok = tid < NSH;
 
-  unsigned long long st_vox = 0, st_solves = 0, st_k = 0, st_conv = 0, st_fallback = 0, st_warm_solves = 0;
+  // statistics of this workgroup (32-bit: well below 2^32 per workgroup); |neg| is summed per lane
+  unsigned st_vox = 0, st_solves = 0, st_conv = 0, st_fallback = 0, st_warm_solves = 0;
+  unsigned long long st_kl = 0;
+  const int lane = tid & 63;
   const bool warm = (warm_F0 != nullptr || warm_neg != nullptr) && P.niter > 0;
 
   for (;;) {
@@ -416,9 +435,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
       if (tid >= ni) f_plain = 0.0;
     }
 
-    unsigned long long oldw[kMaxDirWords];
+    // The negative sets as bitmasks held one 64-bit word per lane (lane w: directions 64w..64w+63),
+    // so they occupy one VGPR pair instead of scalar registers; |neg| is counted per lane.
+    unsigned long long my_old = 0ull;
     int n = 0;
-    int last_k = 0;
+    int last_kl = 0;
     bool converged = false;
     double f = 0.0;
     // pass 0: warm start (proposed F0 and/or negative set), at most warm_budget iterate() calls;
@@ -435,25 +456,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
         wave_sync();
         store_lower_row<NSH>(s_W, s_F, A0);
       }
-#pragma unroll
-      for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = 0ull;
+      my_old = 0ull;
       bool have_old = false;   // old_neg = [-1] (csd.h:240)
-      last_k = 0;
+      last_kl = 0;
 
       for (n = 0; n < limit; ++n) {
-        unsigned long long neww[kMaxDirWords];
-        bool same = have_old;
-        int k = 0;
+        unsigned long long my_new = 0ull;
+        bool same = false;
         if (proposal && n == 0) {
-          // the proposed negative set replaces the first amplitude evaluation
-#pragma unroll
-          for (int w = 0; w < kMaxDirWords; ++w) {
-            const int left = R - 64 * w;   // only bits of existing directions are honoured
-            const unsigned long long valid_bits = left >= 64 ? ~0ull : (left > 0 ? (1ull << left) - 1ull : 0ull);
-            neww[w] = w < nwio ? (uniform64(warm_neg[vox * nwio + w]) & valid_bits) : 0ull;
-            k += __popcll(neww[w]);
+          // the proposed negative set replaces the first amplitude evaluation; only bits of
+          // existing directions are honoured
+          if (lane < nwio) {
+            const int left = R - 64 * lane;
+            const unsigned long long valid_bits = left >= 64 ? ~0ull : (1ull << left) - 1ull;
+            my_new = warm_neg[vox * nwio + lane] & valid_bits;
           }
-          same = false;
         } else {
           // ---- amplitudes and the negative set (csd.h:246-250) ----
           // One block of NT directions at a time (thread t: direction b*NT + t): NSH independent
@@ -494,40 +511,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
             if ((tid & 63) == 0) s_words[b * NWAVE + wave] = ball;
           }
           wave_sync();
-#pragma unroll
-          for (int w = 0; w < kMaxDirWords; ++w) {
-            neww[w] = w < nwords ? uniform64(s_words[w]) : 0ull;
-            k += __popcll(neww[w]);
-            same = same && (neww[w] == oldw[w]);
-          }
+          my_new = lane < nwords ? s_words[lane] : 0ull;
+          same = have_old && __ballot(my_new != my_old) == 0ull;
         }
         if (same) { converged = true; break; }   // (csd.h:252-253)
         // ---- W = Mt_M + H_neg^T H_neg (csd.h:255-262), updated by the change of the neg set ----
         // Directions that entered (+) or left (-) neg since the last solve are applied to the
         // packed lower triangle in LDS as a signed rank-4 update per MFMA (mfma_update).
         {
+          // word w of the change (scanned in order): delta = new ^ old (new alone at the first
+          // solve), cur_nw = the new word, for the sign
           int w = 0;
-          unsigned long long delta = have_old ? (neww[0] ^ oldw[0]) : neww[0];
+          unsigned long long cur_nw = readlane64(my_new, 0);
+          unsigned long long delta = have_old ? (cur_nw ^ readlane64(my_old, 0)) : cur_nw;
           // next direction of the change (entered: +1, left: -1); 0 when none is left
           auto take1 = [&](int& cnt, double& sgn) __attribute__((always_inline)) -> int {
             while (delta == 0ull && w + 1 < nwords) {
               ++w;
-              unsigned long long nw = 0ull, ow = 0ull;
-#pragma unroll
-              for (int q = 0; q < kMaxDirWords; ++q) {
-                nw = (q == w) ? neww[q] : nw;
-                ow = (q == w) ? oldw[q] : ow;
-              }
-              delta = have_old ? (nw ^ ow) : nw;
+              cur_nw = readlane64(my_new, w);
+              delta = have_old ? (cur_nw ^ readlane64(my_old, w)) : cur_nw;
             }
             sgn = 0.0;
             if (delta == 0ull) return 0;
             const int bit = __ffsll((long long)delta) - 1;
             delta &= delta - 1;
-            unsigned long long nw = 0ull;
-#pragma unroll
-            for (int q = 0; q < kMaxDirWords; ++q) nw = (q == w) ? neww[q] : nw;
-            sgn = ((nw >> bit) & 1ull) ? 1.0 : -1.0;
+            sgn = ((cur_nw >> bit) & 1ull) ? 1.0 : -1.0;
             ++cnt;
             return w * 64 + bit;
           };
@@ -550,12 +558,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
         wave_sync();
         double A[NSH];
         load_sym_row<NSH>(s_W, A);
-#pragma unroll
-        for (int w = 0; w < kMaxDirWords; ++w) oldw[w] = neww[w];
+        my_old = my_new;
         have_old = true;
-        last_k = k;
+        last_kl = __popcll(my_new);
         ++st_solves;
-        st_k += (unsigned long long)k;
+        st_kl += (unsigned long long)last_kl;
 
         // ---- Gauss-Jordan sweep on [W | Mt_b] (thread i owns row i) ----
         // Step kk eliminates column kk from every other row.  Pivot row kk equals column kk past
@@ -579,28 +586,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 3
     }
 
     if (row_ok) fod[vox * NSH + tid] = (float)f;
-    if (tid == 0) {
-      if (iters_out) iters_out[vox] = (uint16_t)n;
-      if (nneg_out) nneg_out[vox] = (uint16_t)last_k;
+    if (nneg_out) {
+      const unsigned long long last_k = wave_sum((unsigned long long)last_kl);
+      if (tid == 0) nneg_out[vox] = (uint16_t)last_k;
     }
-    if (negmask) {
-      unsigned long long mine = 0ull;
-#pragma unroll
-      for (int w = 0; w < kMaxDirWords; ++w) mine = (tid == w) ? oldw[w] : mine;
-      if (tid < nwio) negmask[vox * nwio + tid] = mine;
-    }
+    if (tid == 0 && iters_out) iters_out[vox] = (uint16_t)n;
+    if (negmask && tid < nwio) negmask[vox * nwio + tid] = my_old;
     ++st_vox;
     if (converged && P.niter) ++st_conv;
     wave_sync();
   }
 
-  if (stats && tid == 0) {
-    atomicAdd(stats + 0, st_vox);
-    atomicAdd(stats + 1, st_solves);
-    atomicAdd(stats + 2, st_k);
-    atomicAdd(stats + 3, st_conv);
-    atomicAdd(stats + 4, st_fallback);
-    atomicAdd(stats + 5, st_warm_solves);
+  if (stats && wave == 0) {
+    const unsigned long long st_k = wave_sum(st_kl);
+    if (tid == 0) {
+      atomicAdd(stats + 0, (unsigned long long)st_vox);
+      atomicAdd(stats + 1, (unsigned long long)st_solves);
+      atomicAdd(stats + 2, st_k);
+      atomicAdd(stats + 3, (unsigned long long)st_conv);
+      atomicAdd(stats + 4, (unsigned long long)st_fallback);
+      atomicAdd(stats + 5, (unsigned long long)st_warm_solves);
+    }
   }
 }
 
