@@ -1,8 +1,13 @@
-"""Benchmark: CSD voxels/s on one MI355X (the dwi2fod csd hot path).
+"""Benchmark: CSD voxels/s on one MI355X (the dwi2fod csd / msmt_csd hot path).
 
-Workload: the whole-brain-size single-shell phantom (128x128x72, ellipsoidal mask with 646,128
-voxels, 8 b=0 + 64 b=3000 volumes, lmax 8, 300 constraint directions, reference defaults).
-A step is one solve of every masked voxel with the signals already resident in HBM.
+Default workload (--workload single_shell_brain): the whole-brain-size single-shell phantom
+(128x128x72, ellipsoidal mask with 646,128 voxels, 8 b=0 + 64 b=3000 volumes, lmax 8, 300
+constraint directions, reference defaults), solved plain (cold start) and with the learned warm
+start.  A step is one solve of every masked voxel with the signals already resident in HBM.
+
+Other workloads: --workload super_resolved (same mask, 5 b=0 + 45 b=3000, lmax 12: 91
+coefficients from 45 measurements) and --workload multi_shell (same mask, b=0/1000/2000/3000 with
+10/30/60/90 volumes, three tissues, the ICLS solver behind msmt_csd).
 
 Prints one JSON line with the throughput, the roofline position of the solver kernel (FP64
 algorithmic FLOPs of the reference algorithm, counted from per-voxel iteration statistics the
@@ -24,6 +29,7 @@ METRIC = "dwi2fod voxels/s on 1 MI355X: plain vs BF16+NN warm start at ≤1e-4 S
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
+CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096}
 
 
 def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
@@ -38,6 +44,19 @@ def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
     return dict(total=set_ + gemv + syrk + llt, set=set_, amplitudes=gemv, normal_matrix=syrk, solve=llt)
 
 
+def icls_flops(n, nmeas, ncon, stats):
+    """Algorithmic FLOPs of the reference ICLS solver for the counted work.  stats = (voxels, outer
+    iterations, inner solves, sum k, sum k^2, sum k^3 over inner solves, sum k over constraint
+    updates, ...): per voxel 2n(Nmeas + Ncon) + n^2, per inner solve k(k+1)n + k^3/3 + 2k^2, per
+    outer iteration 2nk + 2n Ncon."""
+    v, outer, inner, s1, s2, s3, sc = [float(x) for x in stats[:7]]
+    n, m, c = float(n), float(nmeas), float(ncon)
+    setup = v * (2 * n * m + 2 * n * c + n * n)
+    solves = n * (s2 + s1) + s3 / 3.0 + 2 * s2
+    updates = 2 * n * sc + 2 * n * c * outer
+    return dict(total=setup + solves + updates, setup=setup, solves=solves, updates=updates)
+
+
 def load_traffic(kernel_tag):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
     copy-kernel calibration), when a profile summary has been recorded for this kernel."""
@@ -48,100 +67,119 @@ def load_traffic(kernel_tag):
     return None
 
 
-def cpu_baseline(ph, gpu_fod, sample):
+def threads():
+    t = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return max(1, min(t, os.cpu_count() or 1))
+
+
+def cpu_baseline_csd(ph, lmax, gpu_fod, sample):
     from oracle import orc
-    o = orc.CsdOracle(ph.grad, ph.response)
+    o = orc.CsdOracle(ph.grad, ph.response, lmax=lmax)
     sample = min(sample, ph.nvox)
     sample -= sample % 128
     sub = np.ascontiguousarray(ph.dw[:sample])
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    nt = threads()
     t0 = time.perf_counter()
-    fod, iters = o.run_volume(sub, (128, sample // 128, 1), nthreads=threads)
+    fod, iters = o.run_volume(sub, (128, sample // 128, 1), nthreads=nt)
     dt = time.perf_counter() - t0
     ref = fod.astype(np.float64)
     g = gpu_fod[:sample].astype(np.float64)
     rel = np.linalg.norm(g - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
-    return dict(value=sample / dt, unit="voxels/s", cores=threads, kind="port",
+    return dict(value=sample / dt, unit="voxels/s", cores=nt, kind="port",
                 sample="first %d masked voxels of the same phantom, oracle C restatement (FP64, "
                        "reference x-line scheduling), %.1f s wall" % (sample, dt)), rel, ref
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=131072)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start")
-    ap.add_argument("--train-phantoms", type=int, default=3,
-                    help="phantoms (646k voxels each, further seeds of the protocol) to train on")
-    ap.add_argument("--epochs", type=int, default=12)
-    ap.add_argument("--warm-budget", type=int, default=6)
-    ap.add_argument("--copy-calibration-mb", type=int, default=0,
-                    help="launch one device copy of this many MiB first (a known-traffic kernel for "
-                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: the CSD voxel solve shards by voxel slabs with no "
-              "exchange, and this build runs and reports a single MI355X (--gpus 1).",
-              file=sys.stderr)
-        sys.exit(2)
+def cpu_baseline_icls(ph, shared, gpu_x, sample):
+    from oracle import orc
+    o = orc.IclsOracle(shared.matrix("C"), shared.matrix("A"), None, 0, 1e-10, 1e-10)
+    sample = min(sample, ph.nvox)
+    sub = np.ascontiguousarray(ph.dw[:sample])
+    nt = threads()
+    t0 = time.perf_counter()
+    x, it = o.run_batch(sub, nthreads=nt)
+    dt = time.perf_counter() - t0
+    ref = x.astype(np.float64)
+    g = gpu_x[:sample].astype(np.float64)
+    rel = np.linalg.norm(g - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
+    return dict(value=sample / dt, unit="voxels/s", cores=nt, kind="port",
+                sample="first %d masked voxels of the same phantom, oracle C restatement of "
+                       "ICLS::Solver (FP64), %.1f s wall" % (sample, dt)), rel
 
-    import torch
-    from mrtrix3_amd import csd, phantom
 
-    dev = torch.device("cuda:0")
-    print("[bench] generating single_shell_brain phantom", file=sys.stderr, flush=True)
-    ph = phantom.single_shell_brain()
-    shared = csd.CSDShared(ph.grad, ph.response)
-    solver = csd.CSD(shared)
-    dw = torch.from_numpy(ph.dw).to(dev)
-    fod = torch.empty((ph.nvox, shared.n_sh), dtype=torch.float32, device=dev)
-    stats = torch.zeros(6, dtype=torch.int64, device=dev)
-    torch.cuda.synchronize()
-    if args.copy_calibration_mb:
-        src = torch.empty(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev).fill_(1.0)
+class Timer:
+    """Steps timed with HIP events on the launch stream; wall time around them."""
+
+    def __init__(self, torch, stream, steps):
+        self.torch, self.stream, self.steps = torch, stream, steps
+
+    def run(self, run_once, label):
+        torch = self.torch
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(self.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(self.steps)]
+        print("[bench] timing %d steps (%s)" % (self.steps, label), file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(self.steps):
+            starts[i].record(self.stream)
+            run_once(i)
+            ends[i].record(self.stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+
+
+def calibrate_copy(torch, dev, mb):
+    if mb:
+        src = torch.empty(mb << 18, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
 
+
+def bench_csd(args, torch, workload):
+    from mrtrix3_amd import csd, phantom
+
+    dev = torch.device("cuda:0")
+    if workload == "super_resolved":
+        make, lmax, tag = phantom.super_resolved, 12, "csd_persistent_kernel<91>"
+        desc = ("super_resolved 128x128x72 ellipsoid mask, %d voxels, 5 b=0 + 45 b=3000, lmax 12, "
+                "300 constraint directions")
+    else:
+        make, lmax, tag = phantom.single_shell_brain, 8, "csd_persistent_kernel<45>"
+        desc = ("single_shell_brain 128x128x72 ellipsoid mask, %d voxels, 8 b=0 + 64 b=3000, lmax 8, "
+                "300 constraint directions")
+    print("[bench] generating %s phantom" % workload, file=sys.stderr, flush=True)
+    ph = make()
+    shared = csd.CSDShared(ph.grad, ph.response, lmax=lmax)
+    solver = csd.CSD(shared)
+    dw = torch.from_numpy(ph.dw).to(dev)
+    fod = torch.empty((ph.nvox, shared.n_sh), dtype=torch.float32, device=dev)
+    stats = torch.zeros(6, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    calibrate_copy(torch, dev, args.copy_calibration_mb)
+
     print("[bench] warmup %d" % args.warmup, file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         solver.run(dw, fod=fod, iters=False, nneg=False)
     torch.cuda.synchronize()
-
     stream = torch.cuda.current_stream(dev)
-
-    def timed(run_once, label):
-        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        print("[bench] timing %d steps (%s)" % (args.steps, label), file=sys.stderr, flush=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            starts[i].record(stream)
-            run_once(i)
-            ends[i].record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return wall, float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+    timer = Timer(torch, stream, args.steps)
+    traffic = load_traffic("csd_persistent_kernel_%s" % workload)
 
     def roofline(st, kern_ms):
         fl = csd_flops(shared.n_sh, shared.n_init, shared.n_dw, shared.n_hr, st)
         achieved = fl["total"] / (kern_ms * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": load_traffic("csd_persistent_kernel_single_shell_brain"),
-                "kernel": "csd_persistent_kernel<45>", "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                "kernel": tag, "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
                 "note": "FP64-bound; peak is the MI355X FP64 rate (vector = matrix)"}
 
     # ---- plain GPU CSD (cold start from rconv b) ----
-    wall, kern_ms = timed(lambda i: solver.run(dw, fod=fod, iters=False, nneg=False,
-                                               stats=stats if i == 0 else None), "plain")
+    wall, kern_ms = timer.run(lambda i: solver.run(dw, fod=fod, iters=False, nneg=False,
+                                                   stats=stats if i == 0 else None), "plain")
     st = stats.cpu().numpy()
     plain = {"value": ph.nvox * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
              "solves_per_voxel": float(st[1]) / max(1, st[0]), "mean_neg": float(st[2]) / max(1, st[1]),
@@ -162,8 +200,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "single_shell_brain 128x128x72 ellipsoid mask, %d voxels, 8 b=0 + 64 "
-                               "b=3000, lmax 8, 300 constraint directions" % ph.nvox,
+        "config": {"workload": desc % ph.nvox,
                    "form": "plain GPU CSD: FP64 projection and solve, cold start, persistent kernel"},
         "roofline": plain["roofline"],
         "plain": plain,
@@ -176,7 +213,7 @@ def main():
         dwis = torch.from_numpy(shared.dwis.astype(np.int32)).to(dev)
         b0 = torch.from_numpy(np.nonzero(shared.matrix("grad")[:, 3] <= 10.0)[0].astype(np.int32)).to(dev)
         print("[bench] generating %d training phantoms" % args.train_phantoms, file=sys.stderr, flush=True)
-        train = np.concatenate([phantom.single_shell_brain(seed=100 + i).dw for i in range(args.train_phantoms)])
+        train = np.concatenate([make(seed=100 + i).dw for i in range(args.train_phantoms)])
         gen_s = time.perf_counter() - t0
         net = warm.WarmNet(n_in=shared.n_dw, hidden=256, nsh=shared.n_sh, nlog=shared.n_hr, seed=0)
         dtrain = torch.from_numpy(train).to(dev)
@@ -204,7 +241,7 @@ def main():
                        stats=wstats if i == 0 else None)
             ev[i][2].record(stream)
 
-        wall_w, _ = timed(warm_once, "warm")
+        wall_w, _ = timer.run(warm_once, "warm")
         infer_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
         kern_w = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
         sw = wstats.cpu().numpy()
@@ -235,15 +272,16 @@ def main():
         if ok >= 0.999:
             out["value"] = wr["value"]
             out["ms_per_step"] = wr["ms_per_step"]
-            out["config"]["form"] = ("BF16 MLP warm start (64-256-256-256-345, trained on %d plain-solver "
+            out["config"]["form"] = ("BF16 MLP warm start (%d-256-256-256-%d, trained on %d plain-solver "
                                      "voxels of other phantoms) + FP64 persistent CSD with fixed-point "
-                                     "certificate and plain fallback" % ntrain)
+                                     "certificate and plain fallback" % (net.n_in, net.nout, ntrain))
             out["roofline"] = wr["roofline"]
         fod.copy_(fod_plain)
 
     if not args.no_cpu:
-        print("[bench] CPU baseline on %d voxels" % args.cpu_sample, file=sys.stderr, flush=True)
-        base, rel, ref = cpu_baseline(ph, fod_plain.cpu().numpy(), args.cpu_sample)
+        sample = args.cpu_sample or CPU_SAMPLE[workload]
+        print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
+        base, rel, ref = cpu_baseline_csd(ph, lmax, fod_plain.cpu().numpy(), sample)
         out["cpu_baseline"] = base
         out["parity_on_cpu_sample"] = {"plain": {"max_rel_sh_err": float(rel.max()),
                                                  "p999_rel_sh_err": float(np.quantile(rel, 0.999)),
@@ -254,6 +292,95 @@ def main():
             out["parity_on_cpu_sample"]["warm_start"] = {"max_rel_sh_err": float(relw.max()),
                                                          "p999_rel_sh_err": float(np.quantile(relw, 0.999)),
                                                          "frac_le_1e-4": float(np.mean(relw <= 1e-4))}
+    return out
+
+
+def bench_msmt(args, torch):
+    from mrtrix3_amd import icls, phantom
+
+    dev = torch.device("cuda:0")
+    print("[bench] generating multi_shell phantom", file=sys.stderr, flush=True)
+    ph = phantom.multi_shell()
+    shared = icls.MSMTShared(ph.grad, ph.response)
+    prob = shared.problem()
+    b = torch.from_numpy(ph.dw).to(dev)
+    x = torch.empty((ph.nvox, prob.n), dtype=torch.float32, device=dev)
+    stats = torch.zeros(8, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    calibrate_copy(torch, dev, args.copy_calibration_mb)
+    print("[bench] warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        prob.solve(b, x=x)
+    torch.cuda.synchronize()
+    timer = Timer(torch, torch.cuda.current_stream(dev), args.steps)
+    wall, kern_ms = timer.run(lambda i: prob.solve(b, x=x, stats=stats if i == 0 else None), "msmt")
+    st = stats.cpu().numpy()
+    fl = icls_flops(prob.n, prob.nmeas, prob.ncon, st)
+    achieved = fl["total"] / (kern_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": ph.nvox * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "multi_shell 128x128x72 ellipsoid mask, %d voxels, b=0/1000/2000/3000 x "
+                               "10/30/60/90, WM lmax 8 + GM + CSF (%d parameters, %d constraints)"
+                               % (ph.nvox, prob.n, prob.ncon),
+                   "form": "msmt_csd: ICLS::Solver dual active set, FP64, persistent kernel"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic("icls_persistent_kernel_multi_shell"),
+                     "kernel": "icls_persistent_kernel", "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
+                     "note": "FP64-bound; peak is the MI355X FP64 rate (vector = matrix)"},
+        "icls": {"outer_per_voxel": float(st[1]) / max(1, st[0]), "inner_per_voxel": float(st[2]) / max(1, st[0]),
+                 "mean_active": float(st[3]) / max(1, st[2]), "over_capacity": int(st[7])},
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample or CPU_SAMPLE["multi_shell"]
+        print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
+        base, rel = cpu_baseline_icls(ph, shared, x.cpu().numpy(), sample)
+        out["cpu_baseline"] = base
+        out["parity_on_cpu_sample"] = {"max_rel_err": float(rel.max()),
+                                       "p999_rel_err": float(np.quantile(rel, 0.999)),
+                                       "frac_le_1e-4": float(np.mean(rel <= 1e-4))}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="single_shell_brain", choices=sorted(CPU_SAMPLE))
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="voxels in the CPU baseline sample (0: the workload's default)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start")
+    ap.add_argument("--train-phantoms", type=int, default=3,
+                    help="phantoms (646k voxels each, further seeds of the protocol) to train on")
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--warm-budget", type=int, default=6)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="launch one device copy of this many MiB first (a known-traffic kernel for "
+                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the CSD voxel solve shards by voxel slabs with no "
+              "exchange, and this build runs and reports a single MI355X (--gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if args.workload == "multi_shell":
+        out = bench_msmt(args, torch)
+    else:
+        out = bench_csd(args, torch, args.workload)
     print(json.dumps(out))
 
 

@@ -456,6 +456,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
         wave_sync();
         store_lower_row<NSH>(s_W, s_F, A0);
       }
+      // the first update may read these rows without an amplitude pass (and its barriers) in
+      // between, when a proposed set replaces it: rows written by other wavefronts must be visible
+      wave_sync();
       my_old = 0ull;
       bool have_old = false;   // old_neg = [-1] (csd.h:240)
       last_kl = 0;
