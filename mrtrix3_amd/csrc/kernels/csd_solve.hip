@@ -27,76 +27,20 @@
 #include <utility>
 
 #include "csd_solve.h"
+#include "gauss_jordan.h"
+#include "wave_utils.h"
 
 namespace mrcsd {
 namespace {
 
+using gj::col_stride;
+using gj::gj_sweep;
+using gj::kPipe;
+using gj::opaque_index;
+using gj::opaque_lane;
+
 constexpr int kMaxDirWords = 8;   // up to 512 constraint directions
 constexpr int kMaxDW = 256;       // up to 256 DW volumes per voxel
-// doubles per pivot-column buffer: NT column entries, then (workgroups wider than one wavefront)
-// the NT right-hand sides, so that the pivot's right-hand side is read from LDS.
-template <int NT>
-constexpr int col_stride() { return NT == 64 ? NT + 2 : 2 * NT; }
-
-// A value the whole wavefront agrees on, moved to scalar registers so that loops over it are
-// scalar loops.
-__device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(x & 0xffffffffull));
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-// Lane `src`'s 64-bit word as a wave-uniform value (src must be wave-uniform).
-__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int src) {
-  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(v & 0xffffffffull), src);
-  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), src);
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-// Sum over the wavefront.
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
-}
-
-// Broadcast lane `src`'s double to the whole wavefront (src must be wave-uniform).
-__device__ __forceinline__ double lane_bcast(double v, int src) {
-  const unsigned long long u = __double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(u & 0xffffffffull), src);
-  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// The workgroup is a single wavefront, so LDS hand-offs only need the compiler to keep program
-// order and wait for the LDS queue; __syncthreads() provides both (the barrier itself is elided
-// for one-wave workgroups).
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
-
-// The lane index as a value the compiler cannot treat as loop-invariant: per-column masks and
-// addresses derived from it are recomputed where they are used instead of being hoisted out of
-// the persistent voxel loop (where 45 of them would stay live in registers).
-__device__ __forceinline__ int opaque_lane() {
-  int t = threadIdx.x;
-  asm volatile("" : "+v"(t));
-  return t;
-}
-
-// An LDS index the compiler cannot relate to its neighbours.  Loads from different chunks of a row
-// then cannot be merged into one vectorised batch issued up front (which would keep the whole
-// row in flight and spill), while loads within a chunk still merge into 16-byte reads.
-__device__ __forceinline__ int opaque_index(int i) {
-  asm volatile("" : "+s"(i));
-  return i;
-}
-
-constexpr int kPipe = 4;   // elements per software-pipelined load chunk
-
-// The hot loops below are software pipelined by hand: the loads of chunk k+1 are issued before the
-// FMAs of chunk k (double buffering), and sched_barrier keeps the compiler from moving them back
-// next to their uses.  Left alone, the scheduler places every load directly before its FMA and
-// each FMA then waits for a full LDS or L2 round trip.
-
 // Row `i` of a symmetric matrix whose lower triangle is packed row by row (element (r,c), c <= r,
 // at r(r+1)/2 + c): the lower part comes from row i, the upper part from column i of later rows.
 template <int NSH>
@@ -252,70 +196,6 @@ __device__ __forceinline__ void mfma_update(double* W, double* junk, const doubl
       *(ok ? W + idx : junk + l) = acc[t][r];
     }
   }
-}
-
-// One Gauss-Jordan step on [A | rhs], thread i owning row i: eliminates column KK from every
-// row but KK.  The pivot row past the diagonal equals column KK (symmetric trailing block), read
-// from `cur`; the next step's column is updated first and published to `nxt` (look-ahead) so its
-// LDS round trip overlaps the remaining FMAs.  The first chunk of the pivot column is in flight
-// during the division.  A step has no branches (the divide runs on every lane and is masked;
-// every lane publishes): a conditional block per step lets the compiler sink each step's FMAs
-// into the next one, which keeps all the loaded pivot columns live and spills.  The pivot row's
-// diagonal is taken into `di` at its own step, so columns already eliminated are dead afterwards
-// and the live part of the row shrinks as the sweep proceeds.
-template <int NSH, int NT, int KK>
-__device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& di, double* col0, double* col1,
-                                        int tid) {
-  double* const cur = (KK & 1) ? col1 : col0;
-  double* const nxt = (KK & 1) ? col0 : col1;
-  constexpr int J0 = KK + 1;
-  double d, bk;
-  if constexpr (NT == 64) {
-    d = lane_bcast(A[KK], KK);
-    bk = lane_bcast(rhs, KK);
-  } else {
-    d = cur[KK];    // published by lane KK in the previous step (or before step 0)
-    bk = cur[NT + KK];
-  }
-  double buf[2][kPipe];
-  if constexpr (J0 < NSH) {
-    const double* p = cur + opaque_index(J0);
-#pragma unroll
-    for (int i = 0; i < kPipe; ++i)
-      if (J0 + i < NSH) buf[0][i] = p[i];
-  }
-  const int t = opaque_lane();   // per-step lane masks are recomputed, not hoisted (one SGPR pair each)
-  const double q = A[KK] / d;
-  const double c = (t != KK) ? q : 0.0;
-  di = (t == KK) ? A[KK] : di;
-  const double rn = fma(-c, bk, rhs);
-#pragma unroll
-  for (int k = 0; J0 + k * kPipe < NSH; ++k) {
-    const int j = J0 + k * kPipe;
-    if (j + kPipe < NSH) {
-      const double* p = cur + opaque_index(j + kPipe);
-#pragma unroll
-      for (int i = 0; i < kPipe; ++i)
-        if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < kPipe; ++i)
-      if (j + i < NSH) A[j + i] = fma(-c, buf[k & 1][i], A[j + i]);
-    if (k == 0) {
-      nxt[tid] = A[J0];
-      if constexpr (NT != 64) nxt[NT + tid] = rn;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  rhs = rn;
-  wave_sync();
-}
-
-template <int NSH, int NT, int... KK>
-__device__ __forceinline__ void gj_sweep(double (&A)[NSH], double& rhs, double& di, double* col0, double* col1,
-                                         int tid, std::integer_sequence<int, KK...>) {
-  (gj_step<NSH, NT, KK>(A, rhs, di, col0, col1, tid), ...);
 }
 
 // amdgpu_waves_per_eu: caps the 64-thread kernel at 128 VGPRs so four wavefronts share each SIMD

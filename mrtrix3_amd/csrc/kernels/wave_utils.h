@@ -25,6 +25,20 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 
 __device__ __forceinline__ int lane_bcast(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
 
+// Lane `src`'s 64-bit word as a wave-uniform value (src must be wave-uniform).
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int src) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(v & 0xffffffffull), src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Sum over the wavefront.
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
 // Single-wavefront workgroups: __syncthreads() keeps program order and drains the LDS queue;
 // the barrier itself is elided by the compiler for one-wave groups.
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
