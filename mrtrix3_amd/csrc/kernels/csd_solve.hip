@@ -36,7 +36,6 @@ namespace {
 using gj::col_stride;
 using gj::gj_sweep;
 using gj::kPipe;
-using gj::opaque_index;
 using gj::opaque_lane;
 
 constexpr int kMaxDirWords = 8;   // up to 512 constraint directions

@@ -26,12 +26,16 @@ __device__ __forceinline__ int opaque_lane() {
   return t;
 }
 
-// An LDS index the compiler cannot relate to its neighbours.  Loads from different chunks of a row
-// then cannot be merged into one vectorised batch issued up front (which would keep the whole
-// row in flight and spill), while loads within a chunk still merge into 16-byte reads.
-__device__ __forceinline__ int opaque_index(int i) {
-  asm volatile("" : "+s"(i));
-  return i;
+// A pointer into LDS whose value the compiler cannot see, held in one VGPR (the low 32 bits of a
+// generic pointer into LDS are its LDS address; converting this way avoids the null check of an
+// address-space cast).  Loads through it use immediate offsets, so a chunk of the pivot column
+// costs one address move, and loads through different such pointers are not merged into one
+// batch issued up front.
+typedef __attribute__((address_space(3))) double lds_f64;
+__device__ __forceinline__ const lds_f64* opaque_lds(const double* p) {
+  unsigned a = (unsigned)(size_t)p;
+  asm volatile("" : "+v"(a));
+  return (const lds_f64*)(size_t)a;
 }
 
 constexpr int kPipe = 4;   // elements per software-pipelined load chunk
@@ -66,11 +70,12 @@ __device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& d
   }
   double buf[2][kPipe];
   if constexpr (J0 < NSH) {
-    const double* p = cur + opaque_index(J0);
+    const lds_f64* p = opaque_lds(cur + J0);
 #pragma unroll
     for (int i = 0; i < kPipe; ++i)
       if (J0 + i < NSH) buf[0][i] = p[i];
   }
+  __builtin_amdgcn_sched_barrier(0);   // the first chunk's loads are issued before the divide
   const int t = opaque_lane();   // per-step lane masks are recomputed, not hoisted (one SGPR pair each)
   const double q = A[KK] / d;
   const double c = (t != KK) ? q : 0.0;
@@ -80,7 +85,7 @@ __device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& d
   for (int k = 0; J0 + k * kPipe < NSH; ++k) {
     const int j = J0 + k * kPipe;
     if (j + kPipe < NSH) {
-      const double* p = cur + opaque_index(j + kPipe);
+      const lds_f64* p = opaque_lds(cur + j + kPipe);
 #pragma unroll
       for (int i = 0; i < kPipe; ++i)
         if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
