@@ -220,7 +220,8 @@ def bench_csd(args, torch, workload):
         del train
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        hist = warm.train_warm_start(net, solver, dtrain, dwis, b0, epochs=args.epochs, batch=8192, lr=2e-3,
+        hist = warm.train_warm_start(net, solver, dtrain, dwis, b0, epochs=args.epochs, batch=8192, lr=args.lr,
+                                     wc=args.wc, wl=args.wl,
                                      log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
         torch.cuda.synchronize()
         train_s = time.perf_counter() - t1
@@ -260,7 +261,8 @@ def bench_csd(args, torch, workload):
               "fallback_frac": float(sw[4]) / max(1, sw[0]),
               "parity_vs_plain": {"max_rel_sh_err": float(rel.max()), "p999_rel_sh_err": float(np.quantile(rel, 0.999)),
                                   "identical_neg_set_frac": float(same.mean()), "ok_frac": ok},
-              "train": {"voxels": int(ntrain), "epochs": args.epochs, "seconds": train_s,
+              "train": {"voxels": int(ntrain), "epochs": args.epochs, "lr": args.lr, "wc": args.wc, "wl": args.wl,
+                        "seconds": train_s,
                         "phantom_generation_seconds": gen_s, "final_loss": [float(v) for v in hist[-1]]},
               "roofline": roofline(sw, kern_w),
               "inference": {"kernel": "infer_propose_kernel (features + 4 bf16 MFMA layers + set proposal)",
@@ -362,9 +364,12 @@ def main():
                     help="voxels in the CPU baseline sample (0: the workload's default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start")
-    ap.add_argument("--train-phantoms", type=int, default=3,
+    ap.add_argument("--train-phantoms", type=int, default=4,
                     help="phantoms (646k voxels each, further seeds of the protocol) to train on")
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=8e-3, help="peak learning rate of the warm-start training")
+    ap.add_argument("--wc", type=float, default=1.0, help="weight of the coefficient loss")
+    ap.add_argument("--wl", type=float, default=0.1, help="weight of the negative-set loss")
     ap.add_argument("--warm-budget", type=int, default=6)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="launch one device copy of this many MiB first (a known-traffic kernel for "
