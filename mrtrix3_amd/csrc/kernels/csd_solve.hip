@@ -201,7 +201,10 @@ __device__ __forceinline__ void mfma_update(double* W, double* junk, const doubl
 // (the LDS footprint, ~9.9 KB per wavefront at NSH = 45, allows sixteen per CU); the kernel is
 // latency bound, and the fourth wavefront hides more than the few spilled per-voxel values cost.
 // The 128-thread kernel keeps 91-wide rows in registers and runs two wavefronts per SIMD.
-template <int NSH, int NT>
+// WARM: the launch carries a proposal (warm_F0 and/or warm_neg).  The plain instantiation has no
+// warm pass, so nothing has to stay live across one; the warm instantiation forms rconv b only
+// when a voxel falls back to the plain start.
+template <int NSH, int NT, bool WARM>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4 : 2))) void csd_persistent_kernel(CsdDeviceProblem P, const float* __restrict__ dw,
                                                             int ld_dw, const uint8_t* __restrict__ valid,
                                                             size_t nvox, float* __restrict__ fod,
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
   unsigned st_vox = 0, st_solves = 0, st_conv = 0, st_fallback = 0, st_warm_solves = 0;
   unsigned long long st_kl = 0;
   const int lane = tid & 63;
-  const bool warm = (warm_F0 != nullptr || warm_neg != nullptr) && P.niter > 0;
+  const bool warm = WARM && P.niter > 0;
 
   for (;;) {
     unsigned long long vox = 0;
@@ -300,19 +303,40 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
         for (int i = 0; i < kPipe; ++i) {
           const int r = r0 + i < N ? r0 + i : N - 1;
           m[i] = P.M[r * NSH + rt];
-          g[i] = P.rconvT[r * ni + it];
+          if constexpr (!WARM) g[i] = P.rconvT[r * ni + it];
           bv[i] = r0 + i < N ? (double)s_b[r] : 0.0;
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < kPipe; ++i) {
           mtb = fma(m[i], bv[i], mtb);
-          f_plain = fma(g[i], bv[i], f_plain);
+          if constexpr (!WARM) f_plain = fma(g[i], bv[i], f_plain);
         }
       }
       if (!row_ok) mtb = 0.0;
       if (tid >= ni) f_plain = 0.0;
     }
+    // rconv b for the plain start of a warm launch, from the voxel's row in HBM (the signal
+    // buffer in LDS has been reused by then); same order of accumulation as above
+    auto plain_start = [&]() __attribute__((always_inline)) -> double {
+      const int ni = P.n_init;
+      const int it = tid < ni ? tid : 0;
+      double acc = 0.0;
+      for (int r0 = 0; r0 < N; r0 += kPipe) {
+        double g[kPipe], bv[kPipe];
+#pragma unroll
+        for (int i = 0; i < kPipe; ++i) {
+          const int r = r0 + i < N ? r0 + i : N - 1;
+          g[i] = P.rconvT[r * ni + it];
+          const float v = src[P.dwis[r]];
+          bv[i] = r0 + i < N ? (double)(v < 0.0f ? 0.0f : v) : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < kPipe; ++i) acc = fma(g[i], bv[i], acc);
+      }
+      return tid < ni ? acc : 0.0;
+    };
 
     // The negative sets as bitmasks held one 64-bit word per lane (lane w: directions 64w..64w+63),
     // so they occupy one VGPR pair instead of scalar registers; |neg| is counted per lane.
@@ -326,7 +350,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
     for (int pass = warm ? 0 : 1; pass < 2 && !converged; ++pass) {
       const bool proposal = pass == 0 && warm_neg != nullptr;
       const int limit = pass == 0 ? warm_budget : P.niter;
-      f = (pass == 0 && warm_F0 && row_ok) ? (double)warm_F0[vox * NSH + tid] : f_plain;
+      if constexpr (WARM) {
+        f = pass == 0 ? ((warm_F0 && row_ok) ? (double)warm_F0[vox * NSH + tid] : 0.0) : plain_start();
+      } else {
+        f = f_plain;
+      }
       {
         double A0[NSH];
         const double* mrow = P.MtM + (row_ok ? tid : 0) * NSH;
@@ -498,7 +526,7 @@ __global__ void reset_counter_kernel(unsigned long long* c) {
 
 template <int NSH, int NT>
 hipError_t launch_impl(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream) {
-  auto kern = csd_persistent_kernel<NSH, NT>;
+  auto kern = (L.warm_F0 || L.warm_neg) ? csd_persistent_kernel<NSH, NT, true> : csd_persistent_kernel<NSH, NT, false>;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0);
   if (e != hipSuccess) return e;
