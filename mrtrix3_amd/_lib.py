@@ -70,6 +70,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise MrcsdError("libmrcsd.so is not built (%s); run `python -c 'import __graft_entry__ as g; "
                          "g.build()'` or `python mrtrix3_amd/build.py`" % LIB_PATH)
+    # torch ships its own libamdhip64 / libhsa-runtime64 under the same sonames as ROCm's.  Loading
+    # torch first makes libmrcsd bind to that runtime, so the process holds ONE HIP runtime and torch
+    # device pointers are valid here; loaded the other way round, the process maps two runtimes and
+    # the second one to initialise reports no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in _SIG.items():
         fn = getattr(lib, name)

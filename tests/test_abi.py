@@ -44,3 +44,19 @@ def test_null_arguments_are_errors_not_crashes():
     assert b"null" in lib.mrcsd_last_error()
     assert lib.mrcsd_create(None, ctypes.byref(h)) == -1
     assert lib.mrcsd_run(None, None, 0, None, 0, None, None, None, None) == -1
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    # torch and ROCm each ship a libamdhip64 under the same soname; the package must never map both
+    # (the second runtime to initialise finds no device, and torch pointers are foreign to it)
+    import subprocess
+    import sys
+    code = ("import re\n"
+            "from mrtrix3_amd import _lib\n"
+            "_lib.load()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "print(len(set(re.findall(r'\\S*/libamdhip64\\S*', maps))))\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "1"

@@ -1,0 +1,98 @@
+"""The dwi2fod drop-in end to end on the GPU: .mif image and response text files in, the
+reference's command line, .mif FOD images out, compared voxel by voxel with the CPU oracle
+(cmd/dwi2fod.cpp: CSD_Processor :120-176 and MSMT_Processor :181-231 semantics, including the
+mask and the header keys written)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from mrtrix3_amd import icls, mif, phantom
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "mrtrix3_amd", "bin", "dwi2fod")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(args):
+    r = subprocess.run([EXE] + args, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def write_response(path, rows):
+    with open(path, "w") as f:
+        for row in np.atleast_2d(rows):
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+
+
+def test_cli_csd_matches_oracle(tmp_path):
+    assert os.path.exists(EXE), "dwi2fod is not built"
+    ph = phantom.single_shell_small()
+    shape = (16, 16, 8)
+    n = int(np.prod(shape))
+    img = ph.dw[:n].reshape(shape + (ph.dw.shape[1],))
+    rng = np.random.default_rng(5)
+    mask = (rng.random(shape) < 0.8).astype(np.float32)
+    dwi, msk, resp, out = (str(tmp_path / f) for f in ("dwi.mif", "mask.mif", "resp.txt", "fod.mif"))
+    mif.write(dwi, img, vox=[2.0, 2.0, 2.0], keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+    mif.write(msk, mask, vox=[2.0, 2.0, 2.0])
+    write_response(resp, ph.response)
+    say("[gpu cli] dwi2fod csd on %s voxels (%d in mask)" % (n, int(mask.sum())))
+    run(["csd", dwi, resp, out, "-mask", msk, "-force"])
+    fod, h = mif.read(out)
+    assert fod.shape == shape + (45,)
+    keys = [k for k, _ in h["keyval"]]
+    assert "prior_dw_scheme" in keys and "dw_scheme" not in keys
+    o = orc.CsdOracle(ph.grad, ph.response)
+    rel = []
+    for idx in np.ndindex(shape):
+        if mask[idx] == 0:
+            assert np.all(fod[idx] == 0)
+            continue
+        b = np.maximum(img[idx][o.vols].astype(np.float64), 0.0)
+        F, _, _, _ = o.solve(b)
+        rel.append(np.linalg.norm(fod[idx] - F) / max(np.linalg.norm(F), 1e-30))
+    rel = np.array(rel)
+    say("[gpu cli] csd: max rel %.3e over %d voxels" % (rel.max(), rel.size))
+    assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_cli_msmt_matches_oracle(tmp_path):
+    assert os.path.exists(EXE), "dwi2fod is not built"
+    shape = (8, 8, 4)
+    ph = phantom.multi_shell(seed=4, shape=shape, semi=None)
+    img = ph.dw.reshape(shape + (ph.dw.shape[1],))
+    dwi = str(tmp_path / "dwi.mif")
+    mif.write(dwi, img, vox=[2.0, 2.0, 2.0], keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+    args = ["msmt_csd", dwi]
+    outs = []
+    for name, r in zip(("wm", "gm", "csf"), ph.response):
+        rp, op = str(tmp_path / (name + ".txt")), str(tmp_path / (name + ".mif"))
+        write_response(rp, r)
+        args += [rp, op]
+        outs.append(op)
+    say("[gpu cli] dwi2fod msmt_csd on %d voxels" % ph.nvox)
+    run(args + ["-force"])
+    got = [mif.read(p)[0] for p in outs]
+    assert got[0].shape == shape + (45,) and got[1].shape[:3] == shape and got[2].shape[:3] == shape
+    s = icls.MSMTShared(ph.grad, ph.response)
+    o = orc.IclsOracle(s.matrix("C"), s.matrix("A"), None, 0, 1e-10, 1e-10)
+    rel = []
+    for idx in np.ndindex(shape):
+        x, _, _ = o.solve(img[idx].astype(np.float64))
+        g = np.concatenate([np.atleast_1d(got[0][idx]), np.atleast_1d(got[1][idx]).ravel(),
+                            np.atleast_1d(got[2][idx]).ravel()])
+        rel.append(np.linalg.norm(g - x) / max(np.linalg.norm(x), 1e-30))
+    rel = np.array(rel)
+    say("[gpu cli] msmt_csd: max rel %.3e over %d voxels" % (rel.max(), rel.size))
+    assert np.mean(rel <= 1e-4) >= 0.999

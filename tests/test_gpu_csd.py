@@ -229,3 +229,36 @@ def test_warm_start_from_perturbed_sets(torch_cuda, lmax):
         say("[gpu csd] lmax %d warm start from %s sets: max_rel=%.3e same_set=%.5f fallback=%.4f"
             % (lmax, name, rel.max(), same.mean(), s[4] / s[0]))
         assert np.mean((rel <= 1e-4) & same) >= 0.999
+
+
+@pytest.mark.parametrize("lmax,warm", [(12, False), (8, True), (12, True)])
+def test_mask_nonfinite_negative_all_kernels(torch_cuda, lmax, warm):
+    """dwi2fod.cpp:156-173 semantics (mask -> zeros, a non-finite selected signal -> zeros,
+    negative signals clamped) in the two-wavefront kernel and in the warm-start instantiations
+    (an empty proposed set here)."""
+    torch = torch_cuda
+    ph = phantom.super_resolved_small() if lmax == 12 else phantom.single_shell_small()
+    o = orc.CsdOracle(ph.grad, ph.response, lmax=lmax)
+    say("[gpu csd] masked / NaN / Inf / negative voxels, lmax %d, warm start %s" % (lmax, warm))
+    sub = np.ascontiguousarray(ph.dw[:64]).copy()
+    sub[1, o.vols[7]] = np.nan
+    sub[2, o.vols[0]] = -np.inf
+    sub[3, :] = -1.0
+    sub[4, o.vols[::2]] = -50.0
+    valid = np.ones(64, dtype=np.uint8)
+    valid[5] = 0
+    solver = csd.CSD(csd.CSDShared(ph.grad, ph.response, lmax=lmax))
+    kw = {}
+    if warm:
+        kw = dict(warm_neg=torch.zeros((64, (solver.n_hr + 63) // 64), dtype=torch.int64, device="cuda"))
+    fod, it, nn = solver.run(torch.from_numpy(sub).cuda(), valid=torch.from_numpy(valid).cuda(), **kw)
+    torch.cuda.synchronize()
+    fod = fod.cpu().numpy()
+    it = it.cpu().numpy().view(np.uint16)
+    for v in (1, 2, 5):
+        assert np.all(fod[v] == 0) and it[v] == 0xFFFF
+    assert np.all(fod[3] == 0)
+    for v in (0, 4, 6, 63):
+        b = np.maximum(sub[v, o.vols].astype(np.float64), 0.0)
+        F, n, _, _ = o.solve(b)
+        assert np.linalg.norm(fod[v] - F) <= 1e-4 * np.linalg.norm(F)
