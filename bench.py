@@ -57,13 +57,17 @@ def icls_flops(n, nmeas, ncon, stats):
     return dict(total=setup + solves + updates, setup=setup, solves=solves, updates=updates)
 
 
-def load_traffic(kernel_tag):
+def load_traffic(kernel_tag, leg=None):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), when a profile summary has been recorded for this kernel."""
+    copy-kernel calibration), when a profile summary has been recorded for this kernel (per leg —
+    "plain" or "warm" — where the summary holds both instantiations)."""
     path = os.path.join(ROOT, "profiles", kernel_tag + "_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            d = json.load(f)
+        if leg is not None and leg in d:
+            d = d[leg]
+        return d.get("bytes_per_launch")
     return None
 
 
@@ -72,39 +76,92 @@ def threads():
     return max(1, min(t, os.cpu_count() or 1))
 
 
+def host_info():
+    """CPU model, logical CPUs, physical cores (distinct (package, core) pairs) and the CPUs this
+    process may run on."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "physical_cores": len(cores) or None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def median_rate(run, n, reps=3):
+    """Median voxels/s over reps runs of run() on n voxels; also returns the last run's output."""
+    rates, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = run()
+        rates.append(n / (time.perf_counter() - t0))
+    return float(np.median(rates)), [float(r) for r in rates], out
+
+
 def cpu_baseline_csd(ph, lmax, gpu_fod, sample):
+    """The oracle's CSD on the host cores: the -O3 -march=native build (the value) and the generic
+    x86-64 -O3 build beside it, median of 3 runs each, with the reference's x-line scheduling;
+    plus single-thread GFLOP/s of its rank-k update and LLT at the workload's n."""
     from oracle import orc
-    o = orc.CsdOracle(ph.grad, ph.response, lmax=lmax)
     sample = min(sample, ph.nvox)
     sample -= sample % 128
     sub = np.ascontiguousarray(ph.dw[:sample])
     nt = threads()
-    t0 = time.perf_counter()
-    fod, iters = o.run_volume(sub, (128, sample // 128, 1), nthreads=nt)
-    dt = time.perf_counter() - t0
+    res = {}
+    for name, generic in (("native", False), ("generic", True)):
+        o = orc.CsdOracle(ph.grad, ph.response, lmax=lmax, generic=generic)
+        med, runs, (fod, _) = median_rate(lambda: o.run_volume(sub, (128, sample // 128, 1), nthreads=nt), sample)
+        res[name] = (o, med, runs, fod)
+    o, med, runs, fod = res["native"]
     ref = fod.astype(np.float64)
     g = gpu_fod[:sample].astype(np.float64)
     rel = np.linalg.norm(g - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
-    return dict(value=sample / dt, unit="voxels/s", cores=nt, kind="port",
-                sample="first %d masked voxels of the same phantom, oracle C restatement (FP64, "
-                       "reference x-line scheduling), %.1f s wall" % (sample, dt)), rel, ref
+    single = [o.time_syrk_llt(k) for k in (60, 144)]
+    return dict(value=med, unit="voxels/s", cores=nt, kind="port",
+                sample="first %d masked voxels of the same phantom; oracle C restatement of CSD (FP64, "
+                       "reference x-line scheduling, %d threads), median of 3 runs" % (sample, nt),
+                runs=runs, build="gcc -O3 -march=native",
+                generic_build={"build": "gcc -O3 (generic x86-64)", "value": res["generic"][1],
+                               "runs": res["generic"][2]},
+                single_thread_kernels=single, host=host_info()), rel, ref
 
 
 def cpu_baseline_icls(ph, shared, gpu_x, sample):
     from oracle import orc
-    o = orc.IclsOracle(shared.matrix("C"), shared.matrix("A"), None, 0, 1e-10, 1e-10)
     sample = min(sample, ph.nvox)
     sub = np.ascontiguousarray(ph.dw[:sample])
     nt = threads()
-    t0 = time.perf_counter()
-    x, it = o.run_batch(sub, nthreads=nt)
-    dt = time.perf_counter() - t0
+    res = {}
+    for name, generic in (("native", False), ("generic", True)):
+        o = orc.IclsOracle(shared.matrix("C"), shared.matrix("A"), None, 0, 1e-10, 1e-10, generic=generic)
+        med, runs, (x, _) = median_rate(lambda: o.run_batch(sub, nthreads=nt), sample)
+        res[name] = (med, runs, x)
+    med, runs, x = res["native"]
     ref = x.astype(np.float64)
     g = gpu_x[:sample].astype(np.float64)
     rel = np.linalg.norm(g - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
-    return dict(value=sample / dt, unit="voxels/s", cores=nt, kind="port",
-                sample="first %d masked voxels of the same phantom, oracle C restatement of "
-                       "ICLS::Solver (FP64), %.1f s wall" % (sample, dt)), rel
+    return dict(value=med, unit="voxels/s", cores=nt, kind="port",
+                sample="first %d masked voxels of the same phantom; oracle C restatement of "
+                       "ICLS::Solver (FP64, %d threads), median of 3 runs" % (sample, nt),
+                runs=runs, build="gcc -O3 -march=native",
+                generic_build={"build": "gcc -O3 (generic x86-64)", "value": res["generic"][0],
+                               "runs": res["generic"][1]},
+                host=host_info()), rel
 
 
 class Timer:
@@ -139,20 +196,81 @@ def calibrate_copy(torch, dev, mb):
         del src, dst
 
 
-def bench_csd(args, torch, workload):
-    from mrtrix3_amd import csd, phantom
+def write_response(path, rows):
+    with open(path, "w") as f:
+        for row in np.atleast_2d(np.asarray(rows, dtype=np.float64)):
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+
+
+def end_to_end_cli(ph, algo, extra_args=()):
+    """dwi2fod run as a user would: the phantom written as .mif images (4-D DWI with its gradient
+    table in dw_scheme, a mask, response text files), the command run as a child process, and the
+    FOD images written.  Reports the command's wall time and its own stage breakdown (-info)."""
+    import shutil
+    import subprocess
+    import tempfile
+    from mrtrix3_amd import mif
+    exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "dwi2fod")
+    if not os.path.exists(exe):
+        raise RuntimeError("dwi2fod is not built: run __graft_entry__.build()")
+    work = tempfile.mkdtemp(prefix="mrcsd_e2e_")
+    try:
+        nx, ny, nz = ph.shape
+        vol = np.zeros((nz, ny, nx, ph.dw.shape[1]), dtype=np.float32)
+        vol[ph.mask] = ph.dw
+        dwi, msk = os.path.join(work, "dwi.mif"), os.path.join(work, "mask.mif")
+        mif.write(dwi, vol.transpose(2, 1, 0, 3), vox=[2.0, 2.0, 2.0],
+                  keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+        del vol
+        mif.write(msk, ph.mask.transpose(2, 1, 0).astype(np.float32), vox=[2.0, 2.0, 2.0])
+        cmd = [exe, algo, dwi]
+        responses = ph.response if algo == "msmt_csd" else [ph.response]
+        for t, r in enumerate(responses):
+            rp, op = os.path.join(work, "response%d.txt" % t), os.path.join(work, "fod%d.mif" % t)
+            write_response(rp, r)
+            cmd += [rp, op]
+        cmd += ["-mask", msk, "-force", "-info"] + list(extra_args)
+        print("[bench] end to end: dwi2fod %s on %d voxels" % (algo, ph.nvox), file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError("dwi2fod failed: " + r.stderr[-2000:])
+        line = [l for l in r.stderr.splitlines() if "timing:" in l]
+        stages = {}
+        if line:   # "timing: read 1.128 s, setup 0.036 s, GPU solve incl. transfers 0.088 s, ..."
+            for part in line[-1].split("timing:", 1)[1].split(","):
+                name, sec, unit = part.strip().rsplit(" ", 2)
+                stages[name] = float(sec)
+        solve = stages.get("GPU solve incl. transfers")
+        return {"command": "dwi2fod %s dwi.mif %s-mask mask.mif -force -info%s" % (
+                    algo, "".join("response%d.txt fod%d.mif " % (t, t) for t in range(len(responses))),
+                    "".join(" " + a for a in extra_args)),
+                "voxels": ph.nvox, "wall_s": wall, "stages_s": stages,
+                "voxels_per_s_whole_command": ph.nvox / wall,
+                "voxels_per_s_solve_incl_transfers": ph.nvox / solve if solve else None,
+                "note": "child process: .mif read, solver set-up, HIP initialisation, pageable host "
+                        "buffers over PCIe, GPU solve, .mif write"}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def csd_workload(workload):
+    from mrtrix3_amd import phantom
+    if workload == "super_resolved":
+        return (phantom.super_resolved, 12, "csd_persistent_kernel<91, 128, %s>",
+                "super_resolved 128x128x72 ellipsoid mask, %d voxels, 5 b=0 + 45 b=3000, lmax 12, "
+                "300 constraint directions")
+    return (phantom.single_shell_brain, 8, "csd_persistent_kernel<45, 64, %s>",
+            "single_shell_brain 128x128x72 ellipsoid mask, %d voxels, 8 b=0 + 64 b=3000, lmax 8, "
+            "300 constraint directions")
+
+
+def bench_csd(args, torch, workload, ph):
+    from mrtrix3_amd import csd
 
     dev = torch.device("cuda:0")
-    if workload == "super_resolved":
-        make, lmax, tag = phantom.super_resolved, 12, "csd_persistent_kernel<91>"
-        desc = ("super_resolved 128x128x72 ellipsoid mask, %d voxels, 5 b=0 + 45 b=3000, lmax 12, "
-                "300 constraint directions")
-    else:
-        make, lmax, tag = phantom.single_shell_brain, 8, "csd_persistent_kernel<45>"
-        desc = ("single_shell_brain 128x128x72 ellipsoid mask, %d voxels, 8 b=0 + 64 b=3000, lmax 8, "
-                "300 constraint directions")
-    print("[bench] generating %s phantom" % workload, file=sys.stderr, flush=True)
-    ph = make()
+    make, lmax, tag, desc = csd_workload(workload)
     shared = csd.CSDShared(ph.grad, ph.response, lmax=lmax)
     solver = csd.CSD(shared)
     dw = torch.from_numpy(ph.dw).to(dev)
@@ -167,14 +285,13 @@ def bench_csd(args, torch, workload):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream(dev)
     timer = Timer(torch, stream, args.steps)
-    traffic = load_traffic("csd_persistent_kernel_%s" % workload)
-
-    def roofline(st, kern_ms):
+    def roofline(st, kern_ms, leg):
         fl = csd_flops(shared.n_sh, shared.n_init, shared.n_dw, shared.n_hr, st)
         achieved = fl["total"] / (kern_ms * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                "kernel": tag, "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
+                "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": load_traffic("csd_persistent_kernel_%s" % workload, leg),
+                "kernel": tag % ("true" if leg == "warm" else "false"), "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
                 "note": "FP64-bound; peak is the MI355X FP64 rate (vector = matrix)"}
 
     # ---- plain GPU CSD (cold start from rconv b) ----
@@ -183,7 +300,7 @@ def bench_csd(args, torch, workload):
     st = stats.cpu().numpy()
     plain = {"value": ph.nvox * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
              "solves_per_voxel": float(st[1]) / max(1, st[0]), "mean_neg": float(st[2]) / max(1, st[1]),
-             "converged_frac": float(st[3]) / max(1, st[0]), "roofline": roofline(st, kern_ms)}
+             "converged_frac": float(st[3]) / max(1, st[0]), "roofline": roofline(st, kern_ms, "plain")}
     fod_plain = fod.clone()
     fod_w = None
 
@@ -209,24 +326,35 @@ def bench_csd(args, torch, workload):
     # ---- learned warm start: bf16 MLP proposal, fixed-point certificate, plain fallback ----
     if not args.no_warm:
         from mrtrix3_amd import warm
-        t0 = time.perf_counter()
         dwis = torch.from_numpy(shared.dwis.astype(np.int32)).to(dev)
         b0 = torch.from_numpy(np.nonzero(shared.matrix("grad")[:, 3] <= 10.0)[0].astype(np.int32)).to(dev)
-        print("[bench] generating %d training phantoms" % args.train_phantoms, file=sys.stderr, flush=True)
-        train = np.concatenate([make(seed=100 + i).dw for i in range(args.train_phantoms)])
-        gen_s = time.perf_counter() - t0
-        net = warm.WarmNet(n_in=shared.n_dw, hidden=256, nsh=shared.n_sh, nlog=shared.n_hr, seed=0)
-        dtrain = torch.from_numpy(train).to(dev)
-        del train
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        hist = warm.train_warm_start(net, solver, dtrain, dwis, b0, epochs=args.epochs, batch=8192, lr=args.lr,
-                                     wc=args.wc, wl=args.wl,
-                                     log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
-        torch.cuda.synchronize()
-        train_s = time.perf_counter() - t1
-        ntrain = dtrain.shape[0]
-        del dtrain
+        if args.load_net:
+            net = warm.WarmNet.load(args.load_net)
+            assert (net.n_in, net.nsh, net.nlog) == (shared.n_dw, shared.n_sh, shared.n_hr), "network/protocol mismatch"
+            train_info = {"loaded": args.load_net}
+            ntrain = None
+        else:
+            t0 = time.perf_counter()
+            print("[bench] generating %d training phantoms" % args.train_phantoms, file=sys.stderr, flush=True)
+            train = np.concatenate([make(seed=100 + i).dw for i in range(args.train_phantoms)])
+            gen_s = time.perf_counter() - t0
+            net = warm.WarmNet(n_in=shared.n_dw, hidden=256, nsh=shared.n_sh, nlog=shared.n_hr, seed=0)
+            dtrain = torch.from_numpy(train).to(dev)
+            del train
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            hist = warm.train_warm_start(net, solver, dtrain, dwis, b0, epochs=args.epochs, batch=8192, lr=args.lr,
+                                         wc=args.wc, wl=args.wl,
+                                         log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+            torch.cuda.synchronize()
+            train_s = time.perf_counter() - t1
+            ntrain = dtrain.shape[0]
+            del dtrain
+            train_info = {"voxels": int(ntrain), "epochs": args.epochs, "lr": args.lr, "wc": args.wc,
+                          "wl": args.wl, "seconds": train_s, "phantom_generation_seconds": gen_s,
+                          "final_loss": [float(v) for v in hist[-1]]}
+            if args.save_net:
+                net.save(args.save_net)
         wstats = torch.zeros(6, dtype=torch.int64, device=dev)
         fod_w = torch.empty_like(fod)
         for _ in range(args.warmup):
@@ -261,10 +389,8 @@ def bench_csd(args, torch, workload):
               "fallback_frac": float(sw[4]) / max(1, sw[0]),
               "parity_vs_plain": {"max_rel_sh_err": float(rel.max()), "p999_rel_sh_err": float(np.quantile(rel, 0.999)),
                                   "identical_neg_set_frac": float(same.mean()), "ok_frac": ok},
-              "train": {"voxels": int(ntrain), "epochs": args.epochs, "lr": args.lr, "wc": args.wc, "wl": args.wl,
-                        "seconds": train_s,
-                        "phantom_generation_seconds": gen_s, "final_loss": [float(v) for v in hist[-1]]},
-              "roofline": roofline(sw, kern_w),
+              "train": train_info,
+              "roofline": roofline(sw, kern_w, "warm"),
               "inference": {"kernel": "infer_propose_kernel (features + 4 bf16 MFMA layers + set proposal)",
                             "ms": infer_ms, "flop": mlp_flop, "achieved": mlp_tflops,
                             "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": mlp_tflops / BF16_PEAK_TFLOPS},
@@ -274,11 +400,38 @@ def bench_csd(args, torch, workload):
         if ok >= 0.999:
             out["value"] = wr["value"]
             out["ms_per_step"] = wr["ms_per_step"]
-            out["config"]["form"] = ("BF16 MLP warm start (%d-256-256-256-%d, trained on %d plain-solver "
-                                     "voxels of other phantoms) + FP64 persistent CSD with fixed-point "
-                                     "certificate and plain fallback" % (net.n_in, net.nout, ntrain))
+            trained = ("trained on %d plain-solver voxels of other phantoms" % ntrain if ntrain is not None
+                       else "loaded from %s" % os.path.basename(args.load_net))
+            out["config"]["form"] = ("BF16 MLP warm start (%d-256-256-256-%d, %s) + FP64 persistent CSD with "
+                                     "fixed-point certificate and plain fallback" % (net.n_in, net.nout, trained))
             out["roofline"] = wr["roofline"]
         fod.copy_(fod_plain)
+
+    # ---- signals handed over in host memory: the PCIe-inclusive rate (not the metric) ----
+    print("[bench] host-buffer legs", file=sys.stderr, flush=True)
+    host_dw = np.ascontiguousarray(ph.dw)
+    med, runs, _ = median_rate(lambda: solver.run(host_dw, iters=False, nneg=False), ph.nvox)
+    hostleg = {"plain_host_pointer_entry": {
+        "value": med, "runs": runs,
+        "note": "mrcsd_run_host: pageable host signals in, device buffers allocated, H2D, plain solve, "
+                "D2H of the coefficients (the path dwi2fod takes)"}}
+    if fod_w is not None:
+        pin_in = torch.from_numpy(host_dw).pin_memory()
+        pin_out = torch.empty((ph.nvox, shared.n_sh), dtype=torch.float32).pin_memory()
+
+        def warm_pinned():
+            d = pin_in.to(dev, non_blocking=True)
+            _, neg = net.infer(d, dwis, b0)
+            solver.run(d, fod=fod_w, iters=False, nneg=False, warm_neg=neg, warm_budget=args.warm_budget)
+            pin_out.copy_(fod_w, non_blocking=True)
+            torch.cuda.synchronize()
+
+        warm_pinned()
+        med, runs, _ = median_rate(warm_pinned, ph.nvox)
+        hostleg["warm_pinned"] = {"value": med, "runs": runs,
+                                  "note": "pinned host signals, H2D, fused inference + warm-started solve, D2H"}
+        del pin_in, pin_out
+    out["host_buffers"] = hostleg
 
     if not args.no_cpu:
         sample = args.cpu_sample or CPU_SAMPLE[workload]
@@ -297,12 +450,10 @@ def bench_csd(args, torch, workload):
     return out
 
 
-def bench_msmt(args, torch):
-    from mrtrix3_amd import icls, phantom
+def bench_msmt(args, torch, ph):
+    from mrtrix3_amd import icls
 
     dev = torch.device("cuda:0")
-    print("[bench] generating multi_shell phantom", file=sys.stderr, flush=True)
-    ph = phantom.multi_shell()
     shared = icls.MSMTShared(ph.grad, ph.response)
     prob = shared.problem()
     b = torch.from_numpy(ph.dw).to(dev)
@@ -343,6 +494,13 @@ def bench_msmt(args, torch):
         "icls": {"outer_per_voxel": float(st[1]) / max(1, st[0]), "inner_per_voxel": float(st[2]) / max(1, st[0]),
                  "mean_active": float(st[3]) / max(1, st[2]), "over_capacity": int(st[7])},
     }
+    print("[bench] host-buffer leg", file=sys.stderr, flush=True)
+    host_b = np.ascontiguousarray(ph.dw)
+    med, runs, _ = median_rate(lambda: prob.solve(host_b), ph.nvox)
+    out["host_buffers"] = {"icls_host_pointer_entry": {
+        "value": med, "runs": runs,
+        "note": "mricls_solve_host: pageable host signals in, device buffers allocated, H2D, ICLS solve, "
+                "D2H (the path dwi2fod msmt_csd takes)"}}
     if not args.no_cpu:
         sample = args.cpu_sample or CPU_SAMPLE["multi_shell"]
         print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
@@ -364,6 +522,8 @@ def main():
                     help="voxels in the CPU baseline sample (0: the workload's default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end dwi2fod command on the phantom written as .mif files")
     ap.add_argument("--train-phantoms", type=int, default=4,
                     help="phantoms (646k voxels each, further seeds of the protocol) to train on")
     ap.add_argument("--epochs", type=int, default=32)
@@ -371,6 +531,9 @@ def main():
     ap.add_argument("--wc", type=float, default=1.0, help="weight of the coefficient loss")
     ap.add_argument("--wl", type=float, default=0.1, help="weight of the negative-set loss")
     ap.add_argument("--warm-budget", type=int, default=6)
+    ap.add_argument("--save-net", default="", help="write the trained warm-start network to this .npz")
+    ap.add_argument("--load-net", default="",
+                    help="use a warm-start network saved by --save-net instead of training one")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="launch one device copy of this many MiB first (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -381,11 +544,25 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # The phantom is generated, and the end-to-end command run as a child process, before this
+    # process touches the GPU.
+    from mrtrix3_amd import phantom
+    print("[bench] generating %s phantom" % args.workload, file=sys.stderr, flush=True)
+    if args.workload == "multi_shell":
+        ph = phantom.multi_shell()
+        e2e = None if args.no_e2e else end_to_end_cli(ph, "msmt_csd")
+    else:
+        make, lmax, _, _ = csd_workload(args.workload)
+        ph = make()
+        e2e = None if args.no_e2e else end_to_end_cli(ph, "csd", ["-lmax", str(lmax)])
+
     import torch
     if args.workload == "multi_shell":
-        out = bench_msmt(args, torch)
+        out = bench_msmt(args, torch, ph)
     else:
-        out = bench_csd(args, torch, args.workload)
+        out = bench_csd(args, torch, args.workload, ph)
+    if e2e is not None:
+        out["end_to_end"] = e2e
     print(json.dumps(out))
 
 

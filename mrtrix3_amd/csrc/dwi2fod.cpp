@@ -11,6 +11,7 @@
 // signal give zeros, negative CSD signals are clamped, MSMT uses the raw signal.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +37,27 @@ void console(const std::string& s) {
 void info(const std::string& s) {
   if (g_info) std::fprintf(stderr, "dwi2fod: [INFO] %s\n", s.c_str());
 }
+
+// Wall time of each stage (image read, solver set-up, GPU solve including the host/device
+// transfers, image write), reported with -info as one "timing:" line.
+struct Stages {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now(), last = t0;
+  std::string text;
+  void mark(const char* name) {
+    const auto now = std::chrono::steady_clock::now();
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "%s%s %.3f s", text.empty() ? "" : ", ", name,
+                  std::chrono::duration<double>(now - last).count());
+    text += buf;
+    last = now;
+  }
+  void report() {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, ", total %.3f s",
+                  std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    info("timing: " + text + buf);
+  }
+};
 
 std::vector<std::vector<double>> parse_rows(const std::string& text) {
   std::vector<std::vector<double>> rows;
@@ -198,6 +220,7 @@ int run(int argc, char** argv) {
   const std::string algo = a.pos[0];
   if (algo != "csd" && algo != "msmt_csd")
     throw std::runtime_error("algorithm must be one of: csd,msmt_csd");
+  Stages stages;
   console("reading image \"" + a.pos[1] + "\"");
   mrcsd::MifImage dwi = mrcsd::read_mif(a.pos[1]);
   if (dwi.h.dim.size() != 4) throw std::runtime_error("input DWI must be a 4-D image");
@@ -223,6 +246,7 @@ int run(int argc, char** argv) {
   std::vector<float> rows;
   gather(dwi, a.mask.empty() ? nullptr : &maskv, sel, rows);
   const size_t nsel = sel.size();
+  stages.mark("read");
 
   if (algo == "csd") {
     if (a.pos.size() != 4)
@@ -262,16 +286,20 @@ int run(int argc, char** argv) {
     check(mrcsd_shared_matrix(shared, 5, gused.data(), gused.size()));
     mrcsd_ctx* ctx = nullptr;
     check(mrcsd_create_from_shared(shared, &ctx));
+    stages.mark("setup");
     console("performing constrained spherical deconvolution on " + std::to_string(nsel) + " voxels (GPU)");
     std::vector<float> fod(nsel * size_t(si.n_sh));
     std::vector<uint16_t> iters(nsel);
     check(mrcsd_run_host(ctx, rows.data(), nvol, nullptr, nsel, fod.data(), iters.data(), nullptr));
+    stages.mark("GPU solve incl. transfers");
     size_t nonconv = 0;
     for (size_t i = 0; i < nsel; ++i)
       if (iters[i] != 0xFFFF && si.niter && iters[i] >= si.niter) ++nonconv;
     if (nonconv) info(std::to_string(nonconv) + " voxels did not reach full convergence");
     mrcsd::MifHeader h = output_header(dwi.h, si.n_sh, gused, nvol);
     scatter_write(a.pos[3], h, sel, fod.data(), si.n_sh, si.n_sh, 0);
+    stages.mark("write");
+    stages.report();
     mrcsd_destroy(ctx);
     mrcsd_shared_destroy(shared);
     return 0;
@@ -316,11 +344,13 @@ int run(int argc, char** argv) {
   check(mrmsmt_shared_info(ms, &nshell, &nparams, &ncon, lm.data()));
   mricls_ctx* ic = nullptr;
   check(mrmsmt_icls_create(ms, &ic));
+  stages.mark("setup");
   console("performing MSMT CSD (" + std::to_string(nshell) + " shell" + (nshell > 1 ? "s" : "") + ", " +
           std::to_string(nt) + " tissue" + (nt > 1 ? "s" : "") + ") on " + std::to_string(nsel) + " voxels (GPU)");
   std::vector<float> x(nsel * size_t(nparams));
   std::vector<uint16_t> iters(nsel);
   check(mricls_solve_host(ic, rows.data(), nvol, nullptr, nsel, x.data(), iters.data(), nullptr));
+  stages.mark("GPU solve incl. transfers");
   std::vector<double> gused = grad;   // stash the table as interpreted (normalised) by the set-up
   {
     for (int r = 0; r < nvol; ++r) {
@@ -350,6 +380,8 @@ int run(int argc, char** argv) {
     mrcsd::MifHeader h = dwi.h;
     scatter_write(a.predicted, h, sel, pred.data(), nvol, nvol, 0);
   }
+  stages.mark("write");
+  stages.report();
   mricls_destroy(ic);
   mrmsmt_shared_destroy(ms);
   return 0;

@@ -81,24 +81,26 @@ __device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& d
   const double c = (t != KK) ? q : 0.0;
   di = (t == KK) ? A[KK] : di;
   const double rn = fma(-c, bk, rhs);
+  if constexpr (J0 < NSH) {   // the last step has no trailing columns and no successor
 #pragma unroll
-  for (int k = 0; J0 + k * kPipe < NSH; ++k) {
-    const int j = J0 + k * kPipe;
-    if (j + kPipe < NSH) {
-      const lds_f64* p = opaque_lds(cur + j + kPipe);
+    for (int k = 0; J0 + k * kPipe < NSH; ++k) {
+      const int j = J0 + k * kPipe;
+      if (j + kPipe < NSH) {
+        const lds_f64* p = opaque_lds(cur + j + kPipe);
+#pragma unroll
+        for (int i = 0; i < kPipe; ++i)
+          if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < kPipe; ++i)
-        if (j + kPipe + i < NSH) buf[(k + 1) & 1][i] = p[i];
+        if (j + i < NSH) A[j + i] = fma(-c, buf[k & 1][i], A[j + i]);
+      if (k == 0) {
+        nxt[tid] = A[J0];
+        if constexpr (NT != 64) nxt[NT + tid] = rn;
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < kPipe; ++i)
-      if (j + i < NSH) A[j + i] = fma(-c, buf[k & 1][i], A[j + i]);
-    if (k == 0) {
-      nxt[tid] = A[J0];
-      if constexpr (NT != 64) nxt[NT + tid] = rn;
-    }
-    __builtin_amdgcn_sched_barrier(0);
   }
   rhs = rn;
   wave_sync();

@@ -70,6 +70,20 @@ class WarmNet:
         p = np.ascontiguousarray(p, dtype=np.float32)
         check(_lib.load().mrwarm_set_params(self._h, p.ctypes.data, p.size))
 
+    def save(self, path):
+        """Writes the layer sizes and fp32 master parameters as an .npz (no pickled objects): the
+        network is trained once per acquisition protocol and reused for every subject."""
+        np.savez(path, dims=np.array([self.n_in, self.hidden, self.nsh, self.nlog], dtype=np.int64),
+                 params=self.get_params())
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            n_in, hidden, nsh, nlog = (int(v) for v in z["dims"])
+            net = cls(n_in, hidden=hidden, nsh=nsh, nlog=nlog)
+            net.set_params(z["params"])
+        return net
+
     def features(self, dw, dwis, b0):
         """dw: CUDA float32 [M, nvol]; dwis / b0: CUDA int32 index tensors.  Returns (x bf16, s0)."""
         import torch

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifdef ORC_LONG_DOUBLE
 typedef long double orc_real;
@@ -132,18 +133,27 @@ void orc_zsh2rh(const double* zsh, int n, double* rh) {
 /* ------------------------------------------------------------------------------------------ */
 
 /* In-place Cholesky of the lower triangle of A (n x n, row-major); returns 0 on success,
- * k+1 if the k-th pivot is not positive (A is left partially factored, as Eigen's LLT does). */
+ * k+1 if the k-th pivot is not positive (A is left partially factored, as Eigen's LLT does).
+ * L_ij = (A_ij - sum_k L_ik L_jk) / L_jj with k ascending.  The strictly upper triangle is used as
+ * scratch: row j past the diagonal receives column j of L, so the update of column j reads
+ * earlier columns with unit stride and vectorises, with the same term order per entry. */
 static int llt_inplace(orc_real* A, int n) {
   for (int j = 0; j < n; j++) {
-    orc_real d = A[(size_t)j * n + j];
-    for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    orc_real* rj = A + (size_t)j * n;
+    orc_real d = rj[j];
+    for (int k = 0; k < j; k++) d -= rj[k] * rj[k];
     if (!(d > 0)) return j + 1;
     d = RSQRT(d);
-    A[(size_t)j * n + j] = d;
+    rj[j] = d;
+    for (int i = j + 1; i < n; i++) rj[i] = A[(size_t)i * n + j];
+    for (int k = 0; k < j; k++) {
+      const orc_real ljk = rj[k];
+      const orc_real* ck = A + (size_t)k * n;   /* ck[i] = L_ik for i > k */
+      for (int i = j + 1; i < n; i++) rj[i] -= ck[i] * ljk;
+    }
     for (int i = j + 1; i < n; i++) {
-      orc_real s = A[(size_t)i * n + j];
-      for (int k = 0; k < j; k++) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-      A[(size_t)i * n + j] = s / d;
+      rj[i] /= d;
+      A[(size_t)i * n + j] = rj[i];
     }
   }
   return 0;
@@ -261,6 +271,7 @@ static void pinv_r(const orc_real* M, int rows, int cols, orc_real* out) {
 }
 
 int orc_pinv(const double* M, int rows, int cols, double* out) {
+  if (rows <= 0 || cols <= 0) return -1;
   orc_real* m = (orc_real*)malloc(sizeof(orc_real) * (size_t)rows * cols);
   orc_real* o = (orc_real*)malloc(sizeof(orc_real) * (size_t)rows * cols);
   for (size_t i = 0; i < (size_t)rows * cols; i++) m[i] = M[i];
@@ -477,23 +488,33 @@ void orc_csd_free(orc_csd_shared* s) {
 /* ------------------------------------------------------------------------------------------ */
 /* CSD::set / CSD::iterate (csd.h:236-269) and the dwi2fod loop (dwi2fod.cpp:135-146)           */
 /* ------------------------------------------------------------------------------------------ */
+/* Per-thread workspace.  HRt (HR_trans transposed, [n_sh][n_hr]) and G (the gathered constraint
+ * rows) let the amplitude pass and the rank-k update run as unit-stride loops the host compiler
+ * vectorises; every sum keeps the reference's term order (see csd_iterate), so the results are
+ * the same bit for bit as the plain dot-product loops. */
 typedef struct csd_work {
-  orc_real *F, *Mtb, *amps, *W, *Fd;
+  orc_real *F, *Mtb, *amps, *W, *Fd, *HRt, *G;
   int *neg, *old_neg;
   int nneg, nold;
 } csd_work;
 
 static void work_alloc(csd_work* w, const orc_csd_shared* s) {
-  w->F = (orc_real*)malloc(sizeof(orc_real) * (size_t)s->n_sh);
-  w->Mtb = (orc_real*)malloc(sizeof(orc_real) * (size_t)s->n_sh);
-  w->amps = (orc_real*)malloc(sizeof(orc_real) * (size_t)s->n_hr);
-  w->W = (orc_real*)malloc(sizeof(orc_real) * (size_t)s->n_sh * s->n_sh);
-  w->Fd = (orc_real*)malloc(sizeof(orc_real) * (size_t)s->n_sh);
-  w->neg = (int*)malloc(sizeof(int) * ((size_t)s->n_hr + 1));
-  w->old_neg = (int*)malloc(sizeof(int) * ((size_t)s->n_hr + 1));
+  const int n = s->n_sh, R = s->n_hr;
+  w->F = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
+  w->Mtb = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
+  w->amps = (orc_real*)malloc(sizeof(orc_real) * (size_t)R);
+  w->W = (orc_real*)malloc(sizeof(orc_real) * (size_t)n * n);
+  w->Fd = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
+  w->HRt = (orc_real*)malloc(sizeof(orc_real) * (size_t)n * R);
+  w->G = (orc_real*)malloc(sizeof(orc_real) * (size_t)n * R);
+  w->neg = (int*)malloc(sizeof(int) * ((size_t)R + 1));
+  w->old_neg = (int*)malloc(sizeof(int) * ((size_t)R + 1));
+  for (int r = 0; r < R; r++)
+    for (int j = 0; j < n; j++) w->HRt[(size_t)j * R + r] = (orc_real)s->HR_trans[(size_t)r * n + j];
 }
 static void work_free(csd_work* w) {
-  free(w->F); free(w->Mtb); free(w->amps); free(w->W); free(w->Fd); free(w->neg); free(w->old_neg);
+  free(w->F); free(w->Mtb); free(w->amps); free(w->W); free(w->Fd); free(w->HRt); free(w->G);
+  free(w->neg); free(w->old_neg);
 }
 
 /* csd.h:236-243 (F0 nullable: the plain rconv initialisation). */
@@ -517,30 +538,47 @@ static void csd_set(const orc_csd_shared* s, csd_work* w, const double* b, const
   }
 }
 
+/* W = lower(Mt_M) + H_neg^T H_neg: W_ij = MtM_ij + sum_q h_qi h_qj with q ascending (csd.h:255-262),
+ * accumulated one gathered row q at a time so the j loop is unit stride */
+static void csd_form_normal(const orc_csd_shared* s, csd_work* w) {
+  const int n = s->n_sh;
+  const int k = w->nneg;
+  for (int q = 0; q < k; q++) {
+    const double* h = s->HR_trans + (size_t)w->neg[q] * n;
+    orc_real* g = w->G + (size_t)q * n;
+    for (int j = 0; j < n; j++) g[j] = (orc_real)h[j];
+  }
+  for (int i = 0; i < n; i++) {
+    orc_real* wr = w->W + (size_t)i * n;
+    for (int j = 0; j <= i; j++) wr[j] = (orc_real)s->MtM[(size_t)i * n + j];
+    for (int q = 0; q < k; q++) {
+      const orc_real* g = w->G + (size_t)q * n;
+      const orc_real gi = g[i];
+      for (int j = 0; j <= i; j++) wr[j] += gi * g[j];
+    }
+  }
+}
+
 /* csd.h:245-269.  Returns 1 on convergence (neg unchanged), else solves and returns 0. */
 static int csd_iterate(const orc_csd_shared* s, csd_work* w) {
   const int n = s->n_sh, R = s->n_hr;
-  w->nneg = 0;
-  for (int r = 0; r < R; r++) {
-    orc_real a = 0;
-    for (int j = 0; j < n; j++) a += (orc_real)s->HR_trans[(size_t)r * n + j] * w->F[j];
-    w->amps[r] = a;
-    if (a < (orc_real)s->threshold) w->neg[w->nneg++] = r;
+  /* a = HR_trans F: a_r = sum_j HR(r,j) F_j with j ascending, for all r at once */
+  orc_real* const a = w->amps;
+  for (int r = 0; r < R; r++) a[r] = 0;
+  for (int j = 0; j < n; j++) {
+    const orc_real fj = w->F[j];
+    const orc_real* h = w->HRt + (size_t)j * R;
+    for (int r = 0; r < R; r++) a[r] += h[r] * fj;
   }
+  w->nneg = 0;
+  for (int r = 0; r < R; r++)
+    if (a[r] < (orc_real)s->threshold) w->neg[w->nneg++] = r;
   if (w->nneg == w->nold) {
     int same = 1;
     for (int i = 0; i < w->nneg; i++) if (w->neg[i] != w->old_neg[i]) { same = 0; break; }
     if (same) return 1;
   }
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j <= i; j++) {
-      orc_real acc = (orc_real)s->MtM[(size_t)i * n + j];
-      for (int q = 0; q < w->nneg; q++) {
-        const double* h = s->HR_trans + (size_t)w->neg[q] * n;
-        acc += (orc_real)h[i] * (orc_real)h[j];
-      }
-      w->W[(size_t)i * n + j] = acc;
-    }
+  csd_form_normal(s, w);
   llt_inplace(w->W, n);   /* the reference does not check LLT::info() (csd.h:264) */
   for (int i = 0; i < n; i++) w->F[i] = w->Mtb[i];
   llt_solve_inplace(w->W, n, w->F);
@@ -733,31 +771,70 @@ void orc_icls_problem_free(orc_icls_problem* p) {
 /* ------------------------------------------------------------------------------------------ */
 /* Math::ICLS::Solver::operator() (constrained_least_squares.h:237-357)                         */
 /* ------------------------------------------------------------------------------------------ */
-int orc_icls_solve(const orc_icls_problem* P, const double* b, double* xout, uint8_t* active_out) {
+/* Per-thread workspace, allocated once per thread as the reference's Solver holds its vectors.
+ * Bt (B transposed, [n][ncon]) and Gt (the active rows of B transposed, [n][na]) make every
+ * product below a unit-stride loop the host compiler vectorises; each sum keeps the term order of
+ * the plain dot-product form (ascending summation index), so results are the same bit for bit. */
+typedef struct icls_ws {
+  orc_real *y_u, *x, *c, *c_u, *lambda, *lambda_prev, *l, *BtB, *Bt, *Gt;
+  int* act_idx;
+  uint8_t* active;
+} icls_ws;
+
+static void icls_ws_alloc(icls_ws* w, const orc_icls_problem* P) {
+  const int n = P->n, ncon = P->ncon;
+  w->y_u = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
+  w->x = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
+  w->c = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
+  w->c_u = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
+  w->lambda = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
+  w->lambda_prev = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
+  w->l = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
+  w->BtB = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon * ncon);
+  w->Bt = (orc_real*)malloc(sizeof(orc_real) * (size_t)n * ncon);
+  w->Gt = (orc_real*)malloc(sizeof(orc_real) * (size_t)n * ncon);
+  w->act_idx = (int*)malloc(sizeof(int) * (size_t)ncon);
+  w->active = (uint8_t*)malloc((size_t)ncon);
+  for (int r = 0; r < ncon; r++)
+    for (int j = 0; j < n; j++) w->Bt[(size_t)j * ncon + r] = (orc_real)P->B[(size_t)r * n + j];
+}
+
+static void icls_ws_free(icls_ws* w) {
+  free(w->y_u); free(w->x); free(w->c); free(w->c_u); free(w->lambda); free(w->lambda_prev);
+  free(w->l); free(w->BtB); free(w->Bt); free(w->Gt); free(w->act_idx); free(w->active);
+}
+
+/* c = B v - t: c_r = sum_j B(r,j) v_j (j ascending), then minus t_r */
+static void icls_constraints(const orc_icls_problem* P, const icls_ws* w, const orc_real* v, orc_real* c) {
+  const int n = P->n, ncon = P->ncon;
+  for (int r = 0; r < ncon; r++) c[r] = 0;
+  for (int j = 0; j < n; j++) {
+    const orc_real vj = v[j];
+    const orc_real* bt = w->Bt + (size_t)j * ncon;
+    for (int r = 0; r < ncon; r++) c[r] += bt[r] * vj;
+  }
+  if (P->t)
+    for (int r = 0; r < ncon; r++) c[r] -= (orc_real)P->t[r];
+}
+
+static int icls_solve_ws(const orc_icls_problem* P, icls_ws* w, const double* b, double* xout,
+                         uint8_t* active_out) {
   const int n = P->n, ncon = P->ncon, nmeas = P->nmeas;
   const int num_eq = P->num_eq, num_ineq = ncon - num_eq;
-  orc_real* y_u = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
-  orc_real* x = (orc_real*)malloc(sizeof(orc_real) * (size_t)n);
-  orc_real* c = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
-  orc_real* c_u = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
-  orc_real* lambda = (orc_real*)calloc((size_t)ncon, sizeof(orc_real));
-  orc_real* lambda_prev = (orc_real*)calloc((size_t)ncon, sizeof(orc_real));
-  orc_real* l = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon);
-  orc_real* BtB = (orc_real*)malloc(sizeof(orc_real) * (size_t)ncon * ncon);
-  int* act_idx = (int*)malloc(sizeof(int) * (size_t)ncon);
-  uint8_t* active = (uint8_t*)calloc((size_t)ncon, 1);
+  orc_real *y_u = w->y_u, *x = w->x, *c = w->c, *c_u = w->c_u, *lambda = w->lambda;
+  orc_real *lambda_prev = w->lambda_prev, *l = w->l, *BtB = w->BtB, *Gt = w->Gt;
+  int* act_idx = w->act_idx;
+  uint8_t* active = w->active;
+  for (int r = 0; r < ncon; r++) { lambda[r] = 0; lambda_prev[r] = 0; active[r] = 0; }
 
-  for (int j = 0; j < n; j++) {
-    orc_real s = 0;
-    for (int r = 0; r < nmeas; r++) s += (orc_real)P->b2d[(size_t)r * n + j] * (orc_real)b[r];
-    y_u[j] = s;
+  /* y_u = b2d^T b: y_j = sum_r b2d(r,j) b_r (r ascending) */
+  for (int j = 0; j < n; j++) y_u[j] = 0;
+  for (int r = 0; r < nmeas; r++) {
+    const orc_real br = (orc_real)b[r];
+    const double* row = P->b2d + (size_t)r * n;
+    for (int j = 0; j < n; j++) y_u[j] += (orc_real)row[j] * br;
   }
-  for (int r = 0; r < ncon; r++) {
-    orc_real s = 0;
-    for (int j = 0; j < n; j++) s += (orc_real)P->B[(size_t)r * n + j] * y_u[j];
-    if (P->t) s -= (orc_real)P->t[r];
-    c_u[r] = s;
-  }
+  icls_constraints(P, w, y_u, c_u);
   for (int r = num_ineq; r < ncon; r++) active[r] = 1;
   memcpy(c, c_u, sizeof(orc_real) * (size_t)ncon);
   memcpy(x, y_u, sizeof(orc_real) * (size_t)n);
@@ -775,14 +852,21 @@ int orc_icls_solve(const orc_icls_problem* P, const double* b, double* xout, uin
       int na = 0;
       for (int r = 0; r < ncon; r++)
         if (active[r]) { act_idx[na] = r; l[na] = -c_u[r]; na++; }
-      for (int i = 0; i < na; i++)
-        for (int j = 0; j <= i; j++) {
-          const double* bi = P->B + (size_t)act_idx[i] * n;
-          const double* bj = P->B + (size_t)act_idx[j] * n;
-          orc_real s = 0;
-          for (int k = 0; k < n; k++) s += (orc_real)bi[k] * (orc_real)bj[k];
-          BtB[(size_t)i * na + j] = s;
+      /* BtB = B_act B_act^T (lower): BtB_ij = sum_k B(a_i,k) B(a_j,k), k ascending */
+      for (int j = 0; j < na; j++) {
+        const double* bj = P->B + (size_t)act_idx[j] * n;
+        for (int k = 0; k < n; k++) Gt[(size_t)k * na + j] = (orc_real)bj[k];
+      }
+      for (int i = 0; i < na; i++) {
+        const double* bi = P->B + (size_t)act_idx[i] * n;
+        orc_real* row = BtB + (size_t)i * na;
+        for (int j = 0; j <= i; j++) row[j] = 0;
+        for (int k = 0; k < n; k++) {
+          const orc_real bik = (orc_real)bi[k];
+          const orc_real* g = Gt + (size_t)k * na;
+          for (int j = 0; j <= i; j++) row[j] += bik * g[j];
         }
+      }
       for (int i = 0; i < na; i++) BtB[(size_t)i * na + i] += (orc_real)P->lambda_min_norm;
       llt_inplace(BtB, na);
       llt_solve_inplace(BtB, na, l);
@@ -802,10 +886,12 @@ int orc_icls_solve(const orc_icls_problem* P, const double* b, double* xout, uin
         }
       }
       if (!isfinite((double)s_min)) {
-        for (int j = 0; j < n; j++) {
-          orc_real s = y_u[j];
-          for (int i = 0; i < na; i++) s += (orc_real)P->B[(size_t)act_idx[i] * n + j] * l[i];
-          x[j] = s;
+        /* y = y_u + B_act^T l: x_j = y_u_j + sum_i B(a_i,j) l_i (i ascending) */
+        for (int j = 0; j < n; j++) x[j] = y_u[j];
+        for (int i = 0; i < na; i++) {
+          const double* bi = P->B + (size_t)act_idx[i] * n;
+          const orc_real li = l[i];
+          for (int j = 0; j < n; j++) x[j] += (orc_real)bi[j] * li;
         }
         break;
       }
@@ -815,12 +901,7 @@ int orc_icls_solve(const orc_icls_problem* P, const double* b, double* xout, uin
     memcpy(lambda_prev, lambda, sizeof(orc_real) * (size_t)ncon);
     ++niter;
     if (!changed || niter > P->max_niter) break;
-    for (int r = 0; r < ncon; r++) {
-      orc_real s = 0;
-      for (int j = 0; j < n; j++) s += (orc_real)P->B[(size_t)r * n + j] * x[j];
-      if (P->t) s -= (orc_real)P->t[r];
-      c[r] = s;
-    }
+    icls_constraints(P, w, x, c);
   }
   /* x = L^-T y (constrained_least_squares.h:355) */
   for (int i = n - 1; i >= 0; i--) {
@@ -830,8 +911,14 @@ int orc_icls_solve(const orc_icls_problem* P, const double* b, double* xout, uin
   }
   for (int j = 0; j < n; j++) xout[j] = (double)x[j];
   if (active_out) memcpy(active_out, active, (size_t)ncon);
-  free(y_u); free(x); free(c); free(c_u); free(lambda); free(lambda_prev); free(l); free(BtB);
-  free(act_idx); free(active);
+  return niter;
+}
+
+int orc_icls_solve(const orc_icls_problem* P, const double* b, double* xout, uint8_t* active_out) {
+  icls_ws w;
+  icls_ws_alloc(&w, P);
+  const int niter = icls_solve_ws(P, &w, b, xout, active_out);
+  icls_ws_free(&w);
   return niter;
 }
 
@@ -845,6 +932,8 @@ static void* icls_worker(void* arg) {
   const int nm = J->p->nmeas, n = J->p->n;
   double* b = (double*)malloc(sizeof(double) * (size_t)nm);
   double* x = (double*)malloc(sizeof(double) * (size_t)n);
+  icls_ws w;
+  icls_ws_alloc(&w, J->p);
   const size_t chunk = 128;
   for (;;) {
     pthread_mutex_lock(&J->mtx);
@@ -859,11 +948,12 @@ static void* icls_worker(void* arg) {
         continue;
       }
       for (int r = 0; r < nm; r++) b[r] = J->b[v * nm + r];
-      int it = orc_icls_solve(J->p, b, x, NULL);
+      int it = icls_solve_ws(J->p, &w, b, x, NULL);
       for (int j = 0; j < n; j++) J->x[v * n + j] = (float)x[j];
       if (J->iters) J->iters[v] = (int16_t)it;
     }
   }
+  icls_ws_free(&w);
   free(b); free(x);
   return NULL;
 }
@@ -958,4 +1048,36 @@ int orc_msmt_build(const double* grad, int nvol, const int* shell_of_vol, int ns
   *ncols = nparams; *nrows_A = Mr;
   free(dirs); free(SHT); free(HR);
   return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Single-thread timing of the two dense kernels of CSD::iterate (rank-k update and LLT)        */
+/* ------------------------------------------------------------------------------------------ */
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+int orc_time_syrk_llt(const orc_csd_shared* s, int k, int reps, double* t_syrk, double* t_llt) {
+  if (!s || k < 0 || k > s->n_hr || reps <= 0) return -1;
+  csd_work w;
+  work_alloc(&w, s);
+  for (int q = 0; q < k; q++) w.neg[q] = (int)(((long)q * s->n_hr) / (k ? k : 1));
+  w.nneg = k;
+  double ts = 0, tl = 0, chk = 0;
+  for (int r = 0; r < reps; r++) {
+    const double a = now_s();
+    csd_form_normal(s, &w);
+    const double b = now_s();
+    llt_inplace(w.W, s->n_sh);
+    const double c = now_s();
+    ts += b - a;
+    tl += c - b;
+    chk += (double)w.W[0];
+  }
+  work_free(&w);
+  *t_syrk = ts;
+  *t_llt = tl;
+  return chk > 0 ? 0 : 1;
 }

@@ -122,6 +122,11 @@ int  orc_msmt_build(const double* grad, int nvol, const int* shell_of_vol, int n
 
 const char* orc_real_name(void);
 
+/* Single-thread timing of the rank-k normal-matrix update (k constraint rows, evenly spaced over the
+ * directions) and the LLT of CSD::iterate, reps times each: total seconds of each into *t_syrk and
+ * *t_llt.  FLOP: n(n+1)k for the update, n^3/3 for the factorisation. */
+int orc_time_syrk_llt(const orc_csd_shared* s, int k, int reps, double* t_syrk, double* t_llt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,14 +68,20 @@ _SIG = {
                                      C.c_int, C.c_void_p, C.c_void_p]),
     "orc_msmt_build": (C.c_int, [D, C.c_int, I, C.c_int, D, I, C.c_int, I, D, C.c_int, D, D, I, I]),
     "orc_real_name": (C.c_char_p, []),
+    "orc_time_syrk_llt": (C.c_int, [C.POINTER(CsdShared), C.c_int, C.c_int, D, D]),
 }
 
 
-def lib(long_double=False):
-    key = "ld" if long_double else "d"
+_FILES = {"d": "liborc.so", "ld": "liborc_ld.so", "o3": "liborc_o3.so"}
+
+
+def lib(long_double=False, generic=False):
+    """The FP64 build (-O3 -march=native), its generic-x86-64 -O3 build (generic=True), or the
+    long double build."""
+    key = "ld" if long_double else ("o3" if generic else "d")
     if key in _libs:
         return _libs[key]
-    path = os.path.join(BUILD, "liborc_ld.so" if long_double else "liborc.so")
+    path = os.path.join(BUILD, _FILES[key])
     if not os.path.exists(path):
         build()
     L = C.CDLL(path)
@@ -167,8 +173,9 @@ class CsdOracle:
     """CSD::Shared::init + per-voxel CSD solves, FP64 (or long double) on the CPU."""
 
     def __init__(self, grad, response, lmax=0, filter=None, directions=None, neg_lambda=1.0,
-                 norm_lambda=1.0, threshold=0.0, niter=50, long_double=False, bzero=10.0, eps=80.0):
-        self.L = lib(long_double)
+                 norm_lambda=1.0, threshold=0.0, niter=50, long_double=False, bzero=10.0, eps=80.0,
+                 generic=False):
+        self.L = lib(long_double, generic)
         g, _ = normalise_grad(grad)
         self.grad = g
         self.vols = csd_select_volumes(g, bzero, eps)
@@ -226,6 +233,16 @@ class CsdOracle:
                                   fod.ctypes.data, iters.ctypes.data)
         return fod, iters
 
+    def time_syrk_llt(self, k, reps=2000):
+        """Single-thread GFLOP/s of the rank-k normal-matrix update (n(n+1)k FLOP) and the LLT
+        (n^3/3 FLOP) as CSD::iterate runs them, k constraint rows."""
+        ts, tl = C.c_double(0), C.c_double(0)
+        if self.L.orc_time_syrk_llt(self.p, int(k), int(reps), C.byref(ts), C.byref(tl)) < 0:
+            raise ValueError("bad timing arguments")
+        n = self.n_sh
+        return {"n": n, "k": int(k), "syrk_gflops": n * (n + 1) * k * reps / ts.value / 1e9,
+                "llt_gflops": n ** 3 / 3.0 * reps / tl.value / 1e9}
+
     def __del__(self):
         p = getattr(self, "p", None)
         if p:
@@ -235,8 +252,8 @@ class CsdOracle:
 
 class IclsOracle:
     def __init__(self, H, A, t=None, num_eq=0, sol_reg=0.0, con_reg=0.0, max_iter=0, tol=0.0,
-                 standard_form=False, long_double=False):
-        self.L = lib(long_double)
+                 standard_form=False, long_double=False, generic=False):
+        self.L = lib(long_double, generic)
         H = _arr(H)
         A = _arr(A)
         tt = _arr(t) if t is not None and np.size(t) else None

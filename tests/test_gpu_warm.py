@@ -118,6 +118,24 @@ def test_fused_inference_matches_unfused(torch_cuda, trained):
     assert torch.equal(neg2, neg)
 
 
+def test_save_load_round_trip(torch_cuda, trained, tmp_path):
+    """A network written by WarmNet.save and read back by WarmNet.load proposes the same sets."""
+    torch = torch_cuda
+    net = trained["net"]
+    path = str(tmp_path / "net.npz")
+    net.save(path)
+    net2 = warm.WarmNet.load(path)
+    assert (net2.n_in, net2.hidden, net2.nsh, net2.nlog) == (net.n_in, net.hidden, net.nsh, net.nlog)
+    assert np.array_equal(net2.get_params(), net.get_params())
+    ph = phantom.single_shell_small(seed=97)
+    dw = torch.from_numpy(np.ascontiguousarray(ph.dw[:2000])).cuda()
+    say("[gpu warm] save/load round trip")
+    F0a, nega = net.infer(dw, trained["dwis"], trained["b0"], want_F0=True)
+    F0b, negb = net2.infer(dw, trained["dwis"], trained["b0"], want_F0=True)
+    torch.cuda.synchronize()
+    assert torch.equal(nega, negb) and torch.equal(F0a, F0b)
+
+
 def test_warm_solve_matches_plain_on_held_out_voxels(torch_cuda, trained):
     torch = torch_cuda
     ph = phantom.single_shell_small(seed=99)   # held out: a different seed of the same protocol
