@@ -262,7 +262,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
     // ---- gather the selected DW signals (dwi2fod.cpp:156-173) ----
     bool bad = valid && !valid[vox];
     const float* src = dw + vox * (size_t)ld_dw;
-    for (int r = tid; r < N; r += NT) {
+    for (int r = opaque_lane(); r < N; r += NT) {
       const float v = src[P.dwis[r]];
       if (!isfinite(v)) bad = true;
       s_b[r] = v < 0.0f ? 0.0f : v;
@@ -295,8 +295,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
     double mtb = 0.0, f_plain = 0.0;
     {
       const int ni = P.n_init;
-      const int rt = row_ok ? tid : 0;
-      const int it = tid < ni ? tid : 0;
+      const int t = opaque_lane();
+      const int rt = t < NSH ? t : 0;
+      const int it = t < ni ? t : 0;
       for (int r0 = 0; r0 < N; r0 += kPipe) {
         double m[kPipe], g[kPipe], bv[kPipe];
 #pragma unroll
@@ -320,7 +321,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
     // buffer in LDS has been reused by then); same order of accumulation as above
     auto plain_start = [&]() __attribute__((always_inline)) -> double {
       const int ni = P.n_init;
-      const int it = tid < ni ? tid : 0;
+      const int t = opaque_lane();
+      const int it = t < ni ? t : 0;
       double acc = 0.0;
       for (int r0 = 0; r0 < N; r0 += kPipe) {
         double g[kPipe], bv[kPipe];
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
 #pragma unroll
         for (int i = 0; i < kPipe; ++i) acc = fma(g[i], bv[i], acc);
       }
-      return tid < ni ? acc : 0.0;
+      return t < ni ? acc : 0.0;
     };
 
     // The negative sets as bitmasks held one 64-bit word per lane (lane w: directions 64w..64w+63),
@@ -351,13 +353,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
       const bool proposal = pass == 0 && warm_neg != nullptr;
       const int limit = pass == 0 ? warm_budget : P.niter;
       if constexpr (WARM) {
-        f = pass == 0 ? ((warm_F0 && row_ok) ? (double)warm_F0[vox * NSH + tid] : 0.0) : plain_start();
+        const int t = opaque_lane();
+        f = pass == 0 ? ((warm_F0 && t < NSH) ? (double)warm_F0[vox * NSH + t] : 0.0) : plain_start();
       } else {
         f = f_plain;
       }
       {
         double A0[NSH];
-        const double* mrow = P.MtM + (row_ok ? tid : 0) * NSH;
+        const int t = opaque_lane();
+        const double* mrow = P.MtM + (t < NSH ? t : 0) * NSH;
 #pragma unroll
         for (int j = 0; j < NSH; ++j) A0[j] = mrow[j];
         wave_sync();
@@ -376,19 +380,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
         if (proposal && n == 0) {
           // the proposed negative set replaces the first amplitude evaluation; only bits of
           // existing directions are honoured
-          if (lane < nwio) {
-            const int left = R - 64 * lane;
+          const int l = opaque_lane() & 63;
+          if (l < nwio) {
+            const int left = R - 64 * l;
             const unsigned long long valid_bits = left >= 64 ? ~0ull : (1ull << left) - 1ull;
-            my_new = warm_neg[vox * nwio + lane] & valid_bits;
+            my_new = warm_neg[vox * nwio + l] & valid_bits;
           }
         } else {
           // ---- amplitudes and the negative set (csd.h:246-250) ----
           // One block of NT directions at a time (thread t: direction b*NT + t): NSH independent
           // loads per block, accumulated in coefficient order, then one ballot per wavefront.
-          s_F[tid] = f;
+          s_F[opaque_lane()] = f;
           wave_sync();
           for (int b = 0; b < nq; ++b) {
-            const double* hb = P.HRt + (size_t)b * (NSH * NT) + tid;
+            const double* hb = P.HRt + (size_t)b * (NSH * NT) + opaque_lane();
             double a = 0.0;
             double hbuf[2][kPipe], fbuf[2][kPipe];
 #pragma unroll
@@ -416,12 +421,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
                 if (j + i < NSH) a = fma(hbuf[kc & 1][i], fbuf[kc & 1][i], a);
               __builtin_amdgcn_sched_barrier(0);
             }
-            const int r = tid + b * NT;
+            const int r = opaque_lane() + b * NT;
             const unsigned long long ball = __ballot(r < R && a < P.thr);
             if ((tid & 63) == 0) s_words[b * NWAVE + wave] = ball;
           }
           wave_sync();
-          my_new = lane < nwords ? s_words[lane] : 0ull;
+          {
+            const int l = opaque_lane() & 63;
+            my_new = l < nwords ? s_words[l] : 0ull;
+          }
           same = have_old && __ballot(my_new != my_old) == 0ull;
         }
         if (same) { converged = true; break; }   // (csd.h:252-253)
@@ -483,8 +491,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
         double* const col1 = s_h + kColStride;
         double rhs = mtb;
         wave_sync();   // earlier readers of the pivot columns are done
-        col0[tid] = A[0];
-        if (NT != 64) col0[NT + tid] = rhs;
+        {
+          const int t = opaque_lane();
+          col0[t] = A[0];
+          if (NT != 64) col0[NT + t] = rhs;
+        }
         wave_sync();
         double di = 1.0;
         gj_sweep<NSH, NT>(A, rhs, di, col0, col1, tid, std::make_integer_sequence<int, NSH>{});
@@ -495,13 +506,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4
       if (P.niter == 0 && pass == 1) converged = true;   // -niter 0: the linear deconvolution
     }
 
-    if (row_ok) fod[vox * NSH + tid] = (float)f;
+    {
+      const int t = opaque_lane();
+      if (t < NSH) fod[vox * NSH + t] = (float)f;
+    }
     if (nneg_out) {
       const unsigned long long last_k = wave_sum((unsigned long long)last_kl);
       if (tid == 0) nneg_out[vox] = (uint16_t)last_k;
     }
     if (tid == 0 && iters_out) iters_out[vox] = (uint16_t)n;
-    if (negmask && tid < nwio) negmask[vox * nwio + tid] = my_old;
+    if (negmask) {
+      const int t = opaque_lane();
+      if (t < nwio) negmask[vox * nwio + t] = my_old;
+    }
     ++st_vox;
     if (converged && P.niter) ++st_conv;
     wave_sync();
