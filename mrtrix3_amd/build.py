@@ -31,6 +31,7 @@ LIB_SOURCES = [
     "kernels/csd_solve.hip",
     "kernels/icls_solve.hip",
     "kernels/mlp.hip",
+    "kernels/peaks.hip",
     "host/sh.cpp",
     "host/dense.cpp",
     "host/gradient.cpp",
@@ -41,24 +42,31 @@ LIB_SOURCES = [
     "capi_csd.cpp",
     "capi_icls.cpp",
     "capi_warm.cpp",
+    "capi_peaks.cpp",
 ]
 
 
-def _gen_dirs_inc():
-    src = os.path.join(PKG, "data", "electrostatic_repulsion_300.txt")
-    dst = os.path.join(CSRC, "host", "dirs300.inc")
+def _gen_inc(data, inc, npairs, origin, suffix=""):
+    src = os.path.join(PKG, "data", data)
+    dst = os.path.join(CSRC, "host", inc)
     vals = []
     for line in open(src):
         line = line.split("#")[0].strip()
         if line:
             vals.extend(line.split())
-    if len(vals) != 600:
-        raise RuntimeError("expected 300 [az el] pairs in %s" % src)
-    text = ("// Generated from mrtrix3_amd/data/electrostatic_repulsion_300.txt by mrtrix3_amd/build.py.\n"
-            "// [azimuth elevation] pairs, MRtrix3 src/dwi/directions/predefined.cpp:92-393 (MPL-2.0).\n")
-    text += "".join("%s, %s,\n" % (vals[i], vals[i + 1]) for i in range(0, 600, 2))
+    if len(vals) != 2 * npairs:
+        raise RuntimeError("expected %d [az el] pairs in %s" % (npairs, src))
+    text = ("// Generated from mrtrix3_amd/data/%s by mrtrix3_amd/build.py.\n"
+            "// [azimuth elevation] pairs, MRtrix3 %s (MPL-2.0).\n" % (data, origin))
+    text += "".join("%s%s, %s%s,\n" % (vals[i], suffix, vals[i + 1], suffix) for i in range(0, 2 * npairs, 2))
     if not os.path.exists(dst) or open(dst).read() != text:
         open(dst, "w").write(text)
+
+
+def _gen_dirs_inc():
+    _gen_inc("electrostatic_repulsion_300.txt", "dirs300.inc", 300, "src/dwi/directions/predefined.cpp:92-393")
+    # double literals converted to float where the table is declared, as the reference's float array is
+    _gen_inc("sh2peaks_seeds_60.txt", "seeds60.inc", 60, "cmd/sh2peaks.cpp:363-424")
 
 
 def _compile(rel):
@@ -87,9 +95,9 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, so)
-    cli = os.path.join(CSRC, "dwi2fod.cpp")
-    if os.path.exists(cli):
-        exe = os.path.join(BIN, "dwi2fod")
+    for name in ("dwi2fod", "sh2peaks"):
+        cli = os.path.join(CSRC, name + ".cpp")
+        exe = os.path.join(BIN, name)
         cmd = [HIPCC] + HIPFLAGS + [cli, os.path.join(CSRC, "host", "mif.cpp"), "-o", exe + ".tmp",
                                     "-L" + LIB, "-lmrcsd",
                                     "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]

@@ -1,0 +1,33 @@
+// Device-side description of a batched sh2peaks run (cmd/sh2peaks.cpp, Math::SH::get_peak).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mrcsd {
+
+constexpr int kPeaksMaxSeeds = 512;     // search directions per voxel
+constexpr int kPeaksMaxAccepted = 64;   // distinct peaks kept per voxel (one per lane)
+constexpr int kPeaksMaxOut = 64;        // reported peaks per voxel (3 values each)
+
+struct PeaksLaunch {
+  const float* sh;          // [nvox][ld_sh]: the first n_sh values of a row are the coefficients
+  int ld_sh, n_sh, lmax;
+  size_t nvox;
+  const float* seeds;       // [nseeds][3] unit search directions (host-made, as Direction(phi, theta))
+  int nseeds;
+  int npeaks;
+  float threshold;
+  const float* match;       // [npeaks][3] (ld_match 0: every voxel) or [nvox][ld_match], or null
+  int ld_match;
+  const uint8_t* mask;      // [nvox] or null
+  float* out;               // [nvox][3 npeaks]
+  unsigned long long* stats;    // [4] or null: voxels searched, Newton iterations, failed searches,
+                                // voxels whose distinct peaks exceeded kPeaksMaxAccepted
+  unsigned long long* counter;  // work counter
+};
+
+hipError_t peaks_launch(const PeaksLaunch& L, hipStream_t stream);
+
+}  // namespace mrcsd

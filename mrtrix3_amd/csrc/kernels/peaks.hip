@@ -1,0 +1,354 @@
+// sh2peaks on gfx950: the peaks of an SH series in every voxel (cmd/sh2peaks.cpp:160-247), each
+// found by Math::SH::get_peak's Gauss-Newton ascent (core/math/SH.h:514-559) from every search
+// direction.
+//
+// One wavefront per voxel, taken from a work counter (persistent grid).  Lane s runs the ascent
+// from search direction s (batches of 64 when there are more), so the wavefront finishes its
+// batch when its slowest search converges.  The accepted peaks are then collected in seed order
+// as the reference does (a peak within acos(0.99) of an accepted one is dropped, then the
+// threshold applies), one accepted peak per lane, and the reported ones are chosen by decreasing
+// amplitude or by closeness to given directions.
+//
+// Arithmetic follows the reference's: single precision, with the steps it takes in double where
+// a double literal or constant enters (Plm_sph_helper, the /2.0, /-2.0, /4.0 of the derivative
+// terms, Math::sqrt2, the 2.0 * daz * del term, the constants of get_peak), and no fused
+// multiply-adds (contraction is off in this file), so results differ from it only through the
+// last bits of the single-precision sin / cos / atan2 / hypot.
+#include <hip/hip_runtime.h>
+#include <limits.h>
+#include <math.h>
+
+#include "peaks.h"
+#include "wave_utils.h"
+
+#pragma clang fp contract(off)
+
+namespace mrcsd {
+namespace {
+
+constexpr double kMaxDirChange = 0.2;    // SH.h:23
+constexpr double kAngleTolerance = 1e-4; // SH.h:24
+constexpr double kDotThreshold = 0.99;   // sh2peaks.cpp:25
+constexpr double kSqrt2 = 1.41421356237309504880;   // Math::sqrt2 (core/math/math.h:43)
+
+constexpr int nforl(int L) { return (L + 1) * (L + 2) / 2; }
+constexpr int nforl_mpos(int L) { return (L / 2 + 1) * (L / 2 + 1); }
+constexpr int sh_index(int l, int m) { return l * (l + 1) / 2 + m; }   // SH.h:52-55
+constexpr int index_mpos(int l, int m) { return l * l / 4 + m; }     // SH.h:64-67
+
+// legendre.h:66-69 with m = M2 / 2 (evaluated in double, as the reference calls it)
+template <int M2>
+__device__ __forceinline__ double plm_helper(double x) {
+  if constexpr (M2 < 1) {
+    return 1.0;
+  } else {
+    return x * ((double)M2 - 1.0) / (double)M2 * plm_helper<M2 - 2>(x);
+  }
+}
+
+__device__ __forceinline__ float fbits(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// legendre.h:129-154 for order M, written into AL at index_mpos(l, M) for even l (SH.h:585-590)
+template <int LMAX, int M>
+__device__ __forceinline__ void plm_order(float (&AL)[nforl_mpos(LMAX)], float x, float sin2) {
+  float buf[LMAX + 1];
+  buf[M] = (float)0.282094791773878;
+  if constexpr (M > 0)
+    buf[M] = (float)((double)buf[M] * sqrt((double)(float)(2 * M + 1) * plm_helper<2 * M>(1.0 * (double)sin2)));
+  if constexpr ((M & 1) != 0) buf[M] = -buf[M];
+  if constexpr (LMAX > M) {
+    float f = sqrtf((float)(2 * M + 3));
+    buf[M + 1] = x * f * buf[M];
+#pragma unroll
+    for (int n = M + 2; n <= LMAX; n++) {
+      buf[n] = x * buf[n - 1] - buf[n - 2] / f;
+      f = sqrtf((float)(4 * n * n - 1) / (float)(n * n - M * M));
+      buf[n] *= f;
+    }
+  }
+  const bool pole = M > 0 && sin2 == 0.0f;   // legendre.h:138-142
+#pragma unroll
+  for (int l = (M & 1) ? M + 1 : M; l <= LMAX; l += 2) AL[index_mpos(l, M)] = pole ? 0.0f : buf[l];
+}
+
+template <int LMAX, int... M>
+__device__ __forceinline__ void plm_all(float (&AL)[nforl_mpos(LMAX)], float x, float sin2,
+                                        std::integer_sequence<int, M...>) {
+  (plm_order<LMAX, M>(AL, x, sin2), ...);
+}
+
+// SH.h:568-647: amplitude and derivatives at (elevation, azimuth); c = the SH coefficients
+template <int LMAX>
+__device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float elevation, float azimuth, float (&out)[6]) {
+  const float sel = sinf(elevation);
+  const float cel = cosf(elevation);
+  const bool atpole = (double)sel < 1e-4;
+  float AL[nforl_mpos(LMAX)];
+  plm_all<LMAX>(AL, cel, sel * sel, std::make_integer_sequence<int, LMAX + 1>{});
+  float dSH_del = 0.0f, dSH_daz = 0.0f, d2SH_del2 = 0.0f, d2SH_deldaz = 0.0f, d2SH_daz2 = 0.0f;
+  float amplitude = c[0] * AL[0];
+#pragma unroll
+  for (int l = 2; l <= LMAX; l += 2) {
+    const float v = c[sh_index(l, 0)];
+    amplitude += v * AL[index_mpos(l, 0)];
+    dSH_del += v * sqrtf((float)(l * (l + 1))) * AL[index_mpos(l, 1)];
+    const float t = v * (sqrtf((float)(l * (l + 1) * (l - 1) * (l + 2))) * AL[index_mpos(l, 2)] -
+                         (float)(l * (l + 1)) * AL[index_mpos(l, 0)]);
+    d2SH_del2 = (float)((double)d2SH_del2 + (double)t / 2.0);
+  }
+#pragma unroll
+  for (int m = 1; m <= LMAX; m++) {
+    const float caz = (float)(kSqrt2 * (double)cosf((float)m * azimuth));
+    const float saz = (float)(kSqrt2 * (double)sinf((float)m * azimuth));
+#pragma unroll
+    for (int l = (m & 1) ? m + 1 : m; l <= LMAX; l += 2) {
+      const float vp = c[sh_index(l, m)];
+      const float vm = c[sh_index(l, -m)];
+      const float al = AL[index_mpos(l, m)];
+      amplitude += (vp * caz + vm * saz) * al;
+      float tmp = sqrtf((float)((l + m) * (l - m + 1))) * AL[index_mpos(l, m - 1)];
+      if (l > m) tmp -= sqrtf((float)((l - m) * (l + m + 1))) * AL[index_mpos(l, m + 1)];
+      tmp = (float)((double)tmp / -2.0);
+      dSH_del += (vp * caz + vm * saz) * tmp;
+      float tmp2 = (float)(-((l + m) * (l - m + 1) + (l - m) * (l + m + 1))) * al;
+      if (m == 1)
+        tmp2 -= sqrtf((float)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * AL[index_mpos(l, 1)];
+      else
+        tmp2 += sqrtf((float)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * AL[index_mpos(l, m >= 2 ? m - 2 : 0)];
+      if (l > m + 1)
+        tmp2 += sqrtf((float)((l - m) * (l + m + 1) * (l - m - 1) * (l + m + 2))) * AL[index_mpos(l, m + 2 <= l ? m + 2 : l)];
+      tmp2 = (float)((double)tmp2 / 4.0);
+      d2SH_del2 += (vp * caz + vm * saz) * tmp2;
+      if (atpole) {
+        dSH_daz += (vm * caz - vp * saz) * tmp;
+      } else {
+        d2SH_deldaz += (float)m * (vm * caz - vp * saz) * tmp;
+        dSH_daz += (float)m * (vm * caz - vp * saz) * al;
+        d2SH_daz2 -= (vp * caz + vm * saz) * (float)m * (float)m * al;
+      }
+    }
+  }
+  if (!atpole) {
+    dSH_daz /= sel;
+    d2SH_deldaz /= sel;
+    d2SH_daz2 /= sel * sel;
+  }
+  out[0] = amplitude;
+  out[1] = dSH_del;
+  out[2] = dSH_daz;
+  out[3] = d2SH_del2;
+  out[4] = d2SH_deldaz;
+  out[5] = d2SH_daz2;
+}
+
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {   // Eigen normalize()
+  const float n2 = x * x + y * y + z * z;
+  if (n2 > 0.0f) {
+    const float s = sqrtf(n2);
+    x /= s;
+    y /= s;
+    z /= s;
+  }
+}
+
+// SH.h:514-559 from (x, y, z); returns the amplitude or NaN (and NaN direction) after 50 iterations
+template <int LMAX>
+__device__ float get_peak(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, int& iters) {
+  for (int i = 0; i < 50; i++) {
+    const float az = atan2f(y, x);
+    const float el = atan2f(hypotf(x, y), z);
+    float d[6];
+    derivatives<LMAX>(c, el, az, d);
+    float del = sqrtf(d[1] * d[1] + d[2] * d[2]);
+    float daz = 0.0f;
+    if (del != 0.0f) {
+      daz = d[2] / del;
+      del = d[1] / del;
+    }
+    const float dSH_dt = daz * d[2] + del * d[1];
+    const float d2SH_dt2 = (float)((double)(daz * daz * d[5]) + 2.0 * (double)daz * (double)del * (double)d[4] +
+                                   (double)(del * del * d[3]));
+    float dt = d2SH_dt2 != 0.0f ? (-dSH_dt / d2SH_dt2) : (float)kMaxDirChange;
+    if (dt < 0.0f) dt = -dt;
+    if ((double)dt > kMaxDirChange) dt = (float)kMaxDirChange;
+    del *= dt;
+    daz *= dt;
+    x += del * cosf(az) * cosf(el) - daz * sinf(az);
+    y += del * sinf(az) * cosf(el) + daz * cosf(az);
+    z -= del * sinf(el);
+    normalize3(x, y, z);
+    if ((double)dt < kAngleTolerance) {
+      iters = i + 1;
+      return d[0];
+    }
+  }
+  iters = 50;
+  x = y = z = NAN;
+  return NAN;
+}
+
+// (value, index) maximum over the wavefront, lower index on ties; every lane gets the result
+__device__ __forceinline__ void wave_argmax(float& v, int& i) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(v, off);
+    const int oi = __shfl_xor(i, off);
+    const bool take = (ov > v) || (ov == v && oi < i);
+    v = take ? ov : v;
+    i = take ? oi : i;
+  }
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(64) void peaks_kernel(PeaksLaunch L) {
+  constexpr int NSH = nforl(LMAX);
+  __shared__ float s_c[128];
+  const int lane = threadIdx.x;
+  const int nout_vals = 3 * L.npeaks;
+  unsigned long long st_vox = 0, st_it = 0, st_fail = 0, st_over = 0;
+
+  for (;;) {
+    unsigned long long vox = 0;
+    if (lane == 0) vox = atomicAdd(L.counter, 1ull);
+    vox = uniform64(vox);
+    if (vox >= L.nvox) break;
+    float* o = L.out + vox * (size_t)nout_vals;
+
+    // ---- input checks: mask (DataLoader, sh2peaks.cpp:126-130) and check_input (:266-286) ----
+    const float* src = L.sh + vox * (size_t)L.ld_sh;
+    const float v0 = lane < L.n_sh ? src[lane] : 0.0f;
+    const float v1 = lane + 64 < L.n_sh ? src[lane + 64] : 0.0f;
+    s_c[lane] = v0;
+    s_c[lane + 64] = v1;
+    const bool has_nan = __ballot(isnan(v0) || isnan(v1)) != 0ull;
+    const bool nonzero = __ballot((lane >= 1 && v0 != 0.0f) || v1 != 0.0f) != 0ull;
+    const float* match = L.match ? L.match + (L.ld_match ? vox * (size_t)L.ld_match : 0) : nullptr;
+    const bool masked = L.mask && !L.mask[vox];
+    const bool match_nan = match && L.ld_match && isnan(match[0]);
+    if (masked || has_nan || !nonzero || match_nan) {
+      for (int k = lane; k < nout_vals; k += 64) o[k] = NAN;
+      continue;
+    }
+    wave_sync();
+    float c[NSH];
+#pragma unroll
+    for (int j = 0; j < NSH; ++j) c[j] = s_c[j];
+    ++st_vox;
+
+    // ---- searches from every direction, accepted in seed order (sh2peaks.cpp:176-189) ----
+    float my_a = NAN, my_x = 0.0f, my_y = 0.0f, my_z = 0.0f;   // accepted peak `lane`
+    int nacc = 0;
+    for (int b0 = 0; b0 < L.nseeds; b0 += 64) {
+      const int s = b0 + lane;
+      float x = 0.0f, y = 0.0f, z = 0.0f, a = NAN;
+      if (s < L.nseeds) {
+        x = L.seeds[3 * s];
+        y = L.seeds[3 * s + 1];
+        z = L.seeds[3 * s + 2];
+        int it = 0;
+        a = get_peak<LMAX>(c, x, y, z, it);
+        st_it += (unsigned long long)it;
+        if (!isfinite(a)) ++st_fail;
+      }
+      const int nb = L.nseeds - b0 < 64 ? L.nseeds - b0 : 64;
+      for (int i = 0; i < nb; ++i) {
+        const float ca = fbits(a, i);
+        if (!isfinite(ca)) continue;
+        const float cx = fbits(x, i), cy = fbits(y, i), cz = fbits(z, i);
+        const bool dup = lane < nacc && lane < kPeaksMaxAccepted &&
+                         (double)fabsf(cx * my_x + cy * my_y + cz * my_z) > kDotThreshold;
+        if (__ballot(dup) != 0ull) continue;
+        if (!(ca >= L.threshold)) continue;
+        if (lane == nacc) {
+          my_a = ca;
+          my_x = cx;
+          my_y = cy;
+          my_z = cz;
+        }
+        ++nacc;
+      }
+    }
+    if (nacc > kPeaksMaxAccepted) {
+      ++st_over;
+      nacc = kPeaksMaxAccepted;
+    }
+
+    // ---- the reported peaks (sh2peaks.cpp:204-246) ----
+    const int nrep = L.npeaks < nacc ? L.npeaks : nacc;
+    unsigned long long used = 0ull;
+    for (int i = 0; i < nrep; ++i) {
+      float key;
+      if (match) {
+        // the accepted peak most nearly parallel to the given direction (|dot| > 0, first on ties)
+        float px = match[3 * i], py = match[3 * i + 1], pz = match[3 * i + 2];
+        normalize3(px, py, pz);
+        key = lane < nacc ? fabsf(px * my_x + py * my_y + pz * my_z) : -1.0f;
+      } else {
+        // decreasing amplitude (partial_sort_copy), first accepted on ties
+        key = (lane < nacc && !((used >> lane) & 1ull)) ? my_a : -INFINITY;
+      }
+      int idx = lane < nacc ? lane : INT_MAX;
+      wave_argmax(key, idx);
+      const int sel = uniform32(idx);
+      const bool ok = sel < nacc && (match ? key > 0.0f : true);
+      if (ok) used |= 1ull << sel;
+      const float a = ok ? fbits(my_a, sel) : NAN;
+      const float px = ok ? fbits(my_x, sel) : NAN, py = ok ? fbits(my_y, sel) : NAN, pz = ok ? fbits(my_z, sel) : NAN;
+      if (lane == 0) o[3 * i] = a * px;
+      if (lane == 1) o[3 * i + 1] = a * py;
+      if (lane == 2) o[3 * i + 2] = a * pz;
+    }
+    for (int k = 3 * nrep + lane; k < nout_vals; k += 64) o[k] = NAN;
+    wave_sync();
+  }
+  if (L.stats) {
+    const unsigned long long it = wave_sum(st_it), fail = wave_sum(st_fail);
+    if (lane == 0) {
+      atomicAdd(L.stats + 0, st_vox);
+      atomicAdd(L.stats + 1, it);
+      atomicAdd(L.stats + 2, fail);
+      atomicAdd(L.stats + 3, st_over);
+    }
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned long long* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+}
+
+template <int LMAX>
+hipError_t launch_lmax(const PeaksLaunch& L, hipStream_t stream) {
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peaks_kernel<LMAX>, 64, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  int dev = 0, ncu = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  size_t grid = (size_t)ncu * (size_t)per_cu;
+  if (grid > L.nvox) grid = L.nvox ? L.nvox : 1;
+  reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
+  peaks_kernel<LMAX><<<dim3((unsigned)grid), dim3(64), 0, stream>>>(L);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t peaks_launch(const PeaksLaunch& L, hipStream_t stream) {
+  if (L.n_sh < nforl(L.lmax) || L.n_sh > 128 || L.nseeds < 1 || L.nseeds > kPeaksMaxSeeds || L.npeaks < 1 ||
+      L.npeaks > kPeaksMaxOut)
+    return hipErrorInvalidValue;
+  switch (L.lmax) {
+    case 2: return launch_lmax<2>(L, stream);
+    case 4: return launch_lmax<4>(L, stream);
+    case 6: return launch_lmax<6>(L, stream);
+    case 8: return launch_lmax<8>(L, stream);
+    case 10: return launch_lmax<10>(L, stream);
+    case 12: return launch_lmax<12>(L, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace mrcsd

@@ -39,6 +39,7 @@ class IclsProblem(C.Structure):
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
+FP = C.POINTER(C.c_float)
 _SIG = {
     "orc_nforl": (C.c_int, [C.c_int]),
     "orc_lforn": (C.c_int, [C.c_int]),
@@ -69,6 +70,12 @@ _SIG = {
     "orc_msmt_build": (C.c_int, [D, C.c_int, I, C.c_int, D, I, C.c_int, I, D, C.c_int, D, D, I, I]),
     "orc_real_name": (C.c_char_p, []),
     "orc_time_syrk_llt": (C.c_int, [C.POINTER(CsdShared), C.c_int, C.c_int, D, D]),
+    "orc_plm_sph_f": (None, [FP, C.c_int, C.c_int, C.c_float, C.c_float]),
+    "orc_sh_derivatives_f": (None, [FP, C.c_int, C.c_float, C.c_float, FP]),
+    "orc_sh_get_peak_f": (C.c_float, [FP, C.c_int, FP, I]),
+    "orc_sh2peaks_voxel": (C.c_int, [FP, C.c_int, C.c_int, FP, C.c_int, C.c_int, C.c_float, FP, FP, I]),
+    "orc_sh2peaks_run": (C.c_longlong, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_int, FP, C.c_int, C.c_int,
+                                        C.c_float, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
 }
 
 
@@ -299,3 +306,66 @@ class IclsOracle:
         if p:
             self.L.orc_icls_problem_free(p)
             self.p = None
+
+
+# ------------------------------------------------------------------------------------------
+# sh2peaks (cmd/sh2peaks.cpp, Math::SH::get_peak): single precision, as the reference computes it
+# ------------------------------------------------------------------------------------------
+def _f(a):
+    return a.ctypes.data_as(FP)
+
+
+def load_peak_seeds():
+    """The 60 built-in search directions of sh2peaks [azimuth, elevation] as float32."""
+    path = os.path.join(os.path.dirname(HERE), "mrtrix3_amd", "data", "sh2peaks_seeds_60.txt")
+    return np.loadtxt(path, dtype=np.float64).astype(np.float32).reshape(-1, 2)
+
+
+def plm_sph_f(lmax, m, x, sin_x):
+    out = np.zeros(lmax + 1, dtype=np.float32)
+    lib().orc_plm_sph_f(_f(out), lmax, m, np.float32(x), np.float32(sin_x))
+    return out
+
+
+def sh_derivatives_f(sh, lmax, el, az):
+    sh = _arr(sh, np.float32)
+    out = np.zeros(6, dtype=np.float32)
+    lib().orc_sh_derivatives_f(_f(sh), lmax, np.float32(el), np.float32(az), _f(out))
+    return out
+
+
+def sh_get_peak_f(sh, lmax, direction):
+    """Returns (amplitude, unit direction, iterations); NaN amplitude when it does not converge."""
+    sh = _arr(sh, np.float32)
+    d = _arr(direction, np.float32).copy()
+    it = C.c_int(0)
+    a = lib().orc_sh_get_peak_f(_f(sh), lmax, _f(d), C.byref(it))
+    return float(a), d, it.value
+
+
+def sh2peaks(sh, lmax=None, seeds=None, npeaks=3, threshold=-np.inf, directions=None, peaks_image=None,
+             mask=None, nthreads=0):
+    """sh [nvox, nsh] float32 -> peaks [nvox, 3 npeaks] (amplitude x direction, NaN past the last).
+    directions: [k, 3] directions reported for every voxel (-direction; npeaks = k);
+    peaks_image: [nvox, 3 npeaks] per-voxel directions (-peaks).  Returns (peaks, Newton iterations)."""
+    sh = _arr(np.atleast_2d(sh), np.float32)
+    nvox, nsh = sh.shape
+    if lmax is None:
+        lmax = lforn(nsh)
+    seeds = load_peak_seeds() if seeds is None else _arr(seeds, np.float32)
+    ld_match = 0
+    m = None
+    if directions is not None:
+        m = _arr(directions, np.float32).reshape(-1)
+        npeaks = m.size // 3
+    elif peaks_image is not None:
+        m = _arr(peaks_image, np.float32)
+        assert m.shape[0] == nvox and m.shape[1] % 3 == 0
+        npeaks = m.shape[1] // 3
+        ld_match = m.shape[1]
+    out = np.empty((nvox, 3 * npeaks), dtype=np.float32)
+    mk = _arr(mask, np.uint8) if mask is not None else None
+    it = lib().orc_sh2peaks_run(sh.ctypes.data, nsh, nsh, nvox, int(lmax), _f(seeds), seeds.shape[0], int(npeaks),
+                                float(threshold), m.ctypes.data if m is not None else None, ld_match,
+                                mk.ctypes.data if mk is not None else None, int(nthreads), out.ctypes.data)
+    return out, int(it)

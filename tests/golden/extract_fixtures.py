@@ -10,6 +10,9 @@ Reads two reference source files *as text* (it never compiles or imports them) a
   ``src/dwi/directions/predefined.cpp:92-393`` as ``azimuth elevation`` rows (radians), the same
   text format ``dwi2fod -directions`` accepts.  This table is an input of the algorithm, so the
   product ships it; it is MRtrix3 data under MPL-2.0 (``LICENCE.txt`` of the reference).
+* ``mrtrix3_amd/data/sh2peaks_seeds_60.txt`` — the 60 built-in search directions of ``sh2peaks``
+  (``cmd/sh2peaks.cpp:363-424``, ``default_directions``) as ``azimuth elevation`` rows, the text
+  format of ``sh2peaks -seeds``; the reference holds them as float.
 
 Usage: ``python tests/golden/extract_fixtures.py [/path/to/reference]``.  Run once; the outputs
 are committed, and nothing at test or run time reads the reference tree.
@@ -69,7 +72,16 @@ def main(ref):
         f.write("# MRtrix3 src/dwi/directions/predefined.cpp:92-393, MPL-2.0\n")
         for az, el in d:
             f.write("%s %s\n" % (repr(float(az)), repr(float(el))))
-    print("wrote icls_golden.npz and electrostatic_repulsion_300.txt")
+    pk = open(os.path.join(ref, "cmd/sh2peaks.cpp")).read()
+    seeds = _array(pk, "default_directions").reshape(-1, 2)
+    assert seeds.shape == (60, 2)
+    path = os.path.join(REPO, "mrtrix3_amd", "data", "sh2peaks_seeds_60.txt")
+    with open(path, "w") as f:
+        f.write("# 60 sh2peaks search directions [ azimuth elevation ] (radians)\n")
+        f.write("# MRtrix3 cmd/sh2peaks.cpp:363-424 (default_directions, float), MPL-2.0\n")
+        for az, el in seeds:
+            f.write("%s %s\n" % (repr(float(az)), repr(float(el))))
+    print("wrote icls_golden.npz, electrostatic_repulsion_300.txt and sh2peaks_seeds_60.txt")
 
 
 if __name__ == "__main__":

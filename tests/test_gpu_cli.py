@@ -96,3 +96,43 @@ def test_cli_msmt_matches_oracle(tmp_path):
     rel = np.array(rel)
     say("[gpu cli] msmt_csd: max rel %.3e over %d voxels" % (rel.max(), rel.size))
     assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_cli_sh2peaks_matches_oracle(tmp_path):
+    """sh2peaks SH peaks -num 2 -mask: .mif in and out (cmd/sh2peaks.cpp), against the oracle."""
+    exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "sh2peaks")
+    assert os.path.exists(exe), "sh2peaks is not built"
+    ph = phantom.single_shell_small()
+    shape = (16, 16, 8)
+    n = int(np.prod(shape))
+    o = orc.CsdOracle(ph.grad, ph.response)
+    fod, _ = o.run_volume(np.ascontiguousarray(ph.dw[:n]), (16, 16, 8), nthreads=8)
+    rng = np.random.default_rng(6)
+    mask = (rng.random(shape[::-1]) < 0.9).astype(np.float32)          # [z, y, x]
+    img = fod.reshape(shape[2], shape[1], shape[0], 45).transpose(2, 1, 0, 3)   # [x, y, z, 45]
+    shp, msk, out = (str(tmp_path / f) for f in ("fod.mif", "mask.mif", "peaks.mif"))
+    mif.write(shp, img, vox=[2.0, 2.0, 2.0])
+    mif.write(msk, mask.transpose(2, 1, 0), vox=[2.0, 2.0, 2.0])
+    say("[gpu cli] sh2peaks on %d voxels" % n)
+    r = subprocess.run([exe, shp, out, "-num", "2", "-mask", msk, "-force"], capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr
+    got, h = mif.read(out)
+    assert got.shape == shape + (6,)
+    ref, _ = orc.sh2peaks(fod, npeaks=2, mask=mask.reshape(-1).astype(np.uint8), nthreads=8)
+    g = got.transpose(2, 1, 0, 3).reshape(n, 6)
+    bad = 0
+    for v in range(n):
+        gv, rv = g[v].reshape(2, 3), ref[v].reshape(2, 3)
+        if not np.array_equal(np.isfinite(gv[:, 0]), np.isfinite(rv[:, 0])):
+            bad += 1
+            continue
+        for i in np.nonzero(np.isfinite(rv[:, 0]))[0]:
+            c = abs(gv[i] @ rv[i]) / (np.linalg.norm(gv[i]) * np.linalg.norm(rv[i]))
+            scale = max(np.linalg.norm(rv[i]), 0.28 * np.linalg.norm(fod[v]))   # the voxel's FOD scale
+            if np.arccos(min(1.0, c)) > 1e-3 or abs(np.linalg.norm(gv[i]) - np.linalg.norm(rv[i])) > 1e-4 * scale:
+                bad += 1
+                break
+    say("[gpu cli] sh2peaks: %d of %d voxels differ" % (bad, n))
+    assert bad <= 0.001 * n + 1
+    assert np.all(np.isnan(g[mask.reshape(-1) == 0]))
