@@ -6,8 +6,9 @@ constraint directions, reference defaults), solved plain (cold start) and with t
 start.  A step is one solve of every masked voxel with the signals already resident in HBM.
 
 Other workloads: --workload super_resolved (same mask, 5 b=0 + 45 b=3000, lmax 12: 91
-coefficients from 45 measurements) and --workload multi_shell (same mask, b=0/1000/2000/3000 with
-10/30/60/90 volumes, three tissues, the ICLS solver behind msmt_csd).
+coefficients from 45 measurements), --workload multi_shell (same mask, b=0/1000/2000/3000 with
+10/30/60/90 volumes, three tissues, the ICLS solver behind msmt_csd) and --workload peaks
+(sh2peaks on the whole-brain FODs the GPU CSD writes: 60 search directions, 3 peaks).
 
 Prints one JSON line with the throughput, the roofline position of the solver kernel (FP64
 algorithmic FLOPs of the reference algorithm, counted from per-voxel iteration statistics the
@@ -29,7 +30,8 @@ METRIC = "dwi2fod voxels/s on 1 MI355X: plain vs BF16+NN warm start at ≤1e-4 S
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
-CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096}
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector)
+CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096, "peaks": 65536}
 
 
 def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
@@ -512,6 +514,161 @@ def bench_msmt(args, torch, ph):
     return out
 
 
+def peaks_flops_per_iteration(L):
+    """Floating-point operations of one Newton iteration of SH::get_peak at lmax L (SH.h:514-647),
+    counted from the restatement's source: +, -, *, / and square roots as one each, and each sin,
+    cos, atan2 or hypot as one (their software cost on the GPU is several times that)."""
+    f = 0
+    for m in range(L + 1):                      # Plm_sph for every order (legendre.h:129-154)
+        f += 1                                  # sin^2
+        if m:
+            f += 4 * m + 3                      # Plm_sph_helper (m levels) and the seed value
+        if L > m:
+            f += 2                              # P_{m+1}
+        f += 4 * max(0, L - m - 1)              # the recursion
+    f += 3                                      # sin, cos of the elevation; c0 * P_00
+    f += 11 * (L // 2)                          # the m = 0 terms
+    for m in range(1, L + 1):
+        f += 6                                  # m az, cos, sin, the two sqrt(2) products
+        for l in range(m + (m & 1), L + 1, 2):
+            f += 30 + (3 if l > m else 0) + (3 if l > m + 1 else 0)
+    f += 4                                      # the divisions by sin(el)
+    f += 3 + 30 + 9                             # atan2, hypot, atan2; the step; normalize
+    return f
+
+
+def cpu_baseline_peaks(fod, gpu_out, sample, lmax):
+    """The oracle's sh2peaks on the host cores (both builds, median of 3) and its agreement with the
+    GPU on the same voxels (direction within 1e-3 rad, amplitude within 1e-4 of the FOD scale)."""
+    from oracle import orc
+    sub = np.ascontiguousarray(fod[:sample])
+    nt = threads()
+    res = {}
+    for name, generic in (("native", False), ("generic", True)):
+        med, runs, (out, _) = median_rate(lambda: orc.sh2peaks(sub, lmax=lmax, nthreads=nt, generic=generic), len(sub))
+        res[name] = (med, runs, out)
+    med, runs, ref = res["native"]
+    g = gpu_out[:sample].reshape(len(sub), -1, 3).astype(np.float64)
+    r = ref.reshape(len(sub), -1, 3).astype(np.float64)
+    scale = 0.28 * np.linalg.norm(sub.astype(np.float64), axis=1)
+    ok = np.ones(len(sub), dtype=bool)
+    for v in range(len(sub)):
+        fg, fr = np.isfinite(g[v, :, 0]), np.isfinite(r[v, :, 0])
+        if not np.array_equal(fg, fr):
+            ok[v] = False
+            continue
+        for i in np.nonzero(fr)[0]:
+            ag, ar = np.linalg.norm(g[v, i]), np.linalg.norm(r[v, i])
+            ang = np.arccos(min(1.0, abs(g[v, i] @ r[v, i]) / (ag * ar)))
+            if ang > 1e-3 or abs(ag - ar) > 1e-4 * max(ar, scale[v]):
+                ok[v] = False
+    return dict(value=med, unit="voxels/s", cores=nt, kind="port",
+                sample="first %d masked voxels of the same FOD volume; oracle C restatement of sh2peaks (single "
+                       "precision as the reference, %d threads), median of 3 runs" % (len(sub), nt),
+                runs=runs, build="gcc -O3 -march=native",
+                generic_build={"build": "gcc -O3 (generic x86-64)", "value": res["generic"][0], "runs": res["generic"][1]},
+                host=host_info()), float(ok.mean())
+
+
+def end_to_end_peaks(ph):
+    """The pipeline as a user runs it: dwi2fod csd writes the FOD image, sh2peaks reads it and writes
+    the peaks image (both child processes, .mif files on disk); each command's wall time and stages."""
+    import shutil
+    import subprocess
+    import tempfile
+    from mrtrix3_amd import mif
+    bindir = os.path.join(ROOT, "mrtrix3_amd", "bin")
+    work = tempfile.mkdtemp(prefix="mrcsd_e2e_")
+    try:
+        nx, ny, nz = ph.shape
+        vol = np.zeros((nz, ny, nx, ph.dw.shape[1]), dtype=np.float32)
+        vol[ph.mask] = ph.dw
+        dwi, msk, rp = (os.path.join(work, f) for f in ("dwi.mif", "mask.mif", "response.txt"))
+        fod, pk = os.path.join(work, "fod.mif"), os.path.join(work, "peaks.mif")
+        mif.write(dwi, vol.transpose(2, 1, 0, 3), vox=[2.0, 2.0, 2.0], keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+        del vol
+        mif.write(msk, ph.mask.transpose(2, 1, 0).astype(np.float32), vox=[2.0, 2.0, 2.0])
+        write_response(rp, ph.response)
+        out = {}
+        for name, cmd in (("dwi2fod", [os.path.join(bindir, "dwi2fod"), "csd", dwi, rp, fod, "-mask", msk]),
+                          ("sh2peaks", [os.path.join(bindir, "sh2peaks"), fod, pk, "-mask", msk])):
+            print("[bench] end to end: %s" % name, file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            r = subprocess.run(cmd + ["-force", "-info"], capture_output=True, text=True, timeout=900)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError("%s failed: %s" % (name, r.stderr[-2000:]))
+            stages = {}
+            for line in r.stderr.splitlines():
+                if "timing:" in line:
+                    for part in line.split("timing:", 1)[1].split(","):
+                        k, sec, unit = part.strip().rsplit(" ", 2)
+                        stages[k] = float(sec)
+            out[name] = {"wall_s": wall, "stages_s": stages}
+        nvox_img = nx * ny * nz
+        out["command"] = "dwi2fod csd dwi.mif response.txt fod.mif -mask mask.mif; sh2peaks fod.mif peaks.mif -mask mask.mif"
+        out["voxels"] = {"masked": ph.nvox, "image": nvox_img}
+        out["note"] = ("sh2peaks visits every voxel of the image, as the reference does (masked-out voxels are "
+                       "written as NaN without a search)")
+        return out
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def bench_peaks(args, torch, ph):
+    """sh2peaks on the whole-brain FODs: the GPU CSD writes the 646,128 FODs into HBM, then a step is
+    one peak search of every voxel (60 search directions, 3 peaks)."""
+    from mrtrix3_amd import csd, peaks
+    dev = torch.device("cuda:0")
+    shared = csd.CSDShared(ph.grad, ph.response)
+    solver = csd.CSD(shared)
+    dw = torch.from_numpy(ph.dw).to(dev)
+    fod, _, _ = solver.run(dw, iters=False, nneg=False)
+    del dw
+    finder = peaks.SH2Peaks(shared.n_sh)
+    out = torch.empty((ph.nvox, 3 * finder.npeaks), dtype=torch.float32, device=dev)
+    stats = torch.zeros(4, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        finder.run(fod, out=out)
+    timer = Timer(torch, torch.cuda.current_stream(dev), args.steps)
+    wall, kern_ms = timer.run(lambda i: finder.run(fod, out=out, stats=stats if i == 0 else None), "peaks")
+    st = stats.cpu().numpy()
+    fl = float(st[1]) * peaks_flops_per_iteration(shared.lmax)
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+    res = {
+        "metric": "sh2peaks voxels/s on 1 MI355X (peaks of the dwi2fod FODs)",
+        "value": ph.nvox * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "sh2peaks on the single_shell_brain FODs (646,128 voxels, lmax 8), 60 search "
+                               "directions, 3 peaks; FODs from the plain GPU CSD, resident in HBM"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "peaks_kernel<8>",
+                     "kernel_ms": kern_ms, "flop_per_launch": fl,
+                     "note": "FP32 VALU-bound (no matrix work); FLOPs counted per Newton iteration from the "
+                             "restatement, transcendental calls as one"},
+        "peaks": {"newton_iterations_per_voxel": float(st[1]) / max(1, st[0]),
+                  "searches_not_converged_frac": float(st[2]) / max(1.0, 60.0 * st[0]),
+                  "flop_per_iteration": peaks_flops_per_iteration(shared.lmax)},
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample or CPU_SAMPLE["peaks"]
+        print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
+        base, agree = cpu_baseline_peaks(fod.cpu().numpy(), out.cpu().numpy(), sample, shared.lmax)
+        res["cpu_baseline"] = base
+        res["parity_on_cpu_sample"] = {"voxels_agreeing": agree}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -551,6 +708,9 @@ def main():
     if args.workload == "multi_shell":
         ph = phantom.multi_shell()
         e2e = None if args.no_e2e else end_to_end_cli(ph, "msmt_csd")
+    elif args.workload == "peaks":
+        ph = phantom.single_shell_brain()
+        e2e = None if args.no_e2e else end_to_end_peaks(ph)
     else:
         make, lmax, _, _ = csd_workload(args.workload)
         ph = make()
@@ -559,6 +719,8 @@ def main():
     import torch
     if args.workload == "multi_shell":
         out = bench_msmt(args, torch, ph)
+    elif args.workload == "peaks":
+        out = bench_peaks(args, torch, ph)
     else:
         out = bench_csd(args, torch, args.workload, ph)
     if e2e is not None:

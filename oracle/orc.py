@@ -344,7 +344,7 @@ def sh_get_peak_f(sh, lmax, direction):
 
 
 def sh2peaks(sh, lmax=None, seeds=None, npeaks=3, threshold=-np.inf, directions=None, peaks_image=None,
-             mask=None, nthreads=0):
+             mask=None, nthreads=0, generic=False):
     """sh [nvox, nsh] float32 -> peaks [nvox, 3 npeaks] (amplitude x direction, NaN past the last).
     directions: [k, 3] directions reported for every voxel (-direction; npeaks = k);
     peaks_image: [nvox, 3 npeaks] per-voxel directions (-peaks).  Returns (peaks, Newton iterations)."""
@@ -365,7 +365,7 @@ def sh2peaks(sh, lmax=None, seeds=None, npeaks=3, threshold=-np.inf, directions=
         ld_match = m.shape[1]
     out = np.empty((nvox, 3 * npeaks), dtype=np.float32)
     mk = _arr(mask, np.uint8) if mask is not None else None
-    it = lib().orc_sh2peaks_run(sh.ctypes.data, nsh, nsh, nvox, int(lmax), _f(seeds), seeds.shape[0], int(npeaks),
+    it = lib(False, generic).orc_sh2peaks_run(sh.ctypes.data, nsh, nsh, nvox, int(lmax), _f(seeds), seeds.shape[0], int(npeaks),
                                 float(threshold), m.ctypes.data if m is not None else None, ld_match,
                                 mk.ctypes.data if mk is not None else None, int(nthreads), out.ctypes.data)
     return out, int(it)
