@@ -34,7 +34,8 @@ struct mrpeaks_ctx {
   bool fixed_dirs = false;
   mrcsd::DeviceBuffer<float> seeds;     // [nseeds][3]
   mrcsd::DeviceBuffer<float> dirs;      // [npeaks][3] (-direction)
-  mrcsd::DeviceBuffer<unsigned long long> counter;
+  mrcsd::DeviceBuffer<unsigned long long> counter;   // [2]
+  mrcsd::DeviceBuffer<float4> work;                  // search results, grown on demand
 };
 
 extern "C" {
@@ -79,7 +80,7 @@ int mrpeaks_create(const mrpeaks_options* opt, mrpeaks_ctx** out) {
       ctx->dirs.upload(d.data(), d.size());
       ctx->fixed_dirs = true;
     }
-    ctx->counter.alloc(1);
+    ctx->counter.alloc(2);
     *out = ctx.release();
     return MRCSD_OK;
   });
@@ -109,6 +110,13 @@ int mrpeaks_run(mrpeaks_ctx* ctx, const float* sh, int ld_sh, const uint8_t* mas
     L.mask = mask; L.out = out;
     L.stats = reinterpret_cast<unsigned long long*>(stats);
     L.counter = ctx->counter.get();
+    const size_t items = mrcsd::peaks_work_items(nvox, ctx->nseeds);
+    if (ctx->work.size() < items) {
+      MRCSD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));   // a previous launch may use it
+      ctx->work.alloc(items);
+    }
+    L.work = ctx->work.get();
+    L.work_items = ctx->work.size();
     MRCSD_HIP(mrcsd::peaks_launch(L, static_cast<hipStream_t>(stream)));
     return MRCSD_OK;
   });

@@ -25,8 +25,18 @@ struct PeaksLaunch {
   float* out;               // [nvox][3 npeaks]
   unsigned long long* stats;    // [4] or null: voxels searched, Newton iterations, failed searches,
                                 // voxels whose distinct peaks exceeded kPeaksMaxAccepted
-  unsigned long long* counter;  // work counter
+  unsigned long long* counter;  // [2] work counters (search queue, voxel queue)
+  float4* work;                 // [work_items] search results: amplitude, x, y, z
+  size_t work_items;            // capacity of `work`; voxels are processed in chunks that fit
 };
+
+// Results of one (voxel, search direction) pair per work item; the buffer a launch needs for nvox
+// voxels at most (the launch processes chunks of work_items / nseeds voxels).
+inline size_t peaks_work_items(size_t nvox, int nseeds) {
+  const size_t cap = size_t(1) << 24;   // 16M searches = 256 MB
+  const size_t all = nvox * size_t(nseeds);
+  return all < cap ? all : (cap / size_t(nseeds)) * size_t(nseeds);
+}
 
 hipError_t peaks_launch(const PeaksLaunch& L, hipStream_t stream);
 

@@ -2,12 +2,16 @@
 // found by Math::SH::get_peak's Gauss-Newton ascent (core/math/SH.h:514-559) from every search
 // direction.
 //
-// One wavefront per voxel, taken from a work counter (persistent grid).  Lane s runs the ascent
-// from search direction s (batches of 64 when there are more), so the wavefront finishes its
-// batch when its slowest search converges.  The accepted peaks are then collected in seed order
-// as the reference does (a peak within acos(0.99) of an accepted one is dropped, then the
-// threshold applies), one accepted peak per lane, and the reported ones are chosen by decreasing
-// amplitude or by closeness to given directions.
+// Two kernels per chunk of voxels:
+//   * peaks_search_kernel: every (voxel, search direction) pair is a work item; each lane takes the
+//     next item as soon as its current search ends (converged, or 50 iterations), so lanes stay busy
+//     whatever the spread of iteration counts (a wavefront per voxel would wait for its slowest
+//     search: about a third of the lanes' time).  Items are handed out with one atomic per
+//     wavefront for all lanes that need one.  Results go to a work buffer: amplitude, direction.
+//   * peaks_select_kernel: one wavefront per voxel reads its searches' results and collects the
+//     accepted peaks in seed order as the reference does (a peak within acos(0.99) of an accepted
+//     one is dropped, then the threshold applies), one accepted peak per lane, then chooses the
+//     reported ones by decreasing amplitude or by closeness to given directions.
 //
 // Arithmetic follows the reference's: single precision, with the steps it takes in double where
 // a double literal or constant enters (Plm_sph_helper, the /2.0, /-2.0, /4.0 of the derivative
@@ -153,40 +157,34 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {   // 
   }
 }
 
-// SH.h:514-559 from (x, y, z); returns the amplitude or NaN (and NaN direction) after 50 iterations
+// One iteration of SH.h:514-559 from (x, y, z): the amplitude at the current direction into amp,
+// the direction updated in place; true when the step was below the angle tolerance (converged).
 template <int LMAX>
-__device__ float get_peak(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, int& iters) {
-  for (int i = 0; i < 50; i++) {
-    const float az = atan2f(y, x);
-    const float el = atan2f(hypotf(x, y), z);
-    float d[6];
-    derivatives<LMAX>(c, el, az, d);
-    float del = sqrtf(d[1] * d[1] + d[2] * d[2]);
-    float daz = 0.0f;
-    if (del != 0.0f) {
-      daz = d[2] / del;
-      del = d[1] / del;
-    }
-    const float dSH_dt = daz * d[2] + del * d[1];
-    const float d2SH_dt2 = (float)((double)(daz * daz * d[5]) + 2.0 * (double)daz * (double)del * (double)d[4] +
-                                   (double)(del * del * d[3]));
-    float dt = d2SH_dt2 != 0.0f ? (-dSH_dt / d2SH_dt2) : (float)kMaxDirChange;
-    if (dt < 0.0f) dt = -dt;
-    if ((double)dt > kMaxDirChange) dt = (float)kMaxDirChange;
-    del *= dt;
-    daz *= dt;
-    x += del * cosf(az) * cosf(el) - daz * sinf(az);
-    y += del * sinf(az) * cosf(el) + daz * cosf(az);
-    z -= del * sinf(el);
-    normalize3(x, y, z);
-    if ((double)dt < kAngleTolerance) {
-      iters = i + 1;
-      return d[0];
-    }
+__device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, float& amp) {
+  const float az = atan2f(y, x);
+  const float el = atan2f(hypotf(x, y), z);
+  float d[6];
+  derivatives<LMAX>(c, el, az, d);
+  float del = sqrtf(d[1] * d[1] + d[2] * d[2]);
+  float daz = 0.0f;
+  if (del != 0.0f) {
+    daz = d[2] / del;
+    del = d[1] / del;
   }
-  iters = 50;
-  x = y = z = NAN;
-  return NAN;
+  const float dSH_dt = daz * d[2] + del * d[1];
+  const float d2SH_dt2 = (float)((double)(daz * daz * d[5]) + 2.0 * (double)daz * (double)del * (double)d[4] +
+                                 (double)(del * del * d[3]));
+  float dt = d2SH_dt2 != 0.0f ? (-dSH_dt / d2SH_dt2) : (float)kMaxDirChange;
+  if (dt < 0.0f) dt = -dt;
+  if ((double)dt > kMaxDirChange) dt = (float)kMaxDirChange;
+  del *= dt;
+  daz *= dt;
+  x += del * cosf(az) * cosf(el) - daz * sinf(az);
+  y += del * sinf(az) * cosf(el) + daz * cosf(az);
+  z -= del * sinf(el);
+  normalize3(x, y, z);
+  amp = d[0];
+  return (double)dt < kAngleTolerance;
 }
 
 // (value, index) maximum over the wavefront, lower index on ties; every lane gets the result
@@ -201,29 +199,106 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
   }
 }
 
+constexpr int kRefill = 32;
+
+// Searches of voxels [v0, v0 + nv): item g = (v0 + g / nseeds, seed g % nseeds), result in work[g].
 template <int LMAX>
-__global__ __launch_bounds__(64) void peaks_kernel(PeaksLaunch L) {
+__global__ __launch_bounds__(64) void peaks_search_kernel(PeaksLaunch L, size_t v0, size_t nv) {
   constexpr int NSH = nforl(LMAX);
+  const int lane = threadIdx.x;
+  const size_t nitems = nv * (size_t)L.nseeds;
+  float c[NSH];
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  int it = 0;
+  bool busy = false, exhausted = false;
+  size_t item = 0;
+  unsigned long long st_it = 0, st_fail = 0;
+  for (;;) {
+    // lanes without a search take the next items, one atomic per wavefront, once at least
+    // kRefill lanes are idle (or all are): fewer rounds of coefficient loads for a short idle wait
+    const unsigned long long need = __ballot(!busy);
+    if (!exhausted && (__popcll(need) >= kRefill || need == ~0ull)) {
+      const int first = __ffsll((long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == first) base = atomicAdd(L.counter, (unsigned long long)__popcll(need));
+      base = readlane64(base, first);
+      if (base >= nitems) exhausted = true;
+      if (!busy) {
+        const unsigned long long below = lane ? (need & (~0ull >> (64 - lane))) : 0ull;
+        item = (size_t)(base + (unsigned long long)__popcll(below));
+        if (item < nitems) {
+          const size_t vox = v0 + item / (size_t)L.nseeds;
+          const int s = (int)(item % (size_t)L.nseeds);
+          const float* src = L.sh + vox * (size_t)L.ld_sh;
+          bool bad = L.mask && !L.mask[vox];
+          bool nonzero = false;
+#pragma unroll
+          for (int j = 0; j < NSH; ++j) {
+            c[j] = src[j];
+            bad = bad || isnan(c[j]);
+            nonzero = nonzero || (j > 0 && c[j] != 0.0f);
+          }
+          if (L.match && L.ld_match && isnan(L.match[vox * (size_t)L.ld_match])) bad = true;
+          if (bad || !nonzero) {
+            L.work[item] = make_float4(NAN, NAN, NAN, NAN);   // the voxel is reported as NaN
+          } else {
+            x = L.seeds[3 * s];
+            y = L.seeds[3 * s + 1];
+            z = L.seeds[3 * s + 2];
+            it = 0;
+            busy = true;
+          }
+        }
+      }
+    }
+    if (__ballot(busy) == 0ull) {
+      if (exhausted) break;
+      continue;
+    }
+    if (busy) {
+      float amp;
+      const bool conv = newton_step<LMAX>(c, x, y, z, amp);
+      ++it;
+      if (conv || it == 50) {
+        L.work[item] = conv ? make_float4(amp, x, y, z) : make_float4(NAN, NAN, NAN, NAN);
+        st_it += (unsigned long long)it;
+        if (!conv) ++st_fail;
+        busy = false;
+      }
+    }
+  }
+  if (L.stats) {
+    const unsigned long long itsum = wave_sum(st_it), fail = wave_sum(st_fail);
+    if (lane == 0) {
+      atomicAdd(L.stats + 1, itsum);
+      atomicAdd(L.stats + 2, fail);
+    }
+  }
+}
+
+// Accepted and reported peaks of voxels [v0, v0 + nv) from their searches' results.
+__global__ __launch_bounds__(64) void peaks_select_kernel(PeaksLaunch L, size_t v0, size_t nv) {
   __shared__ float s_c[128];
   const int lane = threadIdx.x;
   const int nout_vals = 3 * L.npeaks;
-  unsigned long long st_vox = 0, st_it = 0, st_fail = 0, st_over = 0;
+  unsigned long long st_vox = 0, st_over = 0;
 
   for (;;) {
-    unsigned long long vox = 0;
-    if (lane == 0) vox = atomicAdd(L.counter, 1ull);
-    vox = uniform64(vox);
-    if (vox >= L.nvox) break;
+    unsigned long long t = 0;
+    if (lane == 0) t = atomicAdd(L.counter + 1, 1ull);
+    t = uniform64(t);
+    if (t >= nv) break;
+    const size_t vox = v0 + (size_t)t;
     float* o = L.out + vox * (size_t)nout_vals;
 
     // ---- input checks: mask (DataLoader, sh2peaks.cpp:126-130) and check_input (:266-286) ----
     const float* src = L.sh + vox * (size_t)L.ld_sh;
-    const float v0 = lane < L.n_sh ? src[lane] : 0.0f;
-    const float v1 = lane + 64 < L.n_sh ? src[lane + 64] : 0.0f;
-    s_c[lane] = v0;
-    s_c[lane + 64] = v1;
-    const bool has_nan = __ballot(isnan(v0) || isnan(v1)) != 0ull;
-    const bool nonzero = __ballot((lane >= 1 && v0 != 0.0f) || v1 != 0.0f) != 0ull;
+    const float v0c = lane < L.n_sh ? src[lane] : 0.0f;
+    const float v1c = lane + 64 < L.n_sh ? src[lane + 64] : 0.0f;
+    s_c[lane] = v0c;
+    s_c[lane + 64] = v1c;
+    const bool has_nan = __ballot(isnan(v0c) || isnan(v1c)) != 0ull;
+    const bool nonzero = __ballot((lane >= 1 && v0c != 0.0f) || v1c != 0.0f) != 0ull;
     const float* match = L.match ? L.match + (L.ld_match ? vox * (size_t)L.ld_match : 0) : nullptr;
     const bool masked = L.mask && !L.mask[vox];
     const bool match_nan = match && L.ld_match && isnan(match[0]);
@@ -231,34 +306,21 @@ __global__ __launch_bounds__(64) void peaks_kernel(PeaksLaunch L) {
       for (int k = lane; k < nout_vals; k += 64) o[k] = NAN;
       continue;
     }
-    wave_sync();
-    float c[NSH];
-#pragma unroll
-    for (int j = 0; j < NSH; ++j) c[j] = s_c[j];
     ++st_vox;
 
-    // ---- searches from every direction, accepted in seed order (sh2peaks.cpp:176-189) ----
+    // ---- the searches' results, accepted in seed order (sh2peaks.cpp:176-189) ----
+    const float4* res = L.work + (size_t)t * (size_t)L.nseeds;
     float my_a = NAN, my_x = 0.0f, my_y = 0.0f, my_z = 0.0f;   // accepted peak `lane`
     int nacc = 0;
     for (int b0 = 0; b0 < L.nseeds; b0 += 64) {
       const int s = b0 + lane;
-      float x = 0.0f, y = 0.0f, z = 0.0f, a = NAN;
-      if (s < L.nseeds) {
-        x = L.seeds[3 * s];
-        y = L.seeds[3 * s + 1];
-        z = L.seeds[3 * s + 2];
-        int it = 0;
-        a = get_peak<LMAX>(c, x, y, z, it);
-        st_it += (unsigned long long)it;
-        if (!isfinite(a)) ++st_fail;
-      }
+      const float4 r = s < L.nseeds ? res[s] : make_float4(NAN, NAN, NAN, NAN);
       const int nb = L.nseeds - b0 < 64 ? L.nseeds - b0 : 64;
       for (int i = 0; i < nb; ++i) {
-        const float ca = fbits(a, i);
+        const float ca = fbits(r.x, i);
         if (!isfinite(ca)) continue;
-        const float cx = fbits(x, i), cy = fbits(y, i), cz = fbits(z, i);
-        const bool dup = lane < nacc && lane < kPeaksMaxAccepted &&
-                         (double)fabsf(cx * my_x + cy * my_y + cz * my_z) > kDotThreshold;
+        const float cx = fbits(r.y, i), cy = fbits(r.z, i), cz = fbits(r.w, i);
+        const bool dup = lane < nacc && (double)fabsf(cx * my_x + cy * my_y + cz * my_z) > kDotThreshold;
         if (__ballot(dup) != 0ull) continue;
         if (!(ca >= L.threshold)) continue;
         if (lane == nacc) {
@@ -301,44 +363,53 @@ __global__ __launch_bounds__(64) void peaks_kernel(PeaksLaunch L) {
       if (lane == 2) o[3 * i + 2] = a * pz;
     }
     for (int k = 3 * nrep + lane; k < nout_vals; k += 64) o[k] = NAN;
-    wave_sync();
   }
-  if (L.stats) {
-    const unsigned long long it = wave_sum(st_it), fail = wave_sum(st_fail);
-    if (lane == 0) {
-      atomicAdd(L.stats + 0, st_vox);
-      atomicAdd(L.stats + 1, it);
-      atomicAdd(L.stats + 2, fail);
-      atomicAdd(L.stats + 3, st_over);
-    }
+  if (L.stats && lane == 0) {
+    atomicAdd(L.stats + 0, st_vox);
+    atomicAdd(L.stats + 3, st_over);
   }
 }
 
-__global__ void reset_counter_kernel(unsigned long long* c) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+__global__ void reset_counters_kernel(unsigned long long* c) {
+  if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0ull;
 }
 
-template <int LMAX>
-hipError_t launch_lmax(const PeaksLaunch& L, hipStream_t stream) {
+template <class K>
+hipError_t persistent_grid(K kern, size_t work, size_t* grid) {
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peaks_kernel<LMAX>, 64, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   int dev = 0, ncu = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-  size_t grid = (size_t)ncu * (size_t)per_cu;
-  if (grid > L.nvox) grid = L.nvox ? L.nvox : 1;
-  reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
-  peaks_kernel<LMAX><<<dim3((unsigned)grid), dim3(64), 0, stream>>>(L);
-  return hipGetLastError();
+  size_t g = (size_t)ncu * (size_t)per_cu;
+  *grid = g < work ? g : (work ? work : 1);
+  return hipSuccess;
+}
+
+template <int LMAX>
+hipError_t launch_lmax(const PeaksLaunch& L, hipStream_t stream) {
+  const size_t chunk = L.work_items / (size_t)L.nseeds;
+  for (size_t v0 = 0; v0 < L.nvox; v0 += chunk) {
+    const size_t nv = L.nvox - v0 < chunk ? L.nvox - v0 : chunk;
+    size_t g1 = 0, g2 = 0;
+    hipError_t e = persistent_grid(peaks_search_kernel<LMAX>, nv * (size_t)L.nseeds / 64 + 1, &g1);
+    if (e != hipSuccess) return e;
+    if ((e = persistent_grid(peaks_select_kernel, nv, &g2)) != hipSuccess) return e;
+    reset_counters_kernel<<<1, 64, 0, stream>>>(L.counter);
+    peaks_search_kernel<LMAX><<<dim3((unsigned)g1), dim3(64), 0, stream>>>(L, v0, nv);
+    peaks_select_kernel<<<dim3((unsigned)g2), dim3(64), 0, stream>>>(L, v0, nv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace
 
 hipError_t peaks_launch(const PeaksLaunch& L, hipStream_t stream) {
   if (L.n_sh < nforl(L.lmax) || L.n_sh > 128 || L.nseeds < 1 || L.nseeds > kPeaksMaxSeeds || L.npeaks < 1 ||
-      L.npeaks > kPeaksMaxOut)
+      L.npeaks > kPeaksMaxOut || !L.work || L.work_items < (size_t)L.nseeds)
     return hipErrorInvalidValue;
   switch (L.lmax) {
     case 2: return launch_lmax<2>(L, stream);
