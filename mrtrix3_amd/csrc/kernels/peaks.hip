@@ -16,8 +16,9 @@
 // Arithmetic follows the reference's: single precision, with the steps it takes in double where
 // a double literal or constant enters (Plm_sph_helper, the /2.0, /-2.0, /4.0 of the derivative
 // terms, Math::sqrt2, the 2.0 * daz * del term, the constants of get_peak), and no fused
-// multiply-adds (contraction is off in this file), so results differ from it only through the
-// last bits of the single-precision sin / cos / atan2 / hypot.
+// multiply-adds (contraction is off in this file).  Results differ from it through the last bits
+// of the single-precision sin / cos / atan2 / hypot and of two cheaper evaluations (the
+// Legendre seed helper as a power times a constant, cos / sin(m az) by angle addition).
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
@@ -40,13 +41,16 @@ constexpr int nforl_mpos(int L) { return (L / 2 + 1) * (L / 2 + 1); }
 constexpr int sh_index(int l, int m) { return l * (l + 1) / 2 + m; }   // SH.h:52-55
 constexpr int index_mpos(int l, int m) { return l * l / 4 + m; }     // SH.h:64-67
 
-// legendre.h:66-69 with m = M2 / 2 (evaluated in double, as the reference calls it)
+// legendre.h:66-69 with m = M2 / 2, in double as the reference calls it, with each level's
+// (M2 - 1) / M2 folded into one constant at compile time instead of a division per level (the
+// result differs from the reference's in the last bits of the double only)
 template <int M2>
 __device__ __forceinline__ double plm_helper(double x) {
   if constexpr (M2 < 1) {
     return 1.0;
   } else {
-    return x * ((double)M2 - 1.0) / (double)M2 * plm_helper<M2 - 2>(x);
+    constexpr double f = ((double)M2 - 1.0) / (double)M2;
+    return x * f * plm_helper<M2 - 2>(x);
   }
 }
 
@@ -83,11 +87,21 @@ __device__ __forceinline__ void plm_all(float (&AL)[nforl_mpos(LMAX)], float x, 
   (plm_order<LMAX, M>(AL, x, sin2), ...);
 }
 
-// SH.h:568-647: amplitude and derivatives at (elevation, azimuth); c = the SH coefficients
+// SH.h:568-647: amplitude and derivatives at (elevation, azimuth); c = the SH coefficients.
+// trig returns sin / cos of the elevation and of the azimuth for the caller's update step.
+// cos(m az), sin(m az) for m >= 2 come from the m = 1 values by angle addition in double (a few
+// float ulps from the directly evaluated values at m = 8) instead of sixteen sin / cos calls.
 template <int LMAX>
-__device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float elevation, float azimuth, float (&out)[6]) {
+__device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float elevation, float azimuth, float (&out)[6],
+                                            float (&trig)[4]) {
   const float sel = sinf(elevation);
   const float cel = cosf(elevation);
+  const float caz1 = cosf(azimuth), saz1 = sinf(azimuth);
+  trig[0] = sel;
+  trig[1] = cel;
+  trig[2] = saz1;
+  trig[3] = caz1;
+  double cm = (double)caz1, sm = (double)saz1;   // cos(m az), sin(m az) at m = 1
   const bool atpole = (double)sel < 1e-4;
   float AL[nforl_mpos(LMAX)];
   plm_all<LMAX>(AL, cel, sel * sel, std::make_integer_sequence<int, LMAX + 1>{});
@@ -104,8 +118,13 @@ __device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float
   }
 #pragma unroll
   for (int m = 1; m <= LMAX; m++) {
-    const float caz = (float)(kSqrt2 * (double)cosf((float)m * azimuth));
-    const float saz = (float)(kSqrt2 * (double)sinf((float)m * azimuth));
+    if (m > 1) {
+      const double cn = cm * (double)caz1 - sm * (double)saz1;
+      sm = sm * (double)caz1 + cm * (double)saz1;
+      cm = cn;
+    }
+    const float caz = (float)(kSqrt2 * cm);
+    const float saz = (float)(kSqrt2 * sm);
 #pragma unroll
     for (int l = (m & 1) ? m + 1 : m; l <= LMAX; l += 2) {
       const float vp = c[sh_index(l, m)];
@@ -163,8 +182,8 @@ template <int LMAX>
 __device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, float& amp) {
   const float az = atan2f(y, x);
   const float el = atan2f(hypotf(x, y), z);
-  float d[6];
-  derivatives<LMAX>(c, el, az, d);
+  float d[6], t[4];   // t: sin el, cos el, sin az, cos az (the same calls the update would make)
+  derivatives<LMAX>(c, el, az, d, t);
   float del = sqrtf(d[1] * d[1] + d[2] * d[2]);
   float daz = 0.0f;
   if (del != 0.0f) {
@@ -179,9 +198,9 @@ __device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float
   if ((double)dt > kMaxDirChange) dt = (float)kMaxDirChange;
   del *= dt;
   daz *= dt;
-  x += del * cosf(az) * cosf(el) - daz * sinf(az);
-  y += del * sinf(az) * cosf(el) + daz * cosf(az);
-  z -= del * sinf(el);
+  x += del * t[3] * t[1] - daz * t[2];
+  y += del * t[2] * t[1] + daz * t[3];
+  z -= del * t[0];
   normalize3(x, y, z);
   amp = d[0];
   return (double)dt < kAngleTolerance;
