@@ -16,9 +16,10 @@
 // Arithmetic follows the reference's: single precision, with the steps it takes in double where
 // a double literal or constant enters (Plm_sph_helper, the /2.0, /-2.0, /4.0 of the derivative
 // terms, Math::sqrt2, the 2.0 * daz * del term, the constants of get_peak), and no fused
-// multiply-adds (contraction is off in this file).  Results differ from it through the last bits
-// of the single-precision sin / cos / atan2 / hypot and of two cheaper evaluations (the
-// Legendre seed helper as a power times a constant, cos / sin(m az) by angle addition).
+// multiply-adds (contraction is off in this file).  Results differ from it in the last bits
+// through cheaper evaluations of the same quantities: the sines and cosines of the direction's
+// angles from its components (not atan2 then sin / cos), cos / sin(m az) by angle addition, the
+// Legendre seed helper with folded constants and its square root in single precision.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <math.h>
@@ -63,8 +64,8 @@ template <int LMAX, int M>
 __device__ __forceinline__ void plm_order(float (&AL)[nforl_mpos(LMAX)], float x, float sin2) {
   float buf[LMAX + 1];
   buf[M] = (float)0.282094791773878;
-  if constexpr (M > 0)
-    buf[M] = (float)((double)buf[M] * sqrt((double)(float)(2 * M + 1) * plm_helper<2 * M>(1.0 * (double)sin2)));
+  if constexpr (M > 0)   // the product in double as the reference forms it, its root in single precision
+    buf[M] = buf[M] * sqrtf((float)((double)(float)(2 * M + 1) * plm_helper<2 * M>(1.0 * (double)sin2)));
   if constexpr ((M & 1) != 0) buf[M] = -buf[M];
   if constexpr (LMAX > M) {
     float f = sqrtf((float)(2 * M + 3));
@@ -87,20 +88,13 @@ __device__ __forceinline__ void plm_all(float (&AL)[nforl_mpos(LMAX)], float x, 
   (plm_order<LMAX, M>(AL, x, sin2), ...);
 }
 
-// SH.h:568-647: amplitude and derivatives at (elevation, azimuth); c = the SH coefficients.
-// trig returns sin / cos of the elevation and of the azimuth for the caller's update step.
-// cos(m az), sin(m az) for m >= 2 come from the m = 1 values by angle addition in double (a few
-// float ulps from the directly evaluated values at m = 8) instead of sixteen sin / cos calls.
+// SH.h:568-647: amplitude and derivatives at the direction whose elevation and azimuth have the
+// given sines and cosines; c = the SH coefficients.  cos(m az), sin(m az) for m >= 2 come from the
+// m = 1 values by angle addition in double (a few float ulps from the directly evaluated values at
+// m = 8) instead of sixteen sin / cos calls.
 template <int LMAX>
-__device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float elevation, float azimuth, float (&out)[6],
-                                            float (&trig)[4]) {
-  const float sel = sinf(elevation);
-  const float cel = cosf(elevation);
-  const float caz1 = cosf(azimuth), saz1 = sinf(azimuth);
-  trig[0] = sel;
-  trig[1] = cel;
-  trig[2] = saz1;
-  trig[3] = caz1;
+__device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float sel, float cel, float saz1, float caz1,
+                                            float (&out)[6]) {
   double cm = (double)caz1, sm = (double)saz1;   // cos(m az), sin(m az) at m = 1
   const bool atpole = (double)sel < 1e-4;
   float AL[nforl_mpos(LMAX)];
@@ -180,10 +174,13 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {   // 
 // the direction updated in place; true when the step was below the angle tolerance (converged).
 template <int LMAX>
 __device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, float& amp) {
-  const float az = atan2f(y, x);
-  const float el = atan2f(hypotf(x, y), z);
-  float d[6], t[4];   // t: sin el, cos el, sin az, cos az (the same calls the update would make)
-  derivatives<LMAX>(c, el, az, d, t);
+  // the reference takes az = atan2(y, x), el = atan2(hypot(x, y), z) and their sines and cosines;
+  // for the unit direction these are hypot(x, y), z, y / hypot and x / hypot (az = 0 at the pole)
+  const float rho = hypotf(x, y);
+  const float sel = rho, cel = z;
+  const float caz = rho > 0.0f ? x / rho : 1.0f, saz = rho > 0.0f ? y / rho : 0.0f;
+  float d[6];
+  derivatives<LMAX>(c, sel, cel, saz, caz, d);
   float del = sqrtf(d[1] * d[1] + d[2] * d[2]);
   float daz = 0.0f;
   if (del != 0.0f) {
@@ -198,9 +195,9 @@ __device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float
   if ((double)dt > kMaxDirChange) dt = (float)kMaxDirChange;
   del *= dt;
   daz *= dt;
-  x += del * t[3] * t[1] - daz * t[2];
-  y += del * t[2] * t[1] + daz * t[3];
-  z -= del * t[0];
+  x += del * caz * cel - daz * saz;
+  y += del * saz * cel + daz * caz;
+  z -= del * sel;
   normalize3(x, y, z);
   amp = d[0];
   return (double)dt < kAngleTolerance;
@@ -221,8 +218,10 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
 constexpr int kRefill = 32;
 
 // Searches of voxels [v0, v0 + nv): item g = (v0 + g / nseeds, seed g % nseeds), result in work[g].
+// Up to lmax 8 the kernel is held to 168 registers (three wavefronts per SIMD, a few spills), which
+// beats two wavefronts without spills; at lmax 10 two fit, at lmax 12 one.
 template <int LMAX>
-__global__ __launch_bounds__(64) void peaks_search_kernel(PeaksLaunch L, size_t v0, size_t nv) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 3 : (LMAX == 10 ? 2 : 1)))) void peaks_search_kernel(PeaksLaunch L, size_t v0, size_t nv) {
   constexpr int NSH = nforl(LMAX);
   const int lane = threadIdx.x;
   const size_t nitems = nv * (size_t)L.nseeds;
