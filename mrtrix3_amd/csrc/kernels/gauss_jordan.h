@@ -1,4 +1,4 @@
-// Gauss-Jordan elimination with one matrix row per thread, shared by the CSD and ICLS kernels
+// Gauss-Jordan elimination with one matrix row per thread, as the CSD kernel solves its normal equations
 // (gfx950, wavefront size 64).  Thread i owns row i of a symmetric positive definite N x N system
 // [A | rhs]; after the sweep A is diagonal and x_i = rhs_i / di.
 #pragma once
@@ -49,7 +49,7 @@ constexpr int kPipe = 4;   // elements per software-pipelined load chunk
 // row but KK.  The pivot row past the diagonal equals column KK (symmetric trailing block), read
 // from `cur`; the next step's column is updated first and published to `nxt` (look-ahead) so its
 // LDS round trip overlaps the remaining FMAs.  The first chunk of the pivot column is in flight
-// during the division.  A step has no branches (the divide runs on every lane and is masked;
+// during the reciprocal.  A step has no branches (the reciprocal runs on every lane and is masked;
 // every lane publishes): a conditional block per step lets the compiler sink each step's FMAs
 // into the next one, which keeps all the loaded pivot columns live and spills.  The pivot row's
 // diagonal is taken into `di` at its own step, so columns already eliminated are dead afterwards
@@ -75,9 +75,14 @@ __device__ __forceinline__ void gj_step(double (&A)[NSH], double& rhs, double& d
     for (int i = 0; i < kPipe; ++i)
       if (J0 + i < NSH) buf[0][i] = p[i];
   }
-  __builtin_amdgcn_sched_barrier(0);   // the first chunk's loads are issued before the divide
+  __builtin_amdgcn_sched_barrier(0);   // the first chunk's loads are issued before the reciprocal
   const int t = opaque_lane();   // per-step lane masks are recomputed, not hoisted (one SGPR pair each)
-  const double q = A[KK] / d;
+  // A[KK] / d through the pivot's reciprocal: v_rcp_f64 refined by two Newton steps (about one ulp;
+  // a correctly rounded division is a longer sequence, and every lane executes it)
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  const double q = A[KK] * r;
   const double c = (t != KK) ? q : 0.0;
   di = (t == KK) ? A[KK] : di;
   const double rn = fma(-c, bk, rhs);
