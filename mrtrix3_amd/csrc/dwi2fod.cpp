@@ -5,7 +5,7 @@
 //   dwi2fod msmt_csd dwi.mif wm.txt wm.mif gm.txt gm.mif csf.txt csf.mif [options]
 //
 // Options kept from the reference: -mask, -lmax, -directions, -filter, -neg_lambda,
-// -norm_lambda, -threshold, -niter, -grad, -bvalue_scaling, -predicted_signal, -force, -quiet,
+// -norm_lambda, -threshold, -niter, -grad, -fslgrad, -bvalue_scaling, -predicted_signal, -force, -quiet,
 // -info, -nthreads (accepted; the solve runs on the GPU).  Per-voxel semantics follow
 // dwi2fod.cpp:128-231: masked-out voxels are written as zeros, CSD voxels with a non-finite
 // signal give zeros, negative CSD signals are clamped, MSMT uses the raw signal.
@@ -132,6 +132,7 @@ void check(int rc) {
 struct Args {
   std::vector<std::string> pos;
   std::string mask, directions, filter, grad, predicted;
+  std::vector<std::string> fslgrad;   // bvecs, bvals
   std::vector<int> lmax;
   bool has_neg = false, has_norm = false, has_thr = false, has_niter = false, force = false;
   double neg = 0, norm = 0, thr = 0;
@@ -153,6 +154,10 @@ Args parse_args(int argc, char** argv) {
       else if (o == "directions") a.directions = next();
       else if (o == "filter") a.filter = next();
       else if (o == "grad") a.grad = next();
+      else if (o == "fslgrad") {
+        const std::string bvecs = next();
+        a.fslgrad = {bvecs, next()};
+      }
       else if (o == "predicted_signal") a.predicted = next();
       else if (o == "neg_lambda") { a.has_neg = true; a.neg = std::stod(next()); }
       else if (o == "norm_lambda") { a.has_norm = true; a.norm = std::stod(next()); }
@@ -214,6 +219,87 @@ void scatter_write(const std::string& path, const mrcsd::MifHeader& h, const std
   mrcsd::write_mif(path, h, out, true);
 }
 
+// core/dwi/gradient.cpp:127-200 (load_bvecs_bvals): an FSL bvecs / bvals pair as a gradient
+// table in scanner space.  bvecs are given in the image axes of a left-handed system: the first
+// component is negated when the stored transform's 3x3 part has a positive determinant, then the
+// vectors are rotated by it; rows with NaN become b = 0.
+std::vector<double> load_bvecs_bvals(const mrcsd::MifHeader& h, int64_t nvol, const std::string& bvecs_path,
+                                     const std::string& bvals_path) {
+  std::vector<std::vector<double>> bvals, bvecs;
+  try {
+    bvals = load_rows(bvals_path);
+    bvecs = load_rows(bvecs_path);
+  } catch (const std::exception& e) {
+    throw std::runtime_error(std::string(e.what()) + "\nUnable to import files \"" + bvecs_path + "\" and \"" +
+                             bvals_path + "\" as FSL bvecs/bvals pair");
+  }
+  auto transpose = [](const std::vector<std::vector<double>>& m) {
+    std::vector<std::vector<double>> t(m[0].size(), std::vector<double>(m.size()));
+    for (size_t r = 0; r < m.size(); ++r)
+      for (size_t c = 0; c < m[r].size(); ++c) t[c][r] = m[r][c];
+    return t;
+  };
+  if (bvals.size() != 1) {
+    if (bvals[0].size() == 1) bvals = transpose(bvals);
+    else
+      throw std::runtime_error("bvals file must contain 1 row or column only (file \"" + bvals_path + "\" has " +
+                               std::to_string(bvals.size()) + ")");
+  }
+  if (bvecs.size() != 3) {
+    if (bvecs[0].size() == 3) bvecs = transpose(bvecs);
+    else
+      throw std::runtime_error("bvecs file must contain exactly 3 rows or columns (file \"" + bvecs_path + "\" has " +
+                               std::to_string(bvecs.size()) + ")");
+  }
+  const size_t n = bvals[0].size();
+  if (bvecs[0].size() != n)
+    throw std::runtime_error("bvecs and bvals files must have same number of diffusion directions (file \"" + bvecs_path +
+                             "\" has " + std::to_string(bvecs[0].size()) + ", file \"" + bvals_path + "\" has " +
+                             std::to_string(n) + ")");
+  if (int64_t(n) != nvol)
+    throw std::runtime_error("bvecs and bvals files must have same number of diffusion directions as DW-image (gradients: " +
+                             std::to_string(n) + ", image: " + std::to_string(nvol) + ")");
+  double L[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  if (h.transform.size() >= 3)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) L[r][c] = h.transform[size_t(r)][size_t(c)];
+  const double det = L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) - L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0]) +
+                     L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]);
+  if (det > 0.0)
+    for (auto& v : bvecs[0]) v = -v;
+  std::vector<double> grad(n * 4);
+  size_t nan_rows = 0;
+  bool nan_bvecs = false, nan_bvals = false;
+  for (size_t i = 0; i < n; ++i) {
+    double g[3];
+    for (int r = 0; r < 3; ++r) g[r] = L[r][0] * bvecs[0][i] + L[r][1] * bvecs[1][i] + L[r][2] * bvecs[2][i];
+    double b = bvals[0][i];
+    const bool nan_b = std::isnan(b), nan_v = std::isnan(g[0]) || std::isnan(g[1]) || std::isnan(g[2]);
+    if (nan_b && !nan_v && g[0] * g[0] + g[1] * g[1] + g[2] * g[2] > 0.0)
+      throw std::runtime_error("Corrupt content in bvecs/bvals data (" + bvecs_path + " & " + bvals_path +
+                               ") (NaN present in bval but valid direction in bvec)");
+    if (nan_v && b > 0.0)
+      throw std::runtime_error("Corrupt content in bvecs/bvals data (" + bvecs_path + " & " + bvals_path +
+                               ") (NaN bvec direction but non-zero value in bval)");
+    nan_bvals |= nan_b;
+    nan_bvecs |= nan_v;
+    if (nan_b || nan_v) {
+      g[0] = g[1] = g[2] = 0.0;
+      b = 0.0;
+      ++nan_rows;
+    }
+    grad[i * 4] = g[0];
+    grad[i * 4 + 1] = g[1];
+    grad[i * 4 + 2] = g[2];
+    grad[i * 4 + 3] = b;
+  }
+  if (nan_rows)
+    console("[WARNING] " + std::to_string(nan_rows) + " row" + (nan_rows > 1 ? "s" : "") + " with NaN values detected in " +
+            (nan_bvecs ? "bvecs file " + bvecs_path + (nan_bvals ? " and" : "") : std::string()) +
+            (nan_bvals ? "bvals file " + bvals_path : std::string()) + "; these have been interpreted as b=0 volumes by MRtrix");
+  return grad;
+}
+
 int run(int argc, char** argv) {
   Args a = parse_args(argc, argv);
   if (a.pos.size() < 2) throw std::runtime_error("usage: dwi2fod algorithm dwi response odf [response odf ...]");
@@ -227,7 +313,10 @@ int run(int argc, char** argv) {
   const int nvol = int(dwi.h.dim[3]);
 
   std::vector<double> grad;
+  if (!a.grad.empty() && !a.fslgrad.empty())   // gradient.cpp:261-265
+    throw std::runtime_error("Diffusion gradient table can be provided using either -grad or -fslgrad option, but NOT both");
   if (!a.grad.empty()) grad = flatten(load_rows(a.grad));
+  else if (!a.fslgrad.empty()) grad = load_bvecs_bvals(dwi.h, nvol, a.fslgrad[0], a.fslgrad[1]);
   else grad = flatten(parse_rows(dwi.h.get("dw_scheme")));
   if (grad.empty()) throw std::runtime_error("no valid diffusion gradient table found");
   if (grad.size() != size_t(nvol) * 4)

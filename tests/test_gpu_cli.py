@@ -136,3 +136,43 @@ def test_cli_sh2peaks_matches_oracle(tmp_path):
     say("[gpu cli] sh2peaks: %d of %d voxels differ" % (bad, n))
     assert bad <= 0.001 * n + 1
     assert np.all(np.isnan(g[mask.reshape(-1) == 0]))
+
+
+def test_cli_fslgrad_matches_dw_scheme(tmp_path):
+    """-fslgrad bvecs bvals (core/dwi/gradient.cpp:127-200) on an image whose transform is a
+    rotation: the bvecs are the scheme's directions in image axes with the first component negated
+    (positive determinant), one b=0 row has NaN directions.  The FODs equal those of the same
+    image with the scheme in its header."""
+    assert os.path.exists(EXE), "dwi2fod is not built"
+    ph = phantom.single_shell_small()
+    shape = (8, 8, 4)
+    n = int(np.prod(shape))
+    img = ph.dw[:n].reshape(shape + (ph.dw.shape[1],))
+    a, b = 0.4, -0.3
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]]) @ \
+        np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    T = np.concatenate([R, [[10.0], [-5.0], [3.0]]], axis=1)
+    g = ph.grad
+    bvecs = R.T @ g[:, :3].T
+    bvecs[0] *= -1.0
+    b0 = np.nonzero(g[:, 3] == 0)[0]
+    assert len(b0) > 0
+    bvecs[:, b0[0]] = np.nan
+    d1, d2, resp = (str(tmp_path / f) for f in ("hdr.mif", "plain.mif", "resp.txt"))
+    fv, fb = str(tmp_path / "bvecs"), str(tmp_path / "bvals")
+    mif.write(d1, img, vox=[2.0, 2.0, 2.0], transform=T.tolist(), keyval=[("dw_scheme", mif.scheme_text(g))])
+    mif.write(d2, img, vox=[2.0, 2.0, 2.0], transform=T.tolist())
+    with open(fv, "w") as f:
+        f.write("\n".join(" ".join(repr(float(v)) for v in row) for row in bvecs) + "\n")
+    with open(fb, "w") as f:
+        f.write("\n".join(repr(float(v)) for v in g[:, 3]) + "\n")       # one column
+    write_response(resp, ph.response)
+    say("[gpu cli] dwi2fod csd -fslgrad vs dw_scheme on %d voxels" % n)
+    run(["csd", d1, resp, str(tmp_path / "a.mif"), "-force"])
+    r = run(["csd", d2, resp, str(tmp_path / "b.mif"), "-fslgrad", fv, fb, "-force"])
+    assert "1 row with NaN values detected in bvecs file" in r.stderr
+    fa, _ = mif.read(str(tmp_path / "a.mif"))
+    fb_, _ = mif.read(str(tmp_path / "b.mif"))
+    rel = np.linalg.norm((fa - fb_).reshape(n, -1), axis=1) / np.maximum(np.linalg.norm(fa.reshape(n, -1), axis=1), 1e-30)
+    say("[gpu cli] -fslgrad: max rel %.3e" % rel.max())
+    assert np.mean(rel <= 1e-5) >= 0.999

@@ -59,3 +59,37 @@ def test_cli_csd_wrong_pair_count(tmp_path):
     assert r.returncode == 1 and "expects a single input response function" in r.stderr
     r = subprocess.run([EXE, "msmt_csd", p, "r.txt", "o.mif", "x.mif"], capture_output=True, text=True)
     assert r.returncode == 1 and "expects pairs" in r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="dwi2fod not built")
+def test_cli_fslgrad_errors(tmp_path):
+    """core/dwi/gradient.cpp:127-200 and :261-265: the bvecs / bvals checks, all before any GPU work."""
+    p = str(tmp_path / "d.mif")
+    mif.write(p, np.ones((2, 2, 2, 4), dtype=np.float32))
+    bvecs, bvals, grad = (str(tmp_path / f) for f in ("bvecs", "bvals", "grad.txt"))
+
+    def put(path, rows):
+        with open(path, "w") as f:
+            f.write("\n".join(" ".join(str(v) for v in r) for r in rows) + "\n")
+
+    def err(*extra):
+        r = subprocess.run([EXE, "csd", p, "r.txt", str(tmp_path / "o.mif")] + list(extra),
+                           capture_output=True, text=True)
+        assert r.returncode == 1
+        return r.stderr
+
+    put(grad, [[0, 0, 1, 1000]] * 4)
+    put(bvecs, [[1, 0, 0, 1], [0, 1, 0, 0], [0, 0, 1, 0]])
+    put(bvals, [[0, 1000, 1000, 1000]])
+    assert "either -grad or -fslgrad option, but NOT both" in err("-grad", grad, "-fslgrad", bvecs, bvals)
+    put(bvals, [[0, 1000], [1000, 1000]])
+    assert "bvals file must contain 1 row or column only" in err("-fslgrad", bvecs, bvals)
+    put(bvals, [[0, 1000, 1000]])
+    assert "must have same number of diffusion directions (file" in err("-fslgrad", bvecs, bvals)
+    put(bvecs, [[1, 0], [0, 1]])
+    assert "bvecs file must contain exactly 3 rows or columns" in err("-fslgrad", bvecs, bvals)
+    put(bvecs, [[1, 0, 0]] * 3)                       # three columns: read as one vector per row
+    assert "as DW-image (gradients: 3, image: 4)" in err("-fslgrad", bvecs, bvals)
+    put(bvecs, [[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 1]])
+    put(bvals, [[1000], [float("nan")], [1000], [1000]])
+    assert "NaN present in bval but valid direction in bvec" in err("-fslgrad", bvecs, bvals)
