@@ -620,6 +620,7 @@ def bench_peaks(args, torch, ph):
     one peak search of every voxel (60 search directions, 3 peaks)."""
     from mrtrix3_amd import csd, peaks
     dev = torch.device("cuda:0")
+    calibrate_copy(torch, dev, args.copy_calibration_mb)
     shared = csd.CSDShared(ph.grad, ph.response)
     solver = csd.CSD(shared)
     dw = torch.from_numpy(ph.dw).to(dev)
@@ -652,10 +653,12 @@ def bench_peaks(args, torch, ph):
         "config": {"workload": "sh2peaks on the single_shell_brain FODs (646,128 voxels, lmax 8), 60 search "
                                "directions, 3 peaks; FODs from the plain GPU CSD, resident in HBM"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "peaks_kernel<8>",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": load_traffic("peaks_kernel_single_shell_brain"),
+                     "kernel": "peaks_search_kernel<%d> + peaks_select_kernel" % shared.lmax,
                      "kernel_ms": kern_ms, "flop_per_launch": fl,
                      "note": "FP32 VALU-bound (no matrix work); FLOPs counted per Newton iteration from the "
-                             "restatement, transcendental calls as one"},
+                             "restatement, transcendental calls as one; one run is the search kernel (all the "
+                             "FLOPs) and the per-voxel selection kernel, timed together"},
         "peaks": {"newton_iterations_per_voxel": float(st[1]) / max(1, st[0]),
                   "searches_not_converged_frac": float(st[2]) / max(1.0, 60.0 * st[0]),
                   "flop_per_iteration": peaks_flops_per_iteration(shared.lmax)},

@@ -34,7 +34,7 @@ struct mrpeaks_ctx {
   bool fixed_dirs = false;
   mrcsd::DeviceBuffer<float> seeds;     // [nseeds][3]
   mrcsd::DeviceBuffer<float> dirs;      // [npeaks][3] (-direction)
-  mrcsd::DeviceBuffer<unsigned long long> counter;   // [2]
+  mrcsd::DeviceBuffer<unsigned long long> counter;   // the search kernel's work counter
   mrcsd::DeviceBuffer<float4> work;                  // search results, grown on demand
 };
 
@@ -80,7 +80,7 @@ int mrpeaks_create(const mrpeaks_options* opt, mrpeaks_ctx** out) {
       ctx->dirs.upload(d.data(), d.size());
       ctx->fixed_dirs = true;
     }
-    ctx->counter.alloc(2);
+    ctx->counter.alloc(1);
     *out = ctx.release();
     return MRCSD_OK;
   });

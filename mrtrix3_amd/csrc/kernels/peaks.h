@@ -25,7 +25,7 @@ struct PeaksLaunch {
   float* out;               // [nvox][3 npeaks]
   unsigned long long* stats;    // [4] or null: voxels searched, Newton iterations, failed searches,
                                 // voxels whose distinct peaks exceeded kPeaksMaxAccepted
-  unsigned long long* counter;  // [2] work counters (search queue, voxel queue)
+  unsigned long long* counter;  // search work counter
   float4* work;                 // [work_items] search results: amplitude, x, y, z
   size_t work_items;            // capacity of `work`; voxels are processed in chunks that fit
 };

@@ -6,9 +6,10 @@
 //   * peaks_search_kernel: every (voxel, search direction) pair is a work item; each lane takes the
 //     next item as soon as its current search ends (converged, or 50 iterations), so lanes stay busy
 //     whatever the spread of iteration counts (a wavefront per voxel would wait for its slowest
-//     search: about a third of the lanes' time).  Items are handed out with one atomic per
-//     wavefront for all lanes that need one.  Results go to a work buffer: amplitude, direction.
-//   * peaks_select_kernel: one wavefront per voxel reads its searches' results and collects the
+//     search: about a third of the lanes' time).  A wavefront reserves 256 items per atomic and
+//     hands them to its idle lanes.  Results go to a work buffer: amplitude, direction.
+//   * peaks_select_kernel: one wavefront per voxel (a fixed stride over the voxels, no counter:
+//     one atomic per voxel on one address serialises) reads its searches' results and collects the
 //     accepted peaks in seed order as the reference does (a peak within acos(0.99) of an accepted
 //     one is dropped, then the threshold applies), one accepted peak per lane, then chooses the
 //     reported ones by decreasing amplitude or by closeness to given directions.
@@ -18,7 +19,7 @@
 // terms, Math::sqrt2, the 2.0 * daz * del term, the constants of get_peak), and no fused
 // multiply-adds (contraction is off in this file).  Results differ from it in the last bits
 // through cheaper evaluations of the same quantities: the sines and cosines of the direction's
-// angles from its components (not atan2 then sin / cos), cos / sin(m az) by angle addition, the
+// angles from its components (not atan2 then sin / cos), cos / sin(m az) by a recurrence, the
 // Legendre seed helper with folded constants and its square root in single precision.
 #include <hip/hip_runtime.h>
 #include <limits.h>
@@ -59,6 +60,19 @@ __device__ __forceinline__ float fbits(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+// a / f for a divisor known at compile time: y = 1 / f folds to the correctly rounded reciprocal,
+// and one product with two fused corrections (Markstein's theorem) gives the correctly rounded
+// quotient in three instructions instead of a division sequence.  Checked against a / f for every
+// float a and every divisor of the recursion below up to lmax 12: equal (zeros' signs included)
+// for all finite |a| >= 2^-100; below, where the residual leaves the normal range, it can differ
+// by one unit of the last place.
+__device__ __forceinline__ float div_const(float a, float f) {
+  const float y = 1.0f / f;
+  const float q0 = a * y;
+  const float r = __builtin_fmaf(f, q0, -a);
+  return __builtin_fmaf(-r, y, q0);
+}
+
 // legendre.h:129-154 for order M, written into AL at index_mpos(l, M) for even l (SH.h:585-590)
 template <int LMAX, int M>
 __device__ __forceinline__ void plm_order(float (&AL)[nforl_mpos(LMAX)], float x, float sin2) {
@@ -72,7 +86,7 @@ __device__ __forceinline__ void plm_order(float (&AL)[nforl_mpos(LMAX)], float x
     buf[M + 1] = x * f * buf[M];
 #pragma unroll
     for (int n = M + 2; n <= LMAX; n++) {
-      buf[n] = x * buf[n - 1] - buf[n - 2] / f;
+      buf[n] = x * buf[n - 1] - div_const(buf[n - 2], f);
       f = sqrtf((float)(4 * n * n - 1) / (float)(n * n - M * M));
       buf[n] *= f;
     }
@@ -89,13 +103,16 @@ __device__ __forceinline__ void plm_all(float (&AL)[nforl_mpos(LMAX)], float x, 
 }
 
 // SH.h:568-647: amplitude and derivatives at the direction whose elevation and azimuth have the
-// given sines and cosines; c = the SH coefficients.  cos(m az), sin(m az) for m >= 2 come from the
-// m = 1 values by angle addition in double (a few float ulps from the directly evaluated values at
-// m = 8) instead of sixteen sin / cos calls.
+// given sines and cosines; c = the SH coefficients.  sqrt2 cos(m az) and sqrt2 sin(m az) for
+// m >= 2 come from the m = 0 and m = 1 values by the recurrence X(m+1) = 2 cos(az) X(m) - X(m-1)
+// in double (its error, below m^2 double ulps, leaves the float values as the directly evaluated
+// ones but for rounding ties) instead of sixteen sin / cos calls.
 template <int LMAX>
 __device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float sel, float cel, float saz1, float caz1,
                                             float (&out)[6]) {
-  double cm = (double)caz1, sm = (double)saz1;   // cos(m az), sin(m az) at m = 1
+  const double twoc = 2.0 * (double)caz1;
+  double cprev = kSqrt2, sprev = 0.0;                                   // m = 0
+  double cm = kSqrt2 * (double)caz1, sm = kSqrt2 * (double)saz1;        // m = 1
   const bool atpole = (double)sel < 1e-4;
   float AL[nforl_mpos(LMAX)];
   plm_all<LMAX>(AL, cel, sel * sel, std::make_integer_sequence<int, LMAX + 1>{});
@@ -113,12 +130,14 @@ __device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float
 #pragma unroll
   for (int m = 1; m <= LMAX; m++) {
     if (m > 1) {
-      const double cn = cm * (double)caz1 - sm * (double)saz1;
-      sm = sm * (double)caz1 + cm * (double)saz1;
+      const double cn = __builtin_fma(twoc, cm, -cprev), sn = __builtin_fma(twoc, sm, -sprev);
+      cprev = cm;
+      sprev = sm;
       cm = cn;
+      sm = sn;
     }
-    const float caz = (float)(kSqrt2 * cm);
-    const float saz = (float)(kSqrt2 * sm);
+    const float caz = (float)cm;
+    const float saz = (float)sm;
 #pragma unroll
     for (int l = (m & 1) ? m + 1 : m; l <= LMAX; l += 2) {
       const float vp = c[sh_index(l, m)];
@@ -216,6 +235,7 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
 }
 
 constexpr int kRefill = 32;
+constexpr int kGrab = 256;   // items reserved per atomic: one atomic per ~4 refills
 
 // Searches of voxels [v0, v0 + nv): item g = (v0 + g / nseeds, seed g % nseeds), result in work[g].
 // Up to lmax 8 the kernel is held to 168 registers (three wavefronts per SIMD, a few spills), which
@@ -230,20 +250,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
   int it = 0;
   bool busy = false, exhausted = false;
   size_t item = 0;
+  unsigned long long rb = 0, re = 0;   // this wavefront's reserved items [rb, re), uniform
   unsigned long long st_it = 0, st_fail = 0;
   for (;;) {
-    // lanes without a search take the next items, one atomic per wavefront, once at least
-    // kRefill lanes are idle (or all are): fewer rounds of coefficient loads for a short idle wait
+    // lanes without a search take the next items once at least kRefill lanes are idle (or all
+    // are): fewer rounds of coefficient loads for a short idle wait.  Items come from the
+    // wavefront's reservation [rb, re), topped up by kGrab items per atomic.
     const unsigned long long need = __ballot(!busy);
-    if (!exhausted && (__popcll(need) >= kRefill || need == ~0ull)) {
-      const int first = __ffsll((long long)need) - 1;
+    const unsigned nneed = (unsigned)__popcll(need);
+    if (!exhausted && (nneed >= (unsigned)kRefill || need == ~0ull)) {
+      const unsigned long long rem = re - rb;
       unsigned long long base = 0;
-      if (lane == first) base = atomicAdd(L.counter, (unsigned long long)__popcll(need));
-      base = readlane64(base, first);
-      if (base >= nitems) exhausted = true;
+      if (rem < nneed) {
+        const int first = __ffsll((long long)need) - 1;
+        if (lane == first) base = atomicAdd(L.counter, (unsigned long long)kGrab);
+        base = readlane64(base, first);
+        if (base >= nitems) exhausted = true;
+      }
       if (!busy) {
         const unsigned long long below = lane ? (need & (~0ull >> (64 - lane))) : 0ull;
-        item = (size_t)(base + (unsigned long long)__popcll(below));
+        const unsigned long long rank = (unsigned long long)__popcll(below);
+        const unsigned long long pick = rank < rem ? rb + rank : base + (rank - rem);
+        item = (size_t)(pick < nitems ? pick : nitems);
         if (item < nitems) {
           const size_t vox = v0 + item / (size_t)L.nseeds;
           const int s = (int)(item % (size_t)L.nseeds);
@@ -267,6 +295,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
             busy = true;
           }
         }
+      }
+      // the reservation left (re <= nitems, so every item taken from it exists; an atomic past
+      // the end leaves it empty and the wavefront exhausted)
+      if (rem < nneed) {
+        const unsigned long long hi = base + (unsigned long long)kGrab < nitems ? base + (unsigned long long)kGrab : nitems;
+        const unsigned long long lo = base + (nneed - rem);
+        rb = lo < hi ? lo : hi;
+        re = hi;
+      } else {
+        rb += nneed;
       }
     }
     if (__ballot(busy) == 0ull) {
@@ -296,25 +334,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
 
 // Accepted and reported peaks of voxels [v0, v0 + nv) from their searches' results.
 __global__ __launch_bounds__(64) void peaks_select_kernel(PeaksLaunch L, size_t v0, size_t nv) {
-  __shared__ float s_c[128];
+  __shared__ float4 s_cand[64];                 // the batch's candidates (amplitude, direction)
+  __shared__ float4 s_acc[kPeaksMaxAccepted];   // accepted peaks, in seed order
   const int lane = threadIdx.x;
   const int nout_vals = 3 * L.npeaks;
   unsigned long long st_vox = 0, st_over = 0;
 
-  for (;;) {
-    unsigned long long t = 0;
-    if (lane == 0) t = atomicAdd(L.counter + 1, 1ull);
-    t = uniform64(t);
-    if (t >= nv) break;
-    const size_t vox = v0 + (size_t)t;
+  // voxels are taken in a fixed stride (the work per voxel is nearly uniform; a shared counter
+  // would serialise one atomic per voxel on one address)
+  for (size_t t = blockIdx.x; t < nv; t += gridDim.x) {
+    const size_t vox = v0 + t;
     float* o = L.out + vox * (size_t)nout_vals;
 
     // ---- input checks: mask (DataLoader, sh2peaks.cpp:126-130) and check_input (:266-286) ----
     const float* src = L.sh + vox * (size_t)L.ld_sh;
     const float v0c = lane < L.n_sh ? src[lane] : 0.0f;
     const float v1c = lane + 64 < L.n_sh ? src[lane + 64] : 0.0f;
-    s_c[lane] = v0c;
-    s_c[lane + 64] = v1c;
     const bool has_nan = __ballot(isnan(v0c) || isnan(v1c)) != 0ull;
     const bool nonzero = __ballot((lane >= 1 && v0c != 0.0f) || v1c != 0.0f) != 0ull;
     const float* match = L.match ? L.match + (L.ld_match ? vox * (size_t)L.ld_match : 0) : nullptr;
@@ -327,33 +362,74 @@ __global__ __launch_bounds__(64) void peaks_select_kernel(PeaksLaunch L, size_t 
     ++st_vox;
 
     // ---- the searches' results, accepted in seed order (sh2peaks.cpp:176-189) ----
+    // Lane j holds candidate j of a batch of 64 seeds.  For candidate i in seed order, one ballot
+    // gives the candidates within acos(0.99) of it (|dot| is symmetric: the same float value
+    // either way round), so it is a duplicate when that mask meets the accepted set A; validity,
+    // the threshold and the earlier batches' accepted peaks are per-lane tests done up front.
+    // The dot product is the reference's p.dot(peaks[i]) term for term, and (double)|dot| > 0.99
+    // is |dot| >= 0.99f (the float nearest 0.99 lies above it).
     const float4* res = L.work + (size_t)t * (size_t)L.nseeds;
-    float my_a = NAN, my_x = 0.0f, my_y = 0.0f, my_z = 0.0f;   // accepted peak `lane`
-    int nacc = 0;
+    int nacc = 0, nstored = 0;
     for (int b0 = 0; b0 < L.nseeds; b0 += 64) {
       const int s = b0 + lane;
       const float4 r = s < L.nseeds ? res[s] : make_float4(NAN, NAN, NAN, NAN);
       const int nb = L.nseeds - b0 < 64 ? L.nseeds - b0 : 64;
-      for (int i = 0; i < nb; ++i) {
-        const float ca = fbits(r.x, i);
-        if (!isfinite(ca)) continue;
-        const float cx = fbits(r.y, i), cy = fbits(r.z, i), cz = fbits(r.w, i);
-        const bool dup = lane < nacc && (double)fabsf(cx * my_x + cy * my_y + cz * my_z) > kDotThreshold;
-        if (__ballot(dup) != 0ull) continue;
-        if (!(ca >= L.threshold)) continue;
-        if (lane == nacc) {
-          my_a = ca;
-          my_x = cx;
-          my_y = cy;
-          my_z = cz;
-        }
-        ++nacc;
+      bool dup_prev = false;   // within acos(0.99) of a peak accepted in an earlier batch
+      for (int k = 0; k < nstored; ++k) {
+        const float4 q = s_acc[k];
+        dup_prev = dup_prev || fabsf(r.y * q.y + r.z * q.z + r.w * q.w) >= 0.99f;
       }
+      // finite, not within acos(0.99) of an earlier batch's peak, and at or above the threshold:
+      // the candidates that are accepted unless within acos(0.99) of one accepted in this batch
+      const unsigned long long cand = __ballot(isfinite(r.x) && !dup_prev && r.x >= L.threshold);
+      s_cand[lane] = r;   // every lane writes: entries past nb hold NaN candidates
+      wave_sync();
+      const int room = kPeaksMaxAccepted - nstored;
+      unsigned long long A = 0ull;   // accepted and stored in this batch
+      if (nb <= room) {
+        // every peak accepted in this batch is stored: the decision per candidate is one mask
+        // test (candidates past nb have no bit in cand)
+        for (int i0 = 0; i0 < nb; i0 += 4) {   // i0 + 3 <= 63: four LDS reads issued together
+          float4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) q[u] = s_cand[i0 + u];
+          unsigned long long close[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) close[u] = __ballot(fabsf(r.y * q[u].y + r.z * q[u].z + r.w * q[u].w) >= 0.99f);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) A |= (close[u] & A) ? 0ull : (cand & (1ull << (i0 + u)));
+        }
+        nacc += __popcll(A);
+      } else {
+        for (int i = 0; i < nb; ++i) {
+          const float4 q = s_cand[i];
+          const unsigned long long close = __ballot(fabsf(r.y * q.y + r.z * q.z + r.w * q.w) >= 0.99f);
+          if (((cand >> i) & 1ull) && !(close & A)) {
+            if (__popcll(A) < room) A |= 1ull << i;
+            ++nacc;
+          }
+        }
+      }
+      if ((A >> lane) & 1ull) {
+        const unsigned long long below = lane ? (A & (~0ull >> (64 - lane))) : 0ull;
+        s_acc[nstored + __popcll(below)] = r;
+      }
+      nstored += __popcll(A);
+      wave_sync();
     }
     if (nacc > kPeaksMaxAccepted) {
       ++st_over;
       nacc = kPeaksMaxAccepted;
     }
+    float my_a = NAN, my_x = 0.0f, my_y = 0.0f, my_z = 0.0f;   // accepted peak `lane`
+    if (lane < nstored) {
+      const float4 q = s_acc[lane];
+      my_a = q.x;
+      my_x = q.y;
+      my_y = q.z;
+      my_z = q.w;
+    }
+    wave_sync();
 
     // ---- the reported peaks (sh2peaks.cpp:204-246) ----
     const int nrep = L.npeaks < nacc ? L.npeaks : nacc;
@@ -388,8 +464,8 @@ __global__ __launch_bounds__(64) void peaks_select_kernel(PeaksLaunch L, size_t 
   }
 }
 
-__global__ void reset_counters_kernel(unsigned long long* c) {
-  if (threadIdx.x < 2 && blockIdx.x == 0) c[threadIdx.x] = 0ull;
+__global__ void reset_counter_kernel(unsigned long long* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
 }
 
 template <class K>
@@ -415,7 +491,7 @@ hipError_t launch_lmax(const PeaksLaunch& L, hipStream_t stream) {
     hipError_t e = persistent_grid(peaks_search_kernel<LMAX>, nv * (size_t)L.nseeds / 64 + 1, &g1);
     if (e != hipSuccess) return e;
     if ((e = persistent_grid(peaks_select_kernel, nv, &g2)) != hipSuccess) return e;
-    reset_counters_kernel<<<1, 64, 0, stream>>>(L.counter);
+    reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
     peaks_search_kernel<LMAX><<<dim3((unsigned)g1), dim3(64), 0, stream>>>(L, v0, nv);
     peaks_select_kernel<<<dim3((unsigned)g2), dim3(64), 0, stream>>>(L, v0, nv);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -1,0 +1,23 @@
+"""The sh2peaks kernel divides by the Legendre recursion's compile-time constants with one
+product and two fused corrections instead of a division sequence (div_const in
+mrtrix3_amd/csrc/kernels/peaks.hip).  tests/native/div_const_check.c compares that quotient with
+the correctly rounded a / f for every divisor up to lmax 12, over every significand at exponents
+from -100 to 100 and both zeros (its "all" mode covers all 2^32 floats: equal for every finite
+|a| >= 2^-100)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_constant_divisor_quotient_is_correctly_rounded(tmp_path):
+    exe = str(tmp_path / "div_const_check")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "native", "div_const_check.c"),
+                    "-lm", "-lpthread"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "66 divisors" in r.stdout and "FAIL" not in r.stdout
