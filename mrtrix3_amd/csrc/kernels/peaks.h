@@ -33,7 +33,7 @@ struct PeaksLaunch {
 // Results of one (voxel, search direction) pair per work item; the buffer a launch needs for nvox
 // voxels at most (the launch processes chunks of work_items / nseeds voxels).
 inline size_t peaks_work_items(size_t nvox, int nseeds) {
-  const size_t cap = size_t(1) << 24;   // 16M searches = 256 MB
+  const size_t cap = size_t(1) << 26;   // 64M searches = 1 GiB (a 646k-voxel volume at 60 seeds in one pass)
   const size_t all = nvox * size_t(nseeds);
   return all < cap ? all : (cap / size_t(nseeds)) * size_t(nseeds);
 }

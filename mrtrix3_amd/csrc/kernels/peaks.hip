@@ -235,7 +235,11 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
 }
 
 constexpr int kRefill = 32;
-constexpr int kGrab = 256;   // items reserved per atomic: one atomic per ~4 refills
+constexpr int kGrab = 256;      // items reserved per atomic: one atomic per ~4 refills
+constexpr int kGrabTail = 64;   // near the end of the queue: a shorter drain
+// one atomic must cover every idle lane (the lanes past the reservation take consecutive items
+// from the new block)
+static_assert(kGrabTail >= 64 && kGrab >= kGrabTail, "a reservation block holds at least a wavefront of items");
 
 // Searches of voxels [v0, v0 + nv): item g = (v0 + g / nseeds, seed g % nseeds), result in work[g].
 // Up to lmax 8 the kernel is held to 168 registers (three wavefronts per SIMD, a few spills), which
@@ -251,11 +255,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
   bool busy = false, exhausted = false;
   size_t item = 0;
   unsigned long long rb = 0, re = 0;   // this wavefront's reserved items [rb, re), uniform
+  unsigned long long grab = kGrab;
+  const unsigned long long tail_window = 2ull * (unsigned long long)kGrab * (unsigned long long)gridDim.x;
   unsigned long long st_it = 0, st_fail = 0;
   for (;;) {
     // lanes without a search take the next items once at least kRefill lanes are idle (or all
     // are): fewer rounds of coefficient loads for a short idle wait.  Items come from the
-    // wavefront's reservation [rb, re), topped up by kGrab items per atomic.
+    // wavefront's reservation [rb, re), topped up by `grab` items per atomic: kGrab, then kGrabTail
+    // once fewer than two rounds of kGrab reservations per wavefront are left, so that the last
+    // wavefronts do not hold four searches per lane each while the others have finished.
     const unsigned long long need = __ballot(!busy);
     const unsigned nneed = (unsigned)__popcll(need);
     if (!exhausted && (nneed >= (unsigned)kRefill || need == ~0ull)) {
@@ -263,7 +271,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
       unsigned long long base = 0;
       if (rem < nneed) {
         const int first = __ffsll((long long)need) - 1;
-        if (lane == first) base = atomicAdd(L.counter, (unsigned long long)kGrab);
+        if (lane == first) base = atomicAdd(L.counter, grab);
         base = readlane64(base, first);
         if (base >= nitems) exhausted = true;
       }
@@ -299,10 +307,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
       // the reservation left (re <= nitems, so every item taken from it exists; an atomic past
       // the end leaves it empty and the wavefront exhausted)
       if (rem < nneed) {
-        const unsigned long long hi = base + (unsigned long long)kGrab < nitems ? base + (unsigned long long)kGrab : nitems;
+        const unsigned long long hi = base + grab < nitems ? base + grab : nitems;
         const unsigned long long lo = base + (nneed - rem);
         rb = lo < hi ? lo : hi;
         re = hi;
+        if (base + tail_window >= nitems) grab = kGrabTail;
       } else {
         rb += nneed;
       }
