@@ -686,10 +686,10 @@ def main():
                     help="skip the end-to-end dwi2fod command on the phantom written as .mif files")
     ap.add_argument("--train-phantoms", type=int, default=4,
                     help="phantoms (646k voxels each, further seeds of the protocol) to train on")
-    ap.add_argument("--epochs", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=64)
     ap.add_argument("--lr", type=float, default=8e-3, help="peak learning rate of the warm-start training")
     ap.add_argument("--wc", type=float, default=1.0, help="weight of the coefficient loss")
-    ap.add_argument("--wl", type=float, default=0.1, help="weight of the negative-set loss")
+    ap.add_argument("--wl", type=float, default=0.3, help="weight of the negative-set loss")
     ap.add_argument("--warm-budget", type=int, default=6)
     ap.add_argument("--save-net", default="", help="write the trained warm-start network to this .npz")
     ap.add_argument("--load-net", default="",
