@@ -58,7 +58,10 @@ int mrpeaks_npeaks(const mrpeaks_ctx* ctx);
  *   stats     uint64[4] or NULL, accumulated: voxels searched, Newton iterations, searches that did
  *             not converge in 50 iterations, voxels with more than 64 distinct peaks
  * A peaks_in direction with no accepted peak at |dot| > 0 is reported as NaN (the reference
- * reports whatever its per-thread buffer last held there). */
+ * reports whatever its per-thread buffer last held there).
+ * The context keeps a device work buffer of 16 B per (voxel, search direction), grown on demand up
+ * to 1 GiB (64M searches); larger calls run in voxel passes.  Calls on one context are ordered on
+ * the stream they are given; concurrent calls on one context from different streams race. */
 int mrpeaks_run(mrpeaks_ctx* ctx, const float* sh, int ld_sh, const uint8_t* mask, const float* peaks_in,
                 int ld_peaks_in, size_t nvox, float* out, uint64_t* stats, void* stream);
 /* The same with host pointers (synchronous). */

@@ -32,6 +32,7 @@ LIB_SOURCES = [
     "kernels/icls_solve.hip",
     "kernels/mlp.hip",
     "kernels/peaks.hip",
+    "kernels/resp.hip",
     "host/sh.cpp",
     "host/dense.cpp",
     "host/gradient.cpp",
@@ -43,6 +44,7 @@ LIB_SOURCES = [
     "capi_icls.cpp",
     "capi_warm.cpp",
     "capi_peaks.cpp",
+    "capi_resp.cpp",
 ]
 
 
