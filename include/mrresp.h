@@ -21,7 +21,8 @@ typedef struct mrresp_ctx mrresp_ctx;
 /* One shell: its unit gradient directions (host, [nvol][3], the Cartesian form of
  * DWI::gen_direction_matrix for the shell's volumes, amp2response.cpp:232-237 and :330), the
  * volume index of each (host, [nvol]) and the zonal harmonic degree (even, 0..12;
- * Accumulator::Shared, amp2response.cpp:116-128). */
+ * Accumulator::Shared, amp2response.cpp:116-128).  The directions must be finite for lmax > 0; at
+ * lmax 0 they do not enter (a b=0 shell's zero directions give NaN ones, as the reference's). */
 int mrresp_create(const double* dirs, const int* volumes, int nvol, int lmax, mrresp_ctx** out);
 void mrresp_destroy(mrresp_ctx* ctx);
 /* Number of zonal coefficients, lmax / 2 + 1. */

@@ -97,10 +97,11 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, so)
-    for name in ("dwi2fod", "sh2peaks"):
+    for name, extra in (("dwi2fod", []), ("sh2peaks", []), ("amp2response", ["gradient.cpp", "dense.cpp"])):
         cli = os.path.join(CSRC, name + ".cpp")
         exe = os.path.join(BIN, name)
-        cmd = [HIPCC] + HIPFLAGS + [cli, os.path.join(CSRC, "host", "mif.cpp"), "-o", exe + ".tmp",
+        host = [os.path.join(CSRC, "host", f) for f in ["mif.cpp"] + extra]
+        cmd = [HIPCC] + HIPFLAGS + [cli] + host + ["-o", exe + ".tmp",
                                     "-L" + LIB, "-lmrcsd",
                                     "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
         r = subprocess.run(cmd, capture_output=True, text=True)

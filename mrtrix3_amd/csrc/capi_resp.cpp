@@ -54,8 +54,8 @@ int mrresp_create(const double* dirs, const int* volumes, int nvol, int lmax, mr
       return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_create: lmax must be even and in [0, 12]");
     for (int i = 0; i < nvol; ++i) {
       if (volumes[i] < 0) return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_create: negative volume index");
-      if (!std::isfinite(dirs[3 * i]) || !std::isfinite(dirs[3 * i + 1]) || !std::isfinite(dirs[3 * i + 2]))
-        return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_create: non-finite direction");
+      if (lmax && (!std::isfinite(dirs[3 * i]) || !std::isfinite(dirs[3 * i + 1]) || !std::isfinite(dirs[3 * i + 2])))
+        return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_create: non-finite direction (at lmax 0 the directions do not enter)");
     }
     int rc = mrcsd::require_gpu();
     if (rc) return rc;

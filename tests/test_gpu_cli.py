@@ -176,3 +176,45 @@ def test_cli_fslgrad_matches_dw_scheme(tmp_path):
     rel = np.linalg.norm((fa - fb_).reshape(n, -1), axis=1) / np.maximum(np.linalg.norm(fa.reshape(n, -1), axis=1), 1e-30)
     say("[gpu cli] -fslgrad: max rel %.3e" % rel.max())
     assert np.mean(rel <= 1e-5) >= 0.999
+
+
+def test_cli_amp2response_matches_oracle(tmp_path):
+    """amp2response amps mask directions response (cmd/amp2response.cpp) on the small phantom's
+    two shells (b=0 at lmax 0, b=3000 at lmax 10 by default), against the oracle's sums and solves;
+    then -noconstraint."""
+    from oracle import resp_oracle as R
+    exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "amp2response")
+    assert os.path.exists(exe), "amp2response is not built"
+    ph = phantom.single_shell_small()
+    shape = (16, 16, 8)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(12)
+    amp = ph.dw[:n].astype(np.float32)
+    fib = rng.standard_normal((n, 3)).astype(np.float32)
+    mask = (rng.random(n) < 0.3).astype(np.float32)
+    paths = [str(tmp_path / f) for f in ("amps.mif", "mask.mif", "dirs.mif", "resp.txt", "resp_ols.txt")]
+    mif.write(paths[0], amp.reshape(shape + (amp.shape[1],)), keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+    mif.write(paths[1], mask.reshape(shape))
+    mif.write(paths[2], fib.reshape(shape + (3,)))
+    say("[gpu cli] amp2response on %d voxels (%d in mask)" % (n, int(mask.sum())))
+    r = subprocess.run([exe] + paths[:4] + ["-force"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    r2 = subprocess.run([exe] + paths[:3] + [paths[4], "-noconstraint", "-force"], capture_output=True, text=True, timeout=240)
+    assert r2.returncode == 0, r2.stderr
+    got = np.loadtxt(paths[3], ndmin=2)
+    got_ols = np.loadtxt(paths[4], ndmin=2)
+    assert got.shape == (2, 6)
+    b = ph.grad[:, 3]
+    m = mask.astype(np.uint8)
+    for row, (sel, lmax) in enumerate([(b <= 10, 0), (b > 10, 10)]):
+        vols = np.nonzero(sel)[0]
+        g = ph.grad[vols, :3]
+        nrm = np.linalg.norm(g, axis=1, keepdims=True)
+        dirs = np.where(nrm > 0, g / np.where(nrm > 0, nrm, 1), np.array([0.0, 0.0, 1.0]))
+        M, bb, _ = R.accumulate(amp, fib, m, dirs, vols, lmax)
+        for out, constrained in ((got, True), (got_ols, False)):
+            ref, _ = R.solve(lmax, M, bb, constrained)
+            k = lmax // 2 + 1
+            err = np.abs(out[row, :k] - ref).max() / np.abs(ref).max()
+            say("[gpu cli] amp2response shell %d constrained %s: rel %.2e" % (row, constrained, err))
+            assert err <= 1e-5 and np.all(out[row, k:] == 0)

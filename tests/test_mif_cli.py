@@ -93,3 +93,29 @@ def test_cli_fslgrad_errors(tmp_path):
     put(bvecs, [[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 1]])
     put(bvals, [[1000], [float("nan")], [1000], [1000]])
     assert "NaN present in bval but valid direction in bvec" in err("-fslgrad", bvecs, bvals)
+
+
+AMP2RESP = os.path.join(ROOT, "mrtrix3_amd", "bin", "amp2response")
+
+
+@pytest.mark.skipif(not os.path.exists(AMP2RESP), reason="amp2response not built")
+def test_amp2response_argument_errors(tmp_path):
+    """cmd/amp2response.cpp:241-300: checks made before any GPU work."""
+    amps, mask, dirs = (str(tmp_path / f) for f in ("amps.mif", "mask.mif", "dirs.mif"))
+    scheme = "\n".join(["0,0,1,0"] * 2 + ["1,0,0,3000", "0,1,0,3000", "0,0,1,3000", "0.6,0.8,0,3000"])
+    mif.write(amps, np.ones((2, 2, 2, 6), dtype=np.float32), keyval=[("dw_scheme", scheme)])
+    mif.write(mask, np.zeros((2, 2, 2), dtype=np.float32))
+    mif.write(dirs, np.ones((2, 2, 2, 3), dtype=np.float32))
+
+    def err(*args):
+        r = subprocess.run([AMP2RESP] + list(args), capture_output=True, text=True)
+        assert r.returncode == 1, r.stderr
+        return r.stderr
+
+    assert "usage" in err(amps, mask, dirs)
+    assert "input mask does not contain any voxels" in err(amps, mask, dirs, str(tmp_path / "r.txt"))
+    mif.write(mask, np.ones((2, 2, 2), dtype=np.float32))
+    assert "does not match number of b-value shells" in err(amps, mask, dirs, str(tmp_path / "r.txt"), "-lmax", "0,4,8")
+    assert "must be even" in err(amps, mask, dirs, str(tmp_path / "r.txt"), "-lmax", "0,5")
+    mif.write(dirs, np.ones((2, 2, 2, 2), dtype=np.float32))
+    assert "does not have expected dimensions" in err(amps, mask, dirs, str(tmp_path / "r.txt"))
