@@ -31,7 +31,8 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector)
-CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096, "peaks": 65536}
+CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096, "peaks": 65536,
+              "response": 8192}
 
 
 def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
@@ -672,6 +673,92 @@ def bench_peaks(args, torch, ph):
     return res
 
 
+def resp_flops_per_sample(lmax):
+    """FP64 FLOPs of one (voxel, direction) step of the amp2response accumulation
+    (cmd/amp2response.cpp:152-191 as the kernel restates it): the dot product with the fibre
+    direction (5), the direction's norm with its square root (6) and the division (1), the zonal
+    Legendre recursion to lmax (2 for degree 1, then 4 per degree: product, division, difference,
+    scaling), and the rank-1 updates of M (2 per lower-triangle entry) and b (2 per coefficient)."""
+    nz = lmax // 2 + 1
+    leg = 2 + 4 * (lmax - 1) if lmax >= 1 else 0
+    return 12 + leg + 2 * nz * (nz + 1) // 2 + 2 * nz
+
+
+def bench_response(args, torch, ph):
+    """amp2response's accumulation over the whole-brain phantom: fibre directions from the GPU CSD
+    and sh2peaks (first peak), the b=3000 shell (64 directions) at lmax 10; a step is one
+    accumulation over all 646,128 voxels (the per-shell constrained solve is timed separately)."""
+    from mrtrix3_amd import csd, peaks, resp
+    from oracle import resp_oracle as R
+    dev = torch.device("cuda:0")
+    shared = csd.CSDShared(ph.grad, ph.response)
+    dw = torch.from_numpy(ph.dw).to(dev)
+    fod, _, _ = csd.CSD(shared).run(dw, iters=False, nneg=False)
+    fib = peaks.SH2Peaks(shared.n_sh).run(fod)[:, :3].contiguous()
+    ok = torch.isfinite(fib).all(dim=1).to(torch.uint8)
+    del fod
+    vols = np.nonzero(ph.grad[:, 3] > 100)[0]
+    dirs = ph.grad[vols, :3] / np.linalg.norm(ph.grad[vols, :3], axis=1, keepdims=True)
+    lmax = 10
+    acc = resp.Amp2Response(dirs, vols, lmax)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        acc.accumulate(dw, fib, ok)
+    timer = Timer(torch, torch.cuda.current_stream(dev), args.steps)
+    holder = {}
+    wall, kern_ms = timer.run(lambda i: holder.__setitem__("r", acc.accumulate(dw, fib, ok)), "response")
+    M, b, count = holder["r"]
+    t0 = time.perf_counter()
+    rf, iters = resp.solve(lmax, M, b, True)
+    solve_s = time.perf_counter() - t0
+    samples = float(count) * len(vols)
+    fl = samples * resp_flops_per_sample(lmax)
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+    res = {
+        "metric": "amp2response voxels/s on 1 MI355X (normal-equation accumulation over a mask)",
+        "value": ph.nvox * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "amp2response on the single_shell_brain phantom (646,128 voxels), b=3000 shell "
+                               "(64 directions), lmax 10, fibre directions from GPU CSD + sh2peaks, resident in HBM"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "resp_accumulate_kernel<10> + resp_sum_kernel (+ the 36-value copy back)",
+                     "kernel_ms": kern_ms, "flop_per_launch": fl,
+                     "note": "FP64; a step is short enough that the launch, the final in-order sum and the "
+                             "copy of the sums to the host are a visible part of it"},
+        "response": {"voxels_in_mask": int(count), "solve_ms": solve_s * 1e3, "icls_iterations": iters,
+                     "rf": [float(x) for x in rf]},
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample or CPU_SAMPLE["response"]
+        print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
+        amp_h = ph.dw[:sample]
+        fib_h = fib[:sample].cpu().numpy()
+        ok_h = ok[:sample].cpu().numpy()
+        t0 = time.perf_counter()
+        Mo, bo, co = R.accumulate(amp_h, fib_h, ok_h, dirs, vols, lmax)
+        cpu_s = time.perf_counter() - t0
+        sub = torch.zeros_like(ok)
+        sub[:sample] = ok[:sample]
+        Mg, bg, cg = acc.accumulate(dw, fib, sub)
+        rel = float(max(np.abs(Mg - Mo).max() / np.abs(Mo).max(), np.abs(bg - bo).max() / np.abs(bo).max()))
+        res["cpu_baseline"] = {"value": sample / cpu_s, "unit": "voxels/s", "cores": 1, "kind": "port",
+                               "sample": "first %d voxels of the same phantom; oracle float64 numpy restatement "
+                                         "(oracle/resp_oracle.py, one voxel at a time, the reference's rotation "
+                                         "route), one thread" % sample}
+        res["parity_on_cpu_sample"] = {"max_rel_err_sums": rel, "voxels_in_mask": int(co)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -714,6 +801,9 @@ def main():
     elif args.workload == "peaks":
         ph = phantom.single_shell_brain()
         e2e = None if args.no_e2e else end_to_end_peaks(ph)
+    elif args.workload == "response":
+        ph = phantom.single_shell_brain()
+        e2e = None
     else:
         make, lmax, _, _ = csd_workload(args.workload)
         ph = make()
@@ -724,6 +814,8 @@ def main():
         out = bench_msmt(args, torch, ph)
     elif args.workload == "peaks":
         out = bench_peaks(args, torch, ph)
+    elif args.workload == "response":
+        out = bench_response(args, torch, ph)
     else:
         out = bench_csd(args, torch, args.workload, ph)
     if e2e is not None:

@@ -7,7 +7,7 @@
 // for the unit fibre direction d and gradient direction g: the reference's randomly chosen x and y
 // axes of the frame (amp2response.cpp:72-90) change nothing but rounding.  FP64 throughout; each
 // thread sums its voxels, a workgroup reduces its threads in a fixed tree, and a second kernel adds
-// the workgroups' sums in order, so the result does not depend on scheduling.
+// the workgroups' sums in a fixed order, so the result does not depend on scheduling.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -100,19 +100,27 @@ __global__ __launch_bounds__(kThreads) void resp_accumulate_kernel(RespLaunch L)
   }
 }
 
-// out[q] = sum over workgroups in order
-__global__ void resp_sum_kernel(const double* partial, int nblocks, int nv, double* out) {
-  const int q = (int)threadIdx.x;
-  if (q >= nv) return;
+// out[q] = the workgroups' sums of value q (one workgroup per value): thread t adds partials t,
+// t + 256, ... in order, then a fixed tree; the same order on every run
+__global__ __launch_bounds__(kThreads) void resp_sum_kernel(const double* partial, int nblocks, int nv, double* out) {
+  __shared__ double s_red[kThreads];
+  const int q = (int)blockIdx.x;
   double s = 0.0;
-  for (int k = 0; k < nblocks; ++k) s += partial[(size_t)k * nv + q];
-  out[q] = s;
+  for (int k = (int)threadIdx.x; k < nblocks; k += kThreads) s += partial[(size_t)k * nv + q];
+  s_red[threadIdx.x] = s;
+  __syncthreads();
+#pragma unroll
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s_red[threadIdx.x] += s_red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = s_red[0];
 }
 
 template <int LMAX>
 hipError_t launch(const RespLaunch& L, hipStream_t stream) {
   resp_accumulate_kernel<LMAX><<<dim3((unsigned)L.nblocks), dim3(kThreads), 0, stream>>>(L);
-  resp_sum_kernel<<<1, 64, 0, stream>>>(L.partial, L.nblocks, resp_values(LMAX), L.out);
+  resp_sum_kernel<<<dim3((unsigned)resp_values(LMAX)), dim3(kThreads), 0, stream>>>(L.partial, L.nblocks, resp_values(LMAX), L.out);
   return hipGetLastError();
 }
 
