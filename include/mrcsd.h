@@ -48,6 +48,8 @@ typedef struct mrcsd_options {
   double threshold;          /* -threshold, default 0    (csd.h:34), range [-1, 10] */
   int niter;                 /* -niter, default 50       (csd.h:35), range [0, 1000] */
   int bvalue_scaling;        /* 0 auto, 1 on, 2 off      (gradient.cpp:280-324) */
+  const double* shells;      /* -shells b-values; NULL = the largest shell (csd.h:57-60, shells.cpp:100-281) */
+  int n_shells;
 } mrcsd_options;
 
 void mrcsd_options_default(mrcsd_options* opt);

@@ -57,11 +57,14 @@ typedef struct mrmsmt_shared mrmsmt_shared;
  * grad [nvol][4] raw table; responses: tissue t is a [resp_rows[t]][resp_cols[t]] row-major block,
  * blocks concatenated; lmax [n_lmax] (the -lmax list) or NULL = from the responses capped at 8
  * (msmt_csd.h:102-114; n_lmax != ntissue is the reference's error);
- * directions NULL = built-in 300; norm_lambda / neg_lambda < 0 = defaults (1e-10). */
+ * directions NULL = built-in 300; norm_lambda / neg_lambda < 0 = defaults (1e-10);
+ * shells [n_shells] the b-values of a -shells option or NULL = every shell (msmt_csd.h:49-54 with
+ * DWI::ShellsOption, shells.cpp:100-281): volumes outside the selected shells get zero rows. */
 int mrmsmt_shared_create(const double* grad, int nvol, const double* responses,
                          const int* resp_rows, const int* resp_cols, int ntissue, const int* lmax,
                          int n_lmax, const double* directions, int n_directions,
-                         double norm_lambda, double neg_lambda, mrmsmt_shared** out);
+                         double norm_lambda, double neg_lambda, const double* shells, int n_shells,
+                         mrmsmt_shared** out);
 void mrmsmt_shared_destroy(mrmsmt_shared* s);
 /* nshell, nparams (sum of NforL(lmax_t)), ncon (constraint rows), lmax_out[ntissue]. */
 int mrmsmt_shared_info(const mrmsmt_shared* s, int* nshell, int* nparams, int* ncon, int* lmax_out);

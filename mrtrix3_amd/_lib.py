@@ -20,7 +20,8 @@ class Options(C.Structure):
     _fields_ = [("lmax", C.c_int), ("filter", C.POINTER(C.c_double)), ("n_filter", C.c_int),
                 ("directions", C.POINTER(C.c_double)), ("n_directions", C.c_int),
                 ("neg_lambda", C.c_double), ("norm_lambda", C.c_double),
-                ("threshold", C.c_double), ("niter", C.c_int), ("bvalue_scaling", C.c_int)]
+                ("threshold", C.c_double), ("niter", C.c_int), ("bvalue_scaling", C.c_int),
+                ("shells", C.POINTER(C.c_double)), ("n_shells", C.c_int)]
 
 
 class SharedInfo(C.Structure):

@@ -20,7 +20,7 @@ MATRIX_IDS = {"rconv": 0, "M": 1, "Mt_M": 2, "HR_trans": 3, "dwis": 4, "grad": 5
 
 class CSDShared:
     def __init__(self, grad, response, lmax=0, filter=None, directions=None, neg_lambda=1.0,
-                 norm_lambda=1.0, threshold=0.0, niter=50, bvalue_scaling=0):
+                 norm_lambda=1.0, threshold=0.0, niter=50, bvalue_scaling=0, shells=None):
         lib = _lib.load()
         grad = np.ascontiguousarray(grad, dtype=np.float64)
         if grad.ndim != 2 or grad.shape[1] != 4:
@@ -40,6 +40,11 @@ class CSDShared:
             self._keep.append(d)
             opt.directions = dptr(d)
             opt.n_directions = d.shape[0]
+        if shells is not None:
+            sh = np.ascontiguousarray(shells, dtype=np.float64).reshape(-1)
+            self._keep.append(sh)
+            opt.shells = dptr(sh)
+            opt.n_shells = sh.size
         opt.neg_lambda = float(neg_lambda)
         opt.norm_lambda = float(norm_lambda)
         opt.threshold = float(threshold)

@@ -57,7 +57,9 @@ int mrcsd_shared_create(const double* grad, int nvol, const double* response, in
                          : o.bvalue_scaling == 2 ? MR::DWI::BValueScaling::UserOff
                                                  : MR::DWI::BValueScaling::Auto;
     Matrix g = MR::DWI::get_DW_scheme(raw, nvol, scaling);
-    auto s = std::make_unique<CSD::Shared>(g);
+    std::vector<double> requested;
+    if (o.shells && o.n_shells > 0) requested.assign(o.shells, o.shells + o.n_shells);
+    auto s = std::make_unique<CSD::Shared>(g, requested);
     CSDOptions co;
     if (o.lmax) co.lmax.push_back(uint32_t(o.lmax));
     if (o.filter && o.n_filter > 0) co.filter.assign(o.filter, o.filter + o.n_filter);

@@ -178,14 +178,17 @@ int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* 
 int mrmsmt_shared_create(const double* grad, int nvol, const double* responses,
                          const int* resp_rows, const int* resp_cols, int ntissue, const int* lmax,
                          int n_lmax, const double* directions, int n_directions,
-                         double norm_lambda, double neg_lambda, mrmsmt_shared** out) {
+                         double norm_lambda, double neg_lambda, const double* shells, int n_shells,
+                         mrmsmt_shared** out) {
   return mrcsd::guarded([&]() -> int {
     if (!grad || nvol <= 0 || !responses || !resp_rows || !resp_cols || ntissue <= 0 || !out)
       return mrcsd::fail(MRCSD_ERR_ARG, "mrmsmt_shared_create: null or empty argument");
     Matrix raw(nvol, 4);
     std::memcpy(raw.data(), grad, sizeof(double) * size_t(nvol) * 4);
     Matrix g = MR::DWI::get_DW_scheme(raw, nvol);
-    auto s = std::make_unique<MR::DWI::SDeconv::MSMT_CSD::Shared>(g);
+    std::vector<double> requested;
+    if (shells && n_shells > 0) requested.assign(shells, shells + n_shells);
+    auto s = std::make_unique<MR::DWI::SDeconv::MSMT_CSD::Shared>(g, requested);
     if (lmax && n_lmax > 0) {
       for (int i = 0; i < n_lmax; ++i)
         if (lmax[i] < 0) return mrcsd::fail(MRCSD_ERR_ARG, "mrmsmt_shared_create: negative lmax");

@@ -5,7 +5,7 @@
 //   dwi2fod msmt_csd dwi.mif wm.txt wm.mif gm.txt gm.mif csf.txt csf.mif [options]
 //
 // Options kept from the reference: -mask, -lmax, -directions, -filter, -neg_lambda,
-// -norm_lambda, -threshold, -niter, -grad, -fslgrad, -bvalue_scaling, -predicted_signal, -force, -quiet,
+// -norm_lambda, -threshold, -niter, -grad, -fslgrad, -shells, -bvalue_scaling, -predicted_signal, -force, -quiet,
 // -info, -nthreads (accepted; the solve runs on the GPU).  Per-voxel semantics follow
 // dwi2fod.cpp:128-231: masked-out voxels are written as zeros, CSD voxels with a non-finite
 // signal give zeros, negative CSD signals are clamped, MSMT uses the raw signal.
@@ -133,6 +133,7 @@ struct Args {
   std::vector<std::string> pos;
   std::string mask, directions, filter, grad, predicted;
   std::vector<std::string> fslgrad;   // bvecs, bvals
+  std::vector<double> shells;         // -shells b-values
   std::vector<int> lmax;
   bool has_neg = false, has_norm = false, has_thr = false, has_niter = false, force = false;
   double neg = 0, norm = 0, thr = 0;
@@ -159,6 +160,17 @@ Args parse_args(int argc, char** argv) {
         a.fslgrad = {bvecs, next()};
       }
       else if (o == "predicted_signal") a.predicted = next();
+      else if (o == "shells") {
+        std::string cur;
+        for (char c : next() + ",") {
+          if (c == ',') {
+            if (!cur.empty()) a.shells.push_back(std::stod(cur));
+            cur.clear();
+          } else if (c != ' ') {
+            cur += c;
+          }
+        }
+      }
       else if (o == "neg_lambda") { a.has_neg = true; a.neg = std::stod(next()); }
       else if (o == "norm_lambda") { a.has_norm = true; a.norm = std::stod(next()); }
       else if (o == "threshold") { a.has_thr = true; a.thr = std::stod(next()); }
@@ -367,6 +379,7 @@ int run(int argc, char** argv) {
     if (a.has_thr) opt.threshold = a.thr;
     if (a.has_niter) opt.niter = a.niter;
     opt.bvalue_scaling = a.bscale;
+    if (!a.shells.empty()) { opt.shells = a.shells.data(); opt.n_shells = int(a.shells.size()); }
     mrcsd_shared* shared = nullptr;
     check(mrcsd_shared_create(grad.data(), nvol, response.data(), int(response.size()), &opt, &shared));
     mrcsd_shared_info si;
@@ -427,7 +440,8 @@ int run(int argc, char** argv) {
   check(mrmsmt_shared_create(grad.data(), nvol, resp.data(), rrows.data(), rcols.data(), nt,
                              a.lmax.empty() ? nullptr : a.lmax.data(), int(a.lmax.size()),
                              dirs.empty() ? nullptr : dirs.data(), ndirs,
-                             a.has_norm ? a.norm : -1.0, a.has_neg ? a.neg : -1.0, &ms));
+                             a.has_norm ? a.norm : -1.0, a.has_neg ? a.neg : -1.0,
+                             a.shells.empty() ? nullptr : a.shells.data(), int(a.shells.size()), &ms));
   int nshell = 0, nparams = 0, ncon = 0;
   std::vector<int> lm(static_cast<size_t>(nt));
   check(mrmsmt_shared_info(ms, &nshell, &nparams, &ncon, lm.data()));

@@ -11,13 +11,13 @@ namespace MR {
 namespace DWI {
 namespace SDeconv {
 
-MSMT_CSD::Shared::Shared(const Matrix& g)
+MSMT_CSD::Shared::Shared(const Matrix& g, const std::vector<double>& requested_shells)
     : grad(g),
       shells(g),
       HR_dirs(electrostatic_repulsion_300()),
       solution_min_norm_regularisation(DEFAULT_MSMTCSD_NORM_LAMBDA),
       constraint_min_norm_regularisation(DEFAULT_MSMTCSD_NEG_LAMBDA) {
-  shells.select_shells(false, false, false);
+  shells.select_shells(false, false, false, requested_shells);
 }
 
 void MSMT_CSD::Shared::set_responses(const std::vector<Matrix>& matrices) {

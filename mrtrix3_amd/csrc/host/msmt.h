@@ -23,7 +23,8 @@ class MSMT_CSD {
  public:
   class Shared {
    public:
-    explicit Shared(const Matrix& grad);                          // msmt_csd.h:49-54
+    // msmt_csd.h:49-54; requested_shells: the b-values of a -shells option (empty: all shells)
+    explicit Shared(const Matrix& grad, const std::vector<double>& requested_shells = {});
     void set_responses(const std::vector<Matrix>& matrices);      // msmt_csd.h:91-95
     void set_responses(const std::vector<std::string>& files);    // msmt_csd.h:76-89
     void init();                                                  // msmt_csd.h:100-218

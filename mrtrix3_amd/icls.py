@@ -30,7 +30,7 @@ def _register():
     _lib.register("mricls_solve", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P, _P, _P])
     _lib.register("mricls_solve_host", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P])
     _lib.register("mrmsmt_shared_create", C.c_int, [_D, C.c_int, _D, _I, _I, C.c_int, _I, C.c_int, _D,
-                                                   C.c_int, C.c_double, C.c_double, C.POINTER(_P)])
+                                                   C.c_int, C.c_double, C.c_double, _D, C.c_int, C.POINTER(_P)])
     _lib.register("mrmsmt_shared_destroy", None, [_P])
     _lib.register("mrmsmt_shared_info", C.c_int, [_P, _I, _I, _I, _I])
     _lib.register("mrmsmt_shared_matrix", C.c_int, [_P, C.c_int, _D, C.c_size_t])
@@ -97,7 +97,7 @@ class IclsProblem:
 
 
 class MSMTShared:
-    def __init__(self, grad, responses, lmax=None, directions=None, norm_lambda=-1.0, neg_lambda=-1.0):
+    def __init__(self, grad, responses, lmax=None, directions=None, norm_lambda=-1.0, neg_lambda=-1.0, shells=None):
         _register()
         lib = _lib.load()
         grad = np.ascontiguousarray(grad, dtype=np.float64)
@@ -107,12 +107,14 @@ class MSMTShared:
         cols = np.array([r.shape[1] for r in resp], dtype=np.int32)
         lm = None if lmax is None else np.ascontiguousarray(lmax, dtype=np.int32)
         d = None if directions is None else np.ascontiguousarray(directions, dtype=np.float64)
+        sh = None if shells is None else np.ascontiguousarray(shells, dtype=np.float64).reshape(-1)
         h = C.c_void_p()
         check(lib.mrmsmt_shared_create(dptr(grad), grad.shape[0], dptr(flat), rows.ctypes.data_as(_I),
                                        cols.ctypes.data_as(_I), len(resp),
                                        lm.ctypes.data_as(_I) if lm is not None else None,
                                        0 if lm is None else lm.size, dptr(d) if d is not None else None, 0 if d is None else d.shape[0],
-                                       float(norm_lambda), float(neg_lambda), C.byref(h)))
+                                       float(norm_lambda), float(neg_lambda),
+                                       dptr(sh) if sh is not None else None, 0 if sh is None else sh.size, C.byref(h)))
         self._h = h
         self.nvol = grad.shape[0]
         self.ntissue = len(resp)

@@ -218,3 +218,57 @@ def test_cli_amp2response_matches_oracle(tmp_path):
             err = np.abs(out[row, :k] - ref).max() / np.abs(ref).max()
             say("[gpu cli] amp2response shell %d constrained %s: rel %.2e" % (row, constrained, err))
             assert err <= 1e-5 and np.all(out[row, k:] == 0)
+
+
+def test_cli_shells_option(tmp_path):
+    """-shells (DWI::ShellsOption): csd on the b=3000 shell of a four-shell image equals the oracle
+    on the image reduced to that shell; msmt_csd -shells 0,1000,3000 with three-row responses
+    equals the oracle's ICLS on the problem whose b=2000 rows are zero (three tissues need three
+    shells: with two, C^T C is singular but for the regularisation)."""
+    shape = (8, 8, 4)
+    ph = phantom.multi_shell(seed=4, shape=shape, semi=None)
+    img = ph.dw.reshape(shape + (ph.dw.shape[1],))
+    dwi = str(tmp_path / "dwi.mif")
+    mif.write(dwi, img, vox=[2.0, 2.0, 2.0], keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+    b = ph.grad[:, 3]
+    # csd on b=3000
+    resp = str(tmp_path / "wm3000.txt")
+    write_response(resp, ph.response[0][3])
+    out = str(tmp_path / "fod.mif")
+    say("[gpu cli] dwi2fod csd -shells 3000 on %d voxels" % ph.nvox)
+    run(["csd", dwi, resp, out, "-shells", "3000", "-force"])
+    fod, _ = mif.read(out)
+    sel = np.nonzero((b < 10) | (np.abs(b - 3000) < 100))[0]
+    o = orc.CsdOracle(ph.grad[sel], ph.response[0][3])
+    rel = []
+    for idx in np.ndindex(shape):
+        sig = np.maximum(img[idx][sel][o.vols].astype(np.float64), 0.0)
+        F, _, _, _ = o.solve(sig)
+        rel.append(np.linalg.norm(fod[idx] - F) / max(np.linalg.norm(F), 1e-30))
+    rel = np.array(rel)
+    say("[gpu cli] csd -shells: max rel %.3e" % rel.max())
+    assert np.mean(rel <= 1e-4) >= 0.999
+    # msmt_csd on b=0, 1000 and 3000
+    args = ["msmt_csd", dwi]
+    two = [r[[0, 1, 3]] for r in ph.response]
+    outs = []
+    for name, r in zip(("wm", "gm", "csf"), two):
+        rp, op = str(tmp_path / (name + "2.txt")), str(tmp_path / (name + "2.mif"))
+        write_response(rp, r)
+        args += [rp, op]
+        outs.append(op)
+    say("[gpu cli] dwi2fod msmt_csd -shells 0,1000,3000")
+    run(args + ["-shells", "0,1000,3000", "-force"])
+    got = [mif.read(p)[0] for p in outs]
+    s = icls.MSMTShared(ph.grad, two, shells=[0, 1000, 3000])
+    assert s.nshell == 3
+    o2 = orc.IclsOracle(s.matrix("C"), s.matrix("A"), None, 0, 1e-10, 1e-10)
+    rel = []
+    for idx in np.ndindex(shape):
+        x, _, _ = o2.solve(img[idx].astype(np.float64))
+        g = np.concatenate([np.atleast_1d(got[0][idx]), np.atleast_1d(got[1][idx]).ravel(),
+                            np.atleast_1d(got[2][idx]).ravel()])
+        rel.append(np.linalg.norm(g - x) / max(np.linalg.norm(x), 1e-30))
+    rel = np.array(rel)
+    say("[gpu cli] msmt_csd -shells: max rel %.3e" % rel.max())
+    assert np.mean(rel <= 1e-4) >= 0.999

@@ -96,3 +96,34 @@ def test_gradient_table_validation():
         csd.CSDShared(np.zeros((10, 3)), [1.0, 2.0])
     with pytest.raises(MrcsdError, match="could not be classified into b-value shells"):
         csd.CSDShared(np.array([[1.0, 0, 0, 1000.0], [0, 1.0, 0, 3000.0]]), [1.0, 2.0])
+
+
+def test_shells_option_selects_the_requested_shell():
+    """-shells (DWI::ShellsOption, shells.cpp:100-281): CSD on a four-shell table uses the
+    requested shell; its set-up equals the oracle's on the table reduced to that shell."""
+    rng = np.random.default_rng(9)
+    grad = phantom.gradient_table([(0, 6), (1000, 20), (2000, 30), (3000, 45)], rng)
+    resp = phantom.zonal_response(2000, 8)
+    s = csd.CSDShared(grad, resp, shells=[2000])
+    sel = np.nonzero((grad[:, 3] == 0) | (np.abs(grad[:, 3] - 2000) < 100))[0]
+    o = orc.CsdOracle(grad[sel], resp)
+    assert s.n_dw == 30 == o.n_dw
+    assert np.array_equal(s.dwis, sel[o.vols])
+    for name, ref in (("rconv", o.rconv), ("M", o.M), ("Mt_M", o.MtM), ("HR_trans", o.HR_trans)):
+        assert np.abs(s.matrix(name) - ref).max() <= 1e-8 * np.abs(ref).max(), name
+    with pytest.raises(MrcsdError):
+        csd.CSDShared(grad, resp, shells=[1500])
+
+
+def test_msmt_shells_option_zeroes_unselected_volumes():
+    from mrtrix3_amd import icls
+    rng = np.random.default_rng(10)
+    grad = phantom.gradient_table([(0, 6), (1000, 20), (3000, 45)], rng)
+    wm = np.stack([phantom.zonal_response(b, 8) if b else np.r_[phantom.zonal_response(0, 8)[:1], np.zeros(4)]
+                   for b in (0, 3000)])
+    gm = np.array([[2.0], [0.5]])
+    s = icls.MSMTShared(grad, [wm, gm], shells=[0, 3000])
+    assert s.nshell == 2
+    C = s.matrix("C")
+    b1000 = np.abs(grad[:, 3] - 1000) < 100
+    assert np.all(C[b1000] == 0) and np.all(np.abs(C[~b1000]).sum(axis=1) > 0)
