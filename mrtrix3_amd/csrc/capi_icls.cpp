@@ -21,7 +21,7 @@ using mrcsd::Matrix;
 struct mricls_ctx {
   int device = 0;
   mrcsd::IclsDeviceProblem P{};
-  mrcsd::DeviceBuffer<double> b2d, Bt, t, G, Hx, Bx;
+  mrcsd::DeviceBuffer<double> b2d, Bt, t, G, L;
   mrcsd::DeviceBuffer<unsigned long long> counter;
 };
 
@@ -30,22 +30,6 @@ struct mrmsmt_shared {
 };
 
 namespace {
-
-// X = Y L^-1 for lower-triangular L: each row x solves x L = y (back substitution).
-Matrix right_solve_lower(const Matrix& L, const Matrix& Y) {
-  const int n = L.rows;
-  Matrix X(Y.rows, n);
-  for (int r = 0; r < Y.rows; ++r) {
-    const double* y = Y.row(r);
-    double* x = X.row(r);
-    for (int j = n - 1; j >= 0; --j) {
-      double s = y[j];
-      for (int k = j + 1; k < n; ++k) s -= x[k] * L(k, j);
-      x[j] = s / L(j, j);
-    }
-  }
-  return X;
-}
 
 int make_ctx(const Problem& pr, mricls_ctx** out) {
   const int n = int(pr.num_parameters()), nm = pr.b2d.rows, ncon = pr.B.rows;
@@ -73,8 +57,9 @@ int make_ctx(const Problem& pr, mricls_ctx** out) {
       G[size_t(a) * ldg + b] = s;
       G[size_t(b) * ldg + a] = s;
     }
-  const Matrix Hx = right_solve_lower(pr.chol_HtH, pr.b2d);
-  const Matrix Bx = right_solve_lower(pr.chol_HtH, pr.B);
+  std::vector<double> Lf(size_t(n) * n, 0.0);   // the lower factor, upper triangle zero
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c <= r; ++c) Lf[size_t(r) * n + c] = pr.chol_HtH(r, c);
 
   auto ctx = std::make_unique<mricls_ctx>();
   MRCSD_HIP(hipGetDevice(&ctx->device));
@@ -82,12 +67,11 @@ int make_ctx(const Problem& pr, mricls_ctx** out) {
   ctx->Bt.upload(Bt.data(), Bt.size());
   ctx->t.upload(t.data(), t.size());
   ctx->G.upload(G.data(), G.size());
-  ctx->Hx.upload(Hx.data(), size_t(nm) * n);
-  ctx->Bx.upload(Bx.data(), size_t(ncon) * n);
+  ctx->L.upload(Lf.data(), Lf.size());
   ctx->counter.alloc(1);
   mrcsd::IclsDeviceProblem& P = ctx->P;
   P.b2d = ctx->b2d.get(); P.Bt = ctx->Bt.get(); P.t = ctx->t.get(); P.G = ctx->G.get();
-  P.Hx = ctx->Hx.get(); P.Bx = ctx->Bx.get();
+  P.L = ctx->L.get();
   P.lambda_min = pr.lambda_min_norm;
   P.tol = pr.tol;
   P.nmeas = nm; P.n = n; P.ncon = ncon; P.ncon_pad = ldg; P.num_eq = num_eq;

@@ -14,8 +14,7 @@ struct IclsDeviceProblem {
   const double* Bt;    // [n][ncon_pad]      transposed unit-row constraints (c_u = B y_u - t)
   const double* t;     // [ncon]             scaled constraint vector (zeros when absent)
   const double* G;     // [ncon][ncon_pad]   Gram matrix B B^T (B_act B_act^T, and c = c_u + G l)
-  const double* Hx;    // [nmeas][n]         b2d L^-1: L^-T y_u = Hx^T b
-  const double* Bx;    // [ncon][n]          B L^-1:   L^-T B_act^T l = Bx_act^T l
+  const double* L;     // [n][n]             lower Cholesky factor of H^T H (+ regularisation): x = L^-T y
   double lambda_min;   // constraint min-norm regularisation added to B_act B_act^T
   double tol;
   int nmeas, n, ncon, ncon_pad, num_eq, max_niter;

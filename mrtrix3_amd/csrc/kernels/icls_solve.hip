@@ -12,8 +12,9 @@
 //   * the k x k system is solved by a Gauss-Jordan sweep, thread a owning row a;
 //   * constraint values are updated as c = c_u + G[:, act] l, which equals B x - t for
 //     x = y_u + B_act^T l;
-//   * the final x = L^-T (y_u + B_act^T l) is assembled as Hx^T b + Bx_act^T l with Hx = b2d L^-1
-//     and Bx = B L^-1 precomputed on the host, so no per-voxel triangular solve runs.
+//   * the final x = L^-T (y_u + B_act^T l) is formed as the reference forms it: y gathered from
+//     B^T, then a back substitution with L^T, one row per step (a precomputed L^-1 would lose
+//     cond(L)^2 rather than cond(L) of accuracy on nearly singular problems).
 // All arithmetic is FP64; results agree with the reference to rounding.
 #include <hip/hip_runtime.h>
 #include <limits.h>
@@ -217,17 +218,23 @@ __global__ __launch_bounds__(64) void icls_persistent_kernel(IclsDeviceProblem P
       }
     }
 
-    // ---- x = L^-T (y_u + B_act^T l) = Hx^T b + Bx_act^T l (:319, :355) ----
+    // ---- x = y_u + B_act^T l (:319), then L^T x = y solved in place (:355) ----
     wave_sync();
     s_l[tid] = tid < k ? la : 0.0;
     s_act[tid] = tid < k ? my_act : 0;
     wave_sync();
+    double yv = 0.0;   // lane j < n: y_j, then x_j
     if (tid < n) {
-      double xv = 0.0;
-      for (int r = 0; r < nm; ++r) xv = fma(P.Hx[r * n + tid], s_b[r], xv);
-      for (int a = 0; a < k; ++a) xv = fma(P.Bx[(size_t)s_act[a] * n + tid], s_l[a], xv);
-      L.x[vox * n + tid] = (float)xv;
+      yv = s_y[tid];
+      for (int a = 0; a < k; ++a) yv = fma(P.Bt[(size_t)tid * ldg + s_act[a]], s_l[a], yv);
     }
+    // back substitution by columns: step i finalises x_i and removes it from the rows above
+    for (int i = n - 1; i >= 0; --i) {
+      const double xi = lane_bcast(yv, i) / P.L[(size_t)i * n + i];
+      if (tid == i) yv = xi;
+      if (tid < i) yv = fma(-P.L[(size_t)i * n + tid], xi, yv);
+    }
+    if (tid < n) L.x[vox * n + tid] = (float)yv;
     if (tid == 0) {
       if (L.iters) L.iters[vox] = fail ? (uint16_t)0xFFFE : (uint16_t)niter;
       if (L.nactive) L.nactive[vox] = (uint16_t)k;
