@@ -100,10 +100,10 @@ def build(verbose=False):
     for name, extra in (("dwi2fod", []), ("sh2peaks", []), ("amp2response", ["gradient.cpp", "dense.cpp"])):
         cli = os.path.join(CSRC, name + ".cpp")
         exe = os.path.join(BIN, name)
-        host = [os.path.join(CSRC, "host", f) for f in ["mif.cpp"] + extra]
+        host = [os.path.join(CSRC, "host", f) for f in ["mif.cpp", "nifti.cpp"] + extra]
         cmd = [HIPCC] + HIPFLAGS + [cli] + host + ["-o", exe + ".tmp",
                                     "-L" + LIB, "-lmrcsd",
-                                    "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
+                                    "-Wl,-rpath,$ORIGIN/../lib", "-lpthread", "-lz"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))

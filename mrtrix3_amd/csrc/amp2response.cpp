@@ -23,6 +23,7 @@
 #include "host/dense.h"
 #include "host/gradient.h"
 #include "host/mif.h"
+#include "host/nifti.h"
 
 namespace {
 
@@ -131,7 +132,7 @@ int run(int argc, char** argv) {
   if (a.pos.size() != 4) throw std::runtime_error("usage: amp2response amps mask directions response [options]");
   if (exists(a.pos[3]) && !a.force)
     throw std::runtime_error("output file \"" + a.pos[3] + "\" already exists (use -force option to force overwrite)");
-  mrcsd::MifImage img = mrcsd::read_mif(a.pos[0]);
+  mrcsd::MifImage img = mrcsd::read_image(a.pos[0]);
   const int64_t nspatial = img.h.dim.size() >= 3 ? img.h.dim[0] * img.h.dim[1] * img.h.dim[2] : 0;
   const int64_t nvol_img = img.h.dim.size() >= 4 ? img.h.dim[3] : 1;
 
@@ -205,13 +206,13 @@ int run(int argc, char** argv) {
     if (l < 0 || l > 12) throw std::runtime_error("lmax must be in [0, 12] on the GPU");
 
   // mask and fibre directions (amp2response.cpp:281-300)
-  mrcsd::MifImage mask_img = mrcsd::read_mif(a.pos[1]);
+  mrcsd::MifImage mask_img = mrcsd::read_image(a.pos[1]);
   for (int d = 0; d < 3; ++d)
     if (mask_img.h.dim.size() < 3 || mask_img.h.dim[size_t(d)] != img.h.dim[size_t(d)])
       throw std::runtime_error("dimension mismatch between \"" + a.pos[0] + "\" and \"" + a.pos[1] + "\"");
   if (!(mask_img.h.dim.size() == 3 || (mask_img.h.dim.size() == 4 && mask_img.h.dim[3] == 1)))
     throw std::runtime_error("input mask must be a 3D image");
-  mrcsd::MifImage dir_img = mrcsd::read_mif(a.pos[2]);
+  mrcsd::MifImage dir_img = mrcsd::read_image(a.pos[2]);
   if (dir_img.h.dim.size() < 4 || dir_img.h.dim[3] < 3)
     throw std::runtime_error("input direction image \"" + a.pos[2] + "\" does not have expected dimensions");
   for (int d = 0; d < 3; ++d)

@@ -26,6 +26,7 @@
 #include "../../include/mrcsd.h"
 #include "../../include/mricls.h"
 #include "host/mif.h"
+#include "host/nifti.h"
 
 namespace {
 
@@ -320,7 +321,7 @@ int run(int argc, char** argv) {
     throw std::runtime_error("algorithm must be one of: csd,msmt_csd");
   Stages stages;
   console("reading image \"" + a.pos[1] + "\"");
-  mrcsd::MifImage dwi = mrcsd::read_mif(a.pos[1]);
+  mrcsd::MifImage dwi = mrcsd::read_image(a.pos[1]);
   if (dwi.h.dim.size() != 4) throw std::runtime_error("input DWI must be a 4-D image");
   const int nvol = int(dwi.h.dim[3]);
 
@@ -338,7 +339,7 @@ int run(int argc, char** argv) {
 
   std::vector<float> maskv;
   if (!a.mask.empty()) {
-    mrcsd::MifImage m = mrcsd::read_mif(a.mask);
+    mrcsd::MifImage m = mrcsd::read_image(a.mask);
     if (m.h.dim.size() < 3 || m.h.dim[0] != dwi.h.dim[0] || m.h.dim[1] != dwi.h.dim[1] || m.h.dim[2] != dwi.h.dim[2])
       throw std::runtime_error("dimension mismatch between \"" + a.pos[1] + "\" and mask \"" + a.mask + "\"");
     maskv.assign(m.data.begin(), m.data.begin() + dwi.h.dim[0] * dwi.h.dim[1] * dwi.h.dim[2]);

@@ -21,6 +21,7 @@
 
 #include "../../include/mrpeaks.h"
 #include "host/mif.h"
+#include "host/nifti.h"
 
 namespace {
 
@@ -114,7 +115,7 @@ int run(int argc, char** argv) {
   if (a.pos.size() != 2) throw std::runtime_error("usage: sh2peaks SH output [options]");
   if (exists(a.pos[1]) && !a.force)
     throw std::runtime_error("output image \"" + a.pos[1] + "\" already exists (use -force option to force overwrite)");
-  mrcsd::MifImage sh = mrcsd::read_mif(a.pos[0]);
+  mrcsd::MifImage sh = mrcsd::read_image(a.pos[0]);
   if (sh.h.dim.size() < 4) throw std::runtime_error("image \"" + a.pos[0] + "\" does not contain SH coefficients - not 4D");
   const int64_t nsh = sh.h.dim[3];
   const int lmax = lforn(nsh);
@@ -124,7 +125,7 @@ int run(int argc, char** argv) {
 
   std::vector<uint8_t> mask;
   if (!a.mask.empty()) {
-    mrcsd::MifImage m = mrcsd::read_mif(a.mask);
+    mrcsd::MifImage m = mrcsd::read_image(a.mask);
     if (m.h.dim.size() < 3 || m.h.dim[0] != sh.h.dim[0] || m.h.dim[1] != sh.h.dim[1] || m.h.dim[2] != sh.h.dim[2])
       throw std::runtime_error("dimension mismatch between \"" + a.pos[0] + "\" and mask \"" + a.mask + "\"");
     mask.resize(size_t(nspatial));
@@ -153,7 +154,7 @@ int run(int argc, char** argv) {
   if (!a.peaks.empty()) {
     if (!dirs.empty())
       throw std::runtime_error("you can't specify both a peaks file and orientations to be estimated at the same time");
-    mrcsd::MifImage p = mrcsd::read_mif(a.peaks);
+    mrcsd::MifImage p = mrcsd::read_image(a.peaks);
     if (p.h.dim.size() < 4 || p.h.dim[0] != sh.h.dim[0] || p.h.dim[1] != sh.h.dim[1] || p.h.dim[2] != sh.h.dim[2])
       throw std::runtime_error("dimension mismatch between \"" + a.pos[0] + "\" and \"" + a.peaks + "\"");
     npeaks = int(p.h.dim[3] / 3);

@@ -272,3 +272,63 @@ def test_cli_shells_option(tmp_path):
     rel = np.array(rel)
     say("[gpu cli] msmt_csd -shells: max rel %.3e" % rel.max())
     assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_cli_nifti_input_matches_mif(tmp_path):
+    """NIfTI-1 input (core/formats/nifti1*.cpp, core/file/nifti_utils.cpp): the phantom as int16
+    with scl_slope / scl_inter, in a .nii.gz with an s-form and a .nii with only a q-form, its
+    gradients as FSL bvecs / bvals in image axes, and a uint8 NIfTI mask, give the FODs of the same
+    values in a .mif with the scheme in its header."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import nifti_util
+    ph = phantom.single_shell_small()
+    shape = (16, 16, 8)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(13)
+    dw = ph.dw[:n].reshape(shape + (ph.dw.shape[1],))
+    slope, inter = np.float32(np.abs(dw).max() / 30000.0), np.float32(0.5)
+    raw = np.round((dw - inter) / slope).astype(np.int16)
+    deq = (np.float64(slope) * raw.astype(np.float64) + np.float64(inter)).astype(np.float32)
+    mask = (rng.random(shape) < 0.7).astype(np.uint8)
+    a, c = 0.3, -0.2
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]]) @ \
+        np.array([[np.cos(c), 0, np.sin(c)], [0, 1, 0], [-np.sin(c), 0, np.cos(c)]])
+    vox = np.array([2.0, 2.0, 2.5])
+    t = np.array([-30.0, 12.0, 5.0])
+    g = ph.grad
+    bvecs = R.T @ g[:, :3].T
+    bvecs[0] *= -1.0                                     # positive determinant: FSL's flipped x
+    fv, fb, resp = (str(tmp_path / f) for f in ("bvecs", "bvals", "resp.txt"))
+    np.savetxt(fv, bvecs, fmt="%.17g")
+    np.savetxt(fb, g[:, 3][None, :], fmt="%.17g")
+    write_response(resp, ph.response)
+    # reference run: .mif with the scheme in its header
+    T = np.concatenate([R, t[:, None]], axis=1)
+    dmif, mmif = str(tmp_path / "dwi.mif"), str(tmp_path / "mask.mif")
+    mif.write(dmif, deq, vox=list(vox), transform=T.tolist(), keyval=[("dw_scheme", mif.scheme_text(g))])
+    mif.write(mmif, mask.astype(np.float32), vox=list(vox), transform=T.tolist())
+    run(["csd", dmif, resp, str(tmp_path / "a.mif"), "-mask", mmif, "-force"])
+    ref, _ = mif.read(str(tmp_path / "a.mif"))
+    # s-form, gzip
+    S = np.concatenate([R * vox[None, :], t[:, None]], axis=1)
+    d1, m1 = str(tmp_path / "dwi.nii.gz"), str(tmp_path / "mask.nii")
+    nifti_util.write(d1, raw, vox=tuple(vox), sform=S, slope=float(slope), inter=float(inter))
+    nifti_util.write(m1, mask, vox=tuple(vox), sform=S)
+    # q-form only: the quaternion of R (w >= 0)
+    w = 0.5 * np.sqrt(1.0 + np.trace(R))
+    q = (np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]]) / (4 * w))
+    d2 = str(tmp_path / "dwi_q.nii")
+    nifti_util.write(d2, raw, vox=tuple(vox), quatern=tuple(q), qoffset=tuple(t), slope=float(slope), inter=float(inter))
+    say("[gpu cli] dwi2fod csd from NIfTI (s-form .nii.gz, q-form .nii) on %d voxels" % n)
+    run(["csd", d1, resp, str(tmp_path / "b.mif"), "-fslgrad", fv, fb, "-mask", m1, "-force"])
+    run(["csd", d2, resp, str(tmp_path / "c.mif"), "-fslgrad", fv, fb, "-mask", m1, "-force"])
+    for name in ("b.mif", "c.mif"):
+        got, h = mif.read(str(tmp_path / name))
+        assert got.shape == ref.shape
+        assert np.allclose(np.array(h["transform"])[:, :3], R, atol=1e-6)
+        diff = np.linalg.norm((got - ref).reshape(n, -1), axis=1)
+        scale = np.maximum(np.linalg.norm(ref.reshape(n, -1), axis=1), 1e-30)
+        rel = diff / scale
+        say("[gpu cli] %s: max rel %.3e" % (name, rel.max()))
+        assert np.mean(rel <= 1e-5) >= 0.999
+        assert np.all(got.reshape(n, -1)[mask.reshape(-1) == 0] == 0)

@@ -119,3 +119,23 @@ def test_amp2response_argument_errors(tmp_path):
     assert "must be even" in err(amps, mask, dirs, str(tmp_path / "r.txt"), "-lmax", "0,5")
     mif.write(dirs, np.ones((2, 2, 2, 2), dtype=np.float32))
     assert "does not have expected dimensions" in err(amps, mask, dirs, str(tmp_path / "r.txt"))
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="dwi2fod not built")
+def test_nifti_input_checks(tmp_path):
+    """NIfTI-1 input (core/file/nifti_utils.cpp): what the reader rejects, before any GPU work."""
+    import nifti_util
+    dwi = str(tmp_path / "dwi.nii")
+    nifti_util.write(dwi, np.ones((2, 2, 2, 7), dtype=np.float32))
+    r = subprocess.run([EXE, "csd", dwi, "r.txt", str(tmp_path / "o.mif")], capture_output=True, text=True)
+    assert r.returncode == 1 and "gradient" in r.stderr.lower()     # NIfTI holds no scheme
+    bad = str(tmp_path / "two.nii")
+    nifti_util.write(bad, np.ones((2, 2, 2), dtype=np.float32), sizeof_hdr=540)
+    r = subprocess.run([EXE, "csd", bad, "r.txt", str(tmp_path / "o.mif")], capture_output=True, text=True)
+    assert r.returncode == 1 and "NIfTI-2" in r.stderr
+    short = str(tmp_path / "short.nii.gz")
+    import gzip
+    with gzip.open(short, "wb") as f:
+        f.write(b"\0" * 100)
+    r = subprocess.run([EXE, "csd", short, "r.txt", str(tmp_path / "o.mif")], capture_output=True, text=True)
+    assert r.returncode == 1 and "too small" in r.stderr
