@@ -1,6 +1,7 @@
 // C ABI of amp2response (include/mrresp.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -15,8 +16,9 @@
 struct mrresp_ctx {
   int device = 0;
   int lmax = 0, nvol = 0, nblocks = 0;
-  mrcsd::DeviceBuffer<double> dirs, partial, out;
+  mrcsd::DeviceBuffer<double> dirs, inv_norm, partial, out;
   mrcsd::DeviceBuffer<int> vols;
+  int max_vol = 0;   // largest volume index (checked against each call's row length)
 };
 
 namespace {
@@ -65,7 +67,14 @@ int mrresp_create(const double* dirs, const int* volumes, int nvol, int lmax, mr
     ctx->nvol = nvol;
     ctx->nblocks = kRespBlocks;
     ctx->dirs.upload(dirs, size_t(nvol) * 3);
+    std::vector<double> inv(static_cast<size_t>(nvol));
+    for (int i = 0; i < nvol; ++i) {
+      const double* g = dirs + 3 * i;
+      inv[size_t(i)] = 1.0 / std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    }
+    ctx->inv_norm.upload(inv.data(), inv.size());
     ctx->vols.upload(volumes, size_t(nvol));
+    for (int i = 0; i < nvol; ++i) ctx->max_vol = std::max(ctx->max_vol, volumes[i]);
     ctx->partial.alloc(size_t(kRespBlocks) * size_t(mrcsd::resp_values(lmax)));
     ctx->out.alloc(size_t(mrcsd::resp_values(lmax)));
     *out = ctx.release();
@@ -85,10 +94,7 @@ int mrresp_accumulate(mrresp_ctx* ctx, const float* amp, int ld_amp, const float
     if (ld_fibre < 3) return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_accumulate: the fibre rows need 3 values");
     MRCSD_HIP(hipSetDevice(ctx->device));
     const int nz = mrcsd::resp_nz(ctx->lmax), nv = mrcsd::resp_values(ctx->lmax);
-    std::vector<int> vols(size_t(ctx->nvol));
-    MRCSD_HIP(hipMemcpy(vols.data(), ctx->vols.get(), vols.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (int v : vols)
-      if (v >= ld_amp) return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_accumulate: a volume index is past the row length");
+    if (ctx->max_vol >= ld_amp) return mrcsd::fail(MRCSD_ERR_ARG, "mrresp_accumulate: a volume index is past the row length");
     std::vector<double> host(size_t(nv), 0.0);
     if (nvox) {
       mrcsd::RespLaunch L{};
@@ -99,6 +105,7 @@ int mrresp_accumulate(mrresp_ctx* ctx, const float* amp, int ld_amp, const float
       L.mask = mask;
       L.nvox = nvox;
       L.dirs = ctx->dirs.get();
+      L.inv_norm = ctx->inv_norm.get();
       L.vols = ctx->vols.get();
       L.nvol = ctx->nvol;
       L.lmax = ctx->lmax;

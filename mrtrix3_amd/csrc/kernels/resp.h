@@ -17,6 +17,7 @@ struct RespLaunch {
   const uint8_t* mask;     // [nvox] or null (every voxel)
   size_t nvox;
   const double* dirs;      // [nvol][3] the shell's unit gradient directions
+  const double* inv_norm;  // [nvol] 1 / |direction|
   const int* vols;         // [nvol] their volume indices
   int nvol, lmax;
   double* partial;         // [nblocks][resp_values(lmax)] per-workgroup sums

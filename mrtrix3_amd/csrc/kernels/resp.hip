@@ -19,6 +19,16 @@ namespace {
 constexpr int kThreads = 256;
 constexpr double kY00 = 0.28209479177387814;   // 1 / sqrt(4 pi)
 
+// a / f for a divisor known at compile time: y = 1 / f folds to the correctly rounded reciprocal
+// and one product with two fused corrections gives the correctly rounded quotient (Markstein's
+// theorem; checked against the division on random significands by tests/native/div_const_check.c)
+__device__ __forceinline__ double div_const(double a, double f) {
+  const double y = 1.0 / f;
+  const double q0 = a * y;
+  const double r = fma(f, q0, -a);
+  return fma(-r, y, q0);
+}
+
 // core/math/legendre.h:100-123 for m = 0: the orthonormal P_l^0(x) for even l <= LMAX
 template <int LMAX>
 __device__ __forceinline__ void zonal_row(double x, double (&t)[LMAX / 2 + 1]) {
@@ -30,7 +40,7 @@ __device__ __forceinline__ void zonal_row(double x, double (&t)[LMAX / 2 + 1]) {
     p[1] = x * f * p[0];
 #pragma unroll
     for (int l = 2; l <= LMAX; ++l) {
-      const double v = x * p[l - 1] - p[l - 2] / f;
+      const double v = x * p[l - 1] - div_const(p[l - 2], f);
       f = sqrt((double)(4 * l * l - 1) / (double)(l * l));
       p[l] = v * f;
     }
@@ -66,8 +76,7 @@ __global__ __launch_bounds__(kThreads) void resp_accumulate_kernel(RespLaunch L)
       const double gx = L.dirs[3 * i], gy = L.dirs[3 * i + 1], gz = L.dirs[3 * i + 2];
       // |R g| is |g| for a unit fibre direction; a zero one leaves R zero (Eigen's normalize keeps a
       // zero vector), so the reference's elevation is acos(0 / 0) and so is this cosine
-      const double r = n2 > 0.0 ? sqrt(gx * gx + gy * gy + gz * gz) : 0.0;
-      const double x = fabs(dx * gx + dy * gy + dz * gz) / r;   // cos of the folded elevation
+      const double x = n2 > 0.0 ? fabs(dx * gx + dy * gy + dz * gz) * L.inv_norm[i] : NAN;   // cos of the folded elevation
       double t[NZ];
       zonal_row<LMAX>(x, t);
       const double ai = (double)a[L.vols[i]];

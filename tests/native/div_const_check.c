@@ -6,6 +6,9 @@
  *   div_const_check            every significand at exponents -100, -60, -20, 0, 20, 60, 100,
  *                              and +-0 (what the unit test runs: a few seconds)
  *   div_const_check all        every float bit pattern (2^32 per divisor; minutes on 8 threads)
+ *   div_const_check double     the double-precision form (mrtrix3_amd/csrc/kernels/resp.hip) for the
+ *                              m = 0 divisors up to lmax 12: 4M random significands per divisor at
+ *                              exponents from -300 to 300, and +-0
  *
  * Prints one line per divisor and "FAIL" lines for mismatches with finite |a| >= 2^-100; exits 1
  * on any such mismatch.  Build with -ffp-contract=off so that only the explicit fmaf calls fuse. */
@@ -62,7 +65,42 @@ static void* run(void* p) {
   return 0;
 }
 
+static int check_double(void) {
+  int failed = 0;
+  uint64_t state = 0x9E3779B97F4A7C15ull;
+  for (int l = 2; l <= 12; ++l) {
+    /* the divisor at degree l is the factor of degree l - 1 (legendre.h:118-121 with m = 0) */
+    const double f = l == 2 ? sqrt(3.0) : sqrt((double)(4 * (l - 1) * (l - 1) - 1) / (double)((l - 1) * (l - 1)));
+    const double y = 1.0 / f;
+    uint64_t bad = 0, checked = 0;
+    for (int i = 0; i < 4000000 + 2; ++i) {
+      double a;
+      if (i < 2) {
+        a = i ? -0.0 : 0.0;
+      } else {
+        state ^= state << 13; state ^= state >> 7; state ^= state << 17;
+        const uint64_t mant = state & ((1ull << 52) - 1);
+        const int ex = (int)((state >> 52) % 601) - 300;
+        const uint64_t bits = ((uint64_t)(state >> 63) << 63) | ((uint64_t)(ex + 1023) << 52) | mant;
+        memcpy(&a, &bits, 8);
+      }
+      const double ref = a / f;
+      const double q0 = a * y;
+      const double r = fma(f, q0, -a);
+      const double q = fma(-r, y, q0);
+      ++checked;
+      if (memcmp(&q, &ref, 8) != 0) ++bad;
+    }
+    printf("%s double f=%.17g checked %llu mismatches %llu\n", bad ? "FAIL" : "ok", f, (unsigned long long)checked,
+           (unsigned long long)bad);
+    failed |= bad != 0;
+  }
+  printf("11 double divisors\n");
+  return failed;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && strcmp(argv[1], "double") == 0) return check_double();
   const int all = argc > 1 && strcmp(argv[1], "all") == 0;
   job jobs[128];
   int nj = 0;

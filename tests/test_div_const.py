@@ -3,7 +3,7 @@ product and two fused corrections instead of a division sequence (div_const in
 mrtrix3_amd/csrc/kernels/peaks.hip).  tests/native/div_const_check.c compares that quotient with
 the correctly rounded a / f for every divisor up to lmax 12, over every significand at exponents
 from -100 to 100 and both zeros (its "all" mode covers all 2^32 floats: equal for every finite
-|a| >= 2^-100)."""
+|a| >= 2^-100); its "double" mode samples the double-precision form of the amp2response kernel."""
 import os
 import shutil
 import subprocess
@@ -21,3 +21,6 @@ def test_constant_divisor_quotient_is_correctly_rounded(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "66 divisors" in r.stdout and "FAIL" not in r.stdout
+    r = subprocess.run([exe, "double"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "11 double divisors" in r.stdout and "FAIL" not in r.stdout
