@@ -80,7 +80,7 @@ def test_small_volume_parity(torch_cuda, small_phantom, small_oracle):
     assert np.array_equal(iters_vol.astype(np.int64), n_ref)
 
 
-@pytest.mark.parametrize("lmax", [4, 6])
+@pytest.mark.parametrize("lmax", [2, 4, 6])
 def test_lower_lmax(torch_cuda, small_phantom, lmax):
     torch = torch_cuda
     ph = small_phantom
@@ -94,6 +94,23 @@ def test_lower_lmax(torch_cuda, small_phantom, lmax):
     _, _, ok = parity_report(fod.cpu().numpy(), F_ref, neg_lists(nm.cpu().numpy(), o.n_hr), neg_ref,
                              "lmax%d" % lmax)
     assert ok >= 0.999
+
+
+def test_lmax10_super_resolved(torch_cuda):
+    """66 coefficients from 45 DW volumes: the lmax-10 instantiation (two-wavefront workgroups)."""
+    torch = torch_cuda
+    ph = phantom.super_resolved_small()
+    say("[gpu csd] super_resolved_small lmax10: %d voxels" % ph.nvox)
+    o = orc.CsdOracle(ph.grad, ph.response, lmax=10)
+    solver = csd.CSD(csd.CSDShared(ph.grad, ph.response, lmax=10))
+    assert solver.n_sh == 66
+    fod, it, nn, nm = solver.run(torch.from_numpy(ph.dw).cuda(), negmask=True)
+    torch.cuda.synchronize()
+    F_ref, n_ref, neg_ref = oracle_solve_all(o, ph.dw, range(ph.nvox))
+    _, _, ok = parity_report(fod.cpu().numpy().astype(np.float64), F_ref,
+                             neg_lists(nm.cpu().numpy(), o.n_hr), neg_ref, "lmax10")
+    assert ok >= 0.999
+    assert np.mean(it.cpu().numpy().astype(np.int64) == n_ref) >= 0.999
 
 
 def test_lmax12_super_resolved(torch_cuda):
