@@ -6,6 +6,7 @@
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 namespace mrcsd {
 
@@ -91,6 +92,53 @@ std::vector<int64_t> file_strides(const MifHeader& h, std::vector<bool>& flip) {
     flip[order[k]] = h.layout[order[k]] < 0;
   }
   return st;
+}
+
+// Runs body(r0, r1) over [0, nrows) split into contiguous ranges on up to 8 host threads (one
+// range when the image is small).
+template <class F>
+void parallel_rows(int64_t nrows, F&& body) {
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({8, int64_t(std::thread::hardware_concurrency()), nrows / 256}));
+  if (nt <= 1) { body(0, nrows); return; }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { body(nrows * t / nt, nrows * (t + 1) / nt); });
+  for (auto& x : th) x.join();
+}
+
+// Calls row(f, c) for every row along axis 0, with f the file offset (in elements) of the row's
+// element x = 0 and c its canonical offset (axis 0 fastest).  Rows are visited with the other axes
+// nested in the order of their file strides (the smallest innermost), so consecutive rows share
+// the cache lines of an interleaved file layout; ranges of rows go to different host threads.
+template <class F>
+void for_each_row(const std::vector<int64_t>& dim, const std::vector<int64_t>& st, const std::vector<bool>& flip,
+                  F&& row) {
+  const size_t nd = dim.size();
+  std::vector<size_t> axes;
+  for (size_t a = 1; a < nd; ++a) axes.push_back(a);
+  std::sort(axes.begin(), axes.end(), [&](size_t a, size_t b) { return st[a] < st[b]; });
+  std::vector<int64_t> cst(nd, 1);
+  for (size_t a = 1; a < nd; ++a) cst[a] = cst[a - 1] * dim[a - 1];
+  int64_t nrows = 1;
+  for (size_t a : axes) nrows *= dim[a];
+  const int64_t f0 = flip[0] ? (dim[0] - 1) * st[0] : 0;
+  parallel_rows(nrows, [&](int64_t r0, int64_t r1) {
+    std::vector<int64_t> idx(nd, 0);
+    int64_t rem = r0;
+    for (size_t a : axes) { idx[a] = rem % dim[a]; rem /= dim[a]; }
+    for (int64_t r = r0; r < r1; ++r) {
+      int64_t f = f0, c = 0;
+      for (size_t a : axes) {
+        f += (flip[a] ? dim[a] - 1 - idx[a] : idx[a]) * st[a];
+        c += idx[a] * cst[a];
+      }
+      row(f, c);
+      for (size_t a : axes) {
+        if (++idx[a] < dim[a]) break;
+        idx[a] = 0;
+      }
+    }
+  });
 }
 
 }  // namespace
@@ -187,17 +235,18 @@ MifImage read_mif(const std::string& path) {
   std::vector<bool> flip;
   const auto st = file_strides(h, flip);
   img.data.resize(size_t(n));
-  const size_t nd = h.dim.size();
-  std::vector<int64_t> idx(nd, 0);
-  for (int64_t c = 0; c < n; ++c) {
-    int64_t f = 0;
-    for (size_t a = 0; a < nd; ++a) f += (flip[a] ? h.dim[a] - 1 - idx[a] : idx[a]) * st[a];
-    img.data[size_t(c)] = float(h.scale_offset + h.scale_multiplier * decode(&raw[size_t(f) * dt.bytes], dt));
-    for (size_t a = 0; a < nd; ++a) {
-      if (++idx[a] < h.dim[a]) break;
-      idx[a] = 0;
+  const int64_t nx = h.dim[0];
+  const int64_t sx = flip[0] ? -st[0] : st[0];
+  const double so = h.scale_offset, sm = h.scale_multiplier;
+  const bool plain = dt.is_float && dt.bytes == 4 && !dt.big_endian && so == 0.0 && sm == 1.0;
+  for_each_row(h.dim, st, flip, [&](int64_t f, int64_t c) {
+    float* dst = &img.data[size_t(c)];
+    if (plain) {
+      for (int64_t x = 0; x < nx; ++x, f += sx) std::memcpy(&dst[x], &raw[size_t(f) * 4], 4);
+    } else {
+      for (int64_t x = 0; x < nx; ++x, f += sx) dst[x] = float(so + sm * decode(&raw[size_t(f) * dt.bytes], dt));
     }
-  }
+  });
   h.datatype = "Float32LE";
   h.scale_offset = 0.0;
   h.scale_multiplier = 1.0;
@@ -244,16 +293,15 @@ void write_mif(const std::string& path, const MifHeader& hin, const std::vector<
   std::vector<bool> flip;
   const auto st = file_strides(h, flip);
   std::vector<float> buf(data.size());
-  std::vector<int64_t> idx(nd, 0);
-  for (size_t c = 0; c < data.size(); ++c) {
-    int64_t f = 0;
-    for (size_t a = 0; a < nd; ++a) f += idx[a] * st[a];
-    buf[size_t(f)] = data[c];
-    for (size_t a = 0; a < nd; ++a) {
-      if (++idx[a] < h.dim[a]) break;
-      idx[a] = 0;
+  const int64_t nx = h.dim[0];
+  for_each_row(h.dim, st, flip, [&](int64_t f, int64_t c) {
+    const float* src = &data[size_t(c)];
+    if (st[0] == 1) {
+      std::memcpy(&buf[size_t(f)], src, size_t(nx) * sizeof(float));
+    } else {
+      for (int64_t x = 0; x < nx; ++x, f += st[0]) buf[size_t(f)] = src[x];
     }
-  }
+  });
   out.write(reinterpret_cast<const char*>(buf.data()), std::streamsize(buf.size() * sizeof(float)));
   if (!out) throw std::runtime_error("failed writing image \"" + path + "\"");
 }
