@@ -11,6 +11,7 @@
 // signal give zeros, negative CSD signals are clamped, MSMT uses the raw signal.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -207,8 +208,14 @@ void gather(const mrcsd::MifImage& dwi, const std::vector<float>* mask, std::vec
   for (int64_t v = 0; v < nspatial; ++v)
     if (!mask || (*mask)[size_t(v)] != 0.0f) sel.push_back(v);
   rows.resize(sel.size() * size_t(nvol));
-  for (size_t i = 0; i < sel.size(); ++i)
-    for (int64_t c = 0; c < nvol; ++c) rows[i * size_t(nvol) + size_t(c)] = dwi.data[size_t(c * nspatial + sel[i])];
+  // blocks of 256 voxels: the reads of one volume and the block's rows both stay in cache
+  for (size_t i0 = 0; i0 < sel.size(); i0 += 256) {
+    const size_t i1 = std::min(sel.size(), i0 + 256);
+    for (int64_t c = 0; c < nvol; ++c) {
+      const float* src = &dwi.data[size_t(c * nspatial)];
+      for (size_t i = i0; i < i1; ++i) rows[i * size_t(nvol) + size_t(c)] = src[sel[i]];
+    }
+  }
 }
 
 mrcsd::MifHeader output_header(const mrcsd::MifHeader& in, int64_t nvol_out, const std::vector<double>& grad,
@@ -227,8 +234,13 @@ void scatter_write(const std::string& path, const mrcsd::MifHeader& h, const std
                    const float* rows, int ncoef, int ld, int off) {
   const int64_t nspatial = h.dim[0] * h.dim[1] * h.dim[2];
   std::vector<float> out(size_t(nspatial * ncoef), 0.0f);
-  for (size_t i = 0; i < sel.size(); ++i)
-    for (int c = 0; c < ncoef; ++c) out[size_t(c * nspatial + sel[i])] = rows[i * size_t(ld) + size_t(off + c)];
+  for (size_t i0 = 0; i0 < sel.size(); i0 += 256) {
+    const size_t i1 = std::min(sel.size(), i0 + 256);
+    for (int c = 0; c < ncoef; ++c) {
+      float* dst = &out[size_t(c) * size_t(nspatial)];
+      for (size_t i = i0; i < i1; ++i) dst[sel[i]] = rows[i * size_t(ld) + size_t(off + c)];
+    }
+  }
   mrcsd::write_mif(path, h, out, true);
 }
 

@@ -9,6 +9,7 @@
 // is used; -nthreads is accepted and ignored.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -26,6 +27,18 @@
 namespace {
 
 bool g_quiet = false, g_info = false;
+
+// dst[j][i] = src[i][j] for src [nr][nc] row-major, in 64 x 64 tiles so that both sides stay in
+// cache (the image's volume axis is slowest, the kernels read voxel rows).
+void transpose_blocked(const float* src, int64_t nr, int64_t nc, float* dst) {
+  constexpr int64_t B = 64;
+  for (int64_t i0 = 0; i0 < nr; i0 += B)
+    for (int64_t j0 = 0; j0 < nc; j0 += B) {
+      const int64_t i1 = std::min(nr, i0 + B), j1 = std::min(nc, j0 + B);
+      for (int64_t i = i0; i < i1; ++i)
+        for (int64_t j = j0; j < j1; ++j) dst[size_t(j * nr + i)] = src[size_t(i * nc + j)];
+    }
+}
 
 void console(const std::string& s) {
   if (!g_quiet) std::fprintf(stderr, "sh2peaks: %s\n", s.c_str());
@@ -160,14 +173,12 @@ int run(int argc, char** argv) {
     npeaks = int(p.h.dim[3] / 3);
     ld_ip = int(p.h.dim[3]);
     ipeaks.resize(size_t(nspatial) * size_t(ld_ip));
-    for (int64_t v = 0; v < nspatial; ++v)
-      for (int c = 0; c < ld_ip; ++c) ipeaks[size_t(v) * size_t(ld_ip) + size_t(c)] = p.data[size_t(c * nspatial + v)];
+    transpose_blocked(p.data.data(), ld_ip, nspatial, ipeaks.data());
   }
 
   // voxel rows (coefficient axis contiguous), every voxel of the image
   std::vector<float> rows(size_t(nspatial) * size_t(nsh));
-  for (int64_t v = 0; v < nspatial; ++v)
-    for (int64_t c = 0; c < nsh; ++c) rows[size_t(v) * size_t(nsh) + size_t(c)] = sh.data[size_t(c * nspatial + v)];
+  transpose_blocked(sh.data.data(), nsh, nspatial, rows.data());
   const auto t1 = std::chrono::steady_clock::now();
 
   mrpeaks_options opt;
@@ -192,8 +203,7 @@ int run(int argc, char** argv) {
   mrcsd::MifHeader h = sh.h;
   h.dim[3] = 3 * npeaks;
   std::vector<float> data(size_t(nspatial) * size_t(3 * npeaks));
-  for (int64_t v = 0; v < nspatial; ++v)
-    for (int c = 0; c < 3 * npeaks; ++c) data[size_t(c * nspatial + v)] = out[size_t(v) * size_t(3 * npeaks) + size_t(c)];
+  transpose_blocked(out.data(), nspatial, 3 * npeaks, data.data());
   mrcsd::write_mif(a.pos[1], h, data, true);
   const auto t4 = std::chrono::steady_clock::now();
   auto sec = [](auto x, auto y) { return std::chrono::duration<double>(y - x).count(); };

@@ -98,6 +98,24 @@ def test_cli_msmt_matches_oracle(tmp_path):
     assert np.mean(rel <= 1e-4) >= 0.999
 
 
+def count_peak_mismatches(g, ref, fod):
+    """Voxels whose reported peaks differ from the oracle's: a different set of reported peaks, or a
+    peak more than 1e-3 rad off or with an amplitude more than 1e-4 of the voxel's FOD scale off."""
+    bad = 0
+    for v in range(g.shape[0]):
+        gv, rv = g[v].reshape(-1, 3), ref[v].reshape(-1, 3)
+        if not np.array_equal(np.isfinite(gv[:, 0]), np.isfinite(rv[:, 0])):
+            bad += 1
+            continue
+        for i in np.nonzero(np.isfinite(rv[:, 0]))[0]:
+            c = abs(gv[i] @ rv[i]) / (np.linalg.norm(gv[i]) * np.linalg.norm(rv[i]))
+            scale = max(np.linalg.norm(rv[i]), 0.28 * np.linalg.norm(fod[v]))   # the voxel's FOD scale
+            if np.arccos(min(1.0, c)) > 1e-3 or abs(np.linalg.norm(gv[i]) - np.linalg.norm(rv[i])) > 1e-4 * scale:
+                bad += 1
+                break
+    return bad
+
+
 def test_cli_sh2peaks_matches_oracle(tmp_path):
     """sh2peaks SH peaks -num 2 -mask: .mif in and out (cmd/sh2peaks.cpp), against the oracle."""
     exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "sh2peaks")
@@ -121,20 +139,23 @@ def test_cli_sh2peaks_matches_oracle(tmp_path):
     assert got.shape == shape + (6,)
     ref, _ = orc.sh2peaks(fod, npeaks=2, mask=mask.reshape(-1).astype(np.uint8), nthreads=8)
     g = got.transpose(2, 1, 0, 3).reshape(n, 6)
-    bad = 0
-    for v in range(n):
-        gv, rv = g[v].reshape(2, 3), ref[v].reshape(2, 3)
-        if not np.array_equal(np.isfinite(gv[:, 0]), np.isfinite(rv[:, 0])):
-            bad += 1
-            continue
-        for i in np.nonzero(np.isfinite(rv[:, 0]))[0]:
-            c = abs(gv[i] @ rv[i]) / (np.linalg.norm(gv[i]) * np.linalg.norm(rv[i]))
-            scale = max(np.linalg.norm(rv[i]), 0.28 * np.linalg.norm(fod[v]))   # the voxel's FOD scale
-            if np.arccos(min(1.0, c)) > 1e-3 or abs(np.linalg.norm(gv[i]) - np.linalg.norm(rv[i])) > 1e-4 * scale:
-                bad += 1
-                break
+    bad = count_peak_mismatches(g, ref, fod)
     say("[gpu cli] sh2peaks: %d of %d voxels differ" % (bad, n))
     assert bad <= 0.001 * n + 1
+    # -peaks: the first run's output as the directions to match (cmd/sh2peaks.cpp:160-170)
+    out2 = str(tmp_path / "peaks2.mif")
+    r = subprocess.run([exe, shp, out2, "-peaks", out, "-mask", msk, "-force"], capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr
+    got2, _ = mif.read(out2)
+    assert got2.shape == shape + (6,)
+    g2 = got2.transpose(2, 1, 0, 3).reshape(n, 6)
+    ref2, _ = orc.sh2peaks(fod, npeaks=2, peaks_image=np.ascontiguousarray(g, dtype=np.float32),
+                           mask=mask.reshape(-1).astype(np.uint8), nthreads=8)
+    bad2 = count_peak_mismatches(g2, ref2, fod)
+    say("[gpu cli] sh2peaks -peaks: %d of %d voxels differ" % (bad2, n))
+    assert bad2 <= 0.001 * n + 1
+    assert np.all(np.isnan(g2[mask.reshape(-1) == 0]))
     assert np.all(np.isnan(g[mask.reshape(-1) == 0]))
 
 
