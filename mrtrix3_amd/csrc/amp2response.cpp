@@ -227,10 +227,8 @@ int run(int argc, char** argv) {
 
   // voxel rows (volume axis contiguous) on the device
   std::vector<float> amp(size_t(nspatial) * size_t(nvol_img)), fib(size_t(nspatial) * 3);
-  for (int64_t c = 0; c < nvol_img; ++c)
-    for (int64_t v = 0; v < nspatial; ++v) amp[size_t(v * nvol_img + c)] = img.data[size_t(c * nspatial + v)];
-  for (int c = 0; c < 3; ++c)
-    for (int64_t v = 0; v < nspatial; ++v) fib[size_t(v * 3 + c)] = dir_img.data[size_t(c * nspatial + v)];
+  mrcsd::transpose_blocked(img.data.data(), nvol_img, nspatial, amp.data());
+  mrcsd::transpose_blocked(dir_img.data.data(), 3, nspatial, fib.data());
   const auto t1 = std::chrono::steady_clock::now();
   float *d_amp = nullptr, *d_fib = nullptr;
   uint8_t* d_mask = nullptr;

@@ -306,4 +306,14 @@ void write_mif(const std::string& path, const MifHeader& hin, const std::vector<
   if (!out) throw std::runtime_error("failed writing image \"" + path + "\"");
 }
 
+void transpose_blocked(const float* src, int64_t nr, int64_t nc, float* dst) {
+  constexpr int64_t B = 64;
+  for (int64_t i0 = 0; i0 < nr; i0 += B)
+    for (int64_t j0 = 0; j0 < nc; j0 += B) {
+      const int64_t i1 = std::min(nr, i0 + B), j1 = std::min(nc, j0 + B);
+      for (int64_t i = i0; i < i1; ++i)
+        for (int64_t j = j0; j < j1; ++j) dst[size_t(j * nr + i)] = src[size_t(i * nc + j)];
+    }
+}
+
 }  // namespace mrcsd

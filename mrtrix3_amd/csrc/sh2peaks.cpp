@@ -28,18 +28,6 @@ namespace {
 
 bool g_quiet = false, g_info = false;
 
-// dst[j][i] = src[i][j] for src [nr][nc] row-major, in 64 x 64 tiles so that both sides stay in
-// cache (the image's volume axis is slowest, the kernels read voxel rows).
-void transpose_blocked(const float* src, int64_t nr, int64_t nc, float* dst) {
-  constexpr int64_t B = 64;
-  for (int64_t i0 = 0; i0 < nr; i0 += B)
-    for (int64_t j0 = 0; j0 < nc; j0 += B) {
-      const int64_t i1 = std::min(nr, i0 + B), j1 = std::min(nc, j0 + B);
-      for (int64_t i = i0; i < i1; ++i)
-        for (int64_t j = j0; j < j1; ++j) dst[size_t(j * nr + i)] = src[size_t(i * nc + j)];
-    }
-}
-
 void console(const std::string& s) {
   if (!g_quiet) std::fprintf(stderr, "sh2peaks: %s\n", s.c_str());
 }
@@ -173,12 +161,12 @@ int run(int argc, char** argv) {
     npeaks = int(p.h.dim[3] / 3);
     ld_ip = int(p.h.dim[3]);
     ipeaks.resize(size_t(nspatial) * size_t(ld_ip));
-    transpose_blocked(p.data.data(), ld_ip, nspatial, ipeaks.data());
+    mrcsd::transpose_blocked(p.data.data(), ld_ip, nspatial, ipeaks.data());
   }
 
   // voxel rows (coefficient axis contiguous), every voxel of the image
   std::vector<float> rows(size_t(nspatial) * size_t(nsh));
-  transpose_blocked(sh.data.data(), nsh, nspatial, rows.data());
+  mrcsd::transpose_blocked(sh.data.data(), nsh, nspatial, rows.data());
   const auto t1 = std::chrono::steady_clock::now();
 
   mrpeaks_options opt;
@@ -203,7 +191,7 @@ int run(int argc, char** argv) {
   mrcsd::MifHeader h = sh.h;
   h.dim[3] = 3 * npeaks;
   std::vector<float> data(size_t(nspatial) * size_t(3 * npeaks));
-  transpose_blocked(out.data(), nspatial, 3 * npeaks, data.data());
+  mrcsd::transpose_blocked(out.data(), nspatial, 3 * npeaks, data.data());
   mrcsd::write_mif(a.pos[1], h, data, true);
   const auto t4 = std::chrono::steady_clock::now();
   auto sec = [](auto x, auto y) { return std::chrono::duration<double>(y - x).count(); };

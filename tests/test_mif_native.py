@@ -120,3 +120,14 @@ def test_native_3d_and_truncated(mif_copy, tmp_path):
         f.write(raw[:-4])
     r = subprocess.run([mif_copy, src, out], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "too small" in r.stderr
+
+
+@pytest.mark.parametrize("shape", [(13, 61, 47, 5), (70, 3, 2, 130), (1, 1, 1, 3)])
+def test_native_transpose_to_voxel_rows(mif_copy, tmp_path, shape):
+    # ragged 64 x 64 tiles on both sides; the commands' image <-> voxel-row conversion
+    data = np.random.default_rng(11).standard_normal(shape).astype(np.float32)
+    src, out = str(tmp_path / "t.mif"), str(tmp_path / "t.raw")
+    write_raw(src, data, ["+1", "+2", "+3", "+0"], "<f4")
+    subprocess.run([mif_copy, "--transpose", src, out], check=True, timeout=60)
+    rows = np.fromfile(out, dtype=np.float32).reshape(-1, shape[3])
+    assert np.array_equal(rows, data.transpose(2, 1, 0, 3).reshape(-1, shape[3]))
