@@ -6,8 +6,8 @@
  *   - sh2peaks' Processor (cmd/sh2peaks.cpp:145-300): the search from every direction, the
  *     de-duplication (|dot| > 0.99), the threshold and the choice of the reported peaks
  *   - Math::SH::get_peak and Math::SH::derivatives (core/math/SH.h:514-647)
- * computed in single precision as the reference computes them.  Not provided: the -fast lookup
- * table (Math::SH::PrecomputedAL, SH.h:402-498).
+ * computed in single precision as the reference computes them, or (-fast) from the reference's
+ * lookup table of Legendre values (Math::SH::PrecomputedAL, SH.h:402-470).
  */
 #ifndef MRPEAKS_H
 #define MRPEAKS_H
@@ -32,7 +32,10 @@ typedef struct mrpeaks_ctx mrpeaks_ctx;
  *               60 directions (sh2peaks.cpp:363-424); at most 512
  *   directions  [n_directions][2] (phi, theta) in radians (-direction, converted from degrees by
  *               the caller as sh2peaks.cpp:331 does), or NULL; reports, for each, the peak most
- *               nearly parallel to it */
+ *               nearly parallel to it
+ *   fast        -fast: nonzero takes the associated Legendre values from a table of 512 elevations
+ *               (n pi / 511, built at create) by linear interpolation, as PrecomputedAL does
+ *               (SH.h:418-470); 0 (the default) evaluates them exactly */
 typedef struct mrpeaks_options {
   int n_sh;
   int npeaks;
@@ -41,6 +44,7 @@ typedef struct mrpeaks_options {
   int n_seeds;
   const float* directions;
   int n_directions;
+  int fast;
 } mrpeaks_options;
 
 void mrpeaks_options_default(mrpeaks_options* opt);

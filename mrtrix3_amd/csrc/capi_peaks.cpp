@@ -36,7 +36,49 @@ struct mrpeaks_ctx {
   mrcsd::DeviceBuffer<float> dirs;      // [npeaks][3] (-direction)
   mrcsd::DeviceBuffer<unsigned long long> counter;   // the search kernel's work counter
   mrcsd::DeviceBuffer<float4> work;                  // search results, grown on demand
+  mrcsd::DeviceBuffer<float> fast_table;             // -fast lookup table, or empty
 };
+
+// Legendre::Plm_sph from cos(elevation) alone (legendre.h:98-122), in single precision with the
+// double steps of the reference: P_l^m for l = m..lmax into a[l].
+void plm_sph_x(float* a, int lmax, int m, float x) {
+  const float x2 = x * x;
+  if (m && double(x2) >= 1.0) {
+    for (int n = m; n <= lmax; ++n) a[n] = 0.0f;
+    return;
+  }
+  a[m] = float(0.282094791773878);
+  if (m) {
+    double h = 1.0;   // Plm_sph_helper(1 - x^2, 2m) (legendre.h:66-69)
+    for (double k = 2.0; k <= 2.0 * m; k += 2.0) h = (1.0 - double(x2)) * (k - 1.0) / k * h;   // innermost first
+    a[m] = float(double(a[m]) * std::sqrt(double(float(2 * m + 1)) * h));
+  }
+  if (m & 1) a[m] = -a[m];
+  if (lmax == m) return;
+  float f = std::sqrt(float(2 * m + 3));
+  a[m + 1] = x * f * a[m];
+  for (int n = m + 2; n <= lmax; ++n) {
+    a[n] = x * a[n - 1] - a[n - 2] / f;
+    f = std::sqrt(float(4 * n * n - 1) / float(n * n - m * m));
+    a[n] *= f;
+  }
+}
+
+// PrecomputedAL::init (SH.h:418-434) with sh2peaks' 512 elevations: row n holds the values at
+// elevation n pi / 511, entry index_mpos(l, m) = l^2/4 + m.
+std::vector<float> fast_table(int lmax) {
+  const int nal = (lmax / 2 + 1) * (lmax / 2 + 1);
+  const float inc = float(M_PI / (mrcsd::kPeaksFastRows - 1));
+  std::vector<float> t(size_t(mrcsd::kPeaksFastRows) * size_t(nal)), buf(size_t(lmax) + 1);
+  for (int n = 0; n < mrcsd::kPeaksFastRows; ++n) {
+    const float cos_el = std::cos(float(n) * inc);
+    for (int m = 0; m <= lmax; ++m) {
+      plm_sph_x(buf.data(), lmax, m, cos_el);
+      for (int l = (m & 1) ? m + 1 : m; l <= lmax; l += 2) t[size_t(n) * nal + size_t(l * l / 4 + m)] = buf[size_t(l)];
+    }
+  }
+  return t;
+}
 
 extern "C" {
 
@@ -80,6 +122,10 @@ int mrpeaks_create(const mrpeaks_options* opt, mrpeaks_ctx** out) {
       ctx->dirs.upload(d.data(), d.size());
       ctx->fixed_dirs = true;
     }
+    if (opt->fast) {
+      const auto t = fast_table(lmax);
+      ctx->fast_table.upload(t.data(), t.size());
+    }
     ctx->counter.alloc(1);
     *out = ctx.release();
     return MRCSD_OK;
@@ -106,6 +152,7 @@ int mrpeaks_run(mrpeaks_ctx* ctx, const float* sh, int ld_sh, const uint8_t* mas
     L.sh = sh; L.ld_sh = ld_sh; L.n_sh = ctx->n_sh; L.lmax = ctx->lmax; L.nvox = nvox;
     L.seeds = ctx->seeds.get(); L.nseeds = ctx->nseeds; L.npeaks = ctx->npeaks; L.threshold = ctx->threshold;
     L.match = peaks_in ? peaks_in : (ctx->fixed_dirs ? ctx->dirs.get() : nullptr);
+    L.fast_table = ctx->fast_table.size() ? ctx->fast_table.get() : nullptr;
     L.ld_match = peaks_in ? ld_peaks_in : 0;
     L.mask = mask; L.out = out;
     L.stats = reinterpret_cast<unsigned long long*>(stats);

@@ -28,7 +28,11 @@ struct PeaksLaunch {
   unsigned long long* counter;  // search work counter
   float4* work;                 // [work_items] search results: amplitude, x, y, z
   size_t work_items;            // capacity of `work`; voxels are processed in chunks that fit
+  const float* fast_table;      // -fast: [kPeaksFastRows][(lmax/2+1)^2] Legendre values, or null (exact)
 };
+
+// rows of the -fast lookup table: elevations n pi / (kPeaksFastRows - 1) (SH.h:406, 418-434)
+constexpr int kPeaksFastRows = 512;
 
 // Results of one (voxel, search direction) pair per work item; the buffer a launch needs for nvox
 // voxels at most (the launch processes chunks of work_items / nseeds voxels).

@@ -37,6 +37,7 @@ constexpr double kMaxDirChange = 0.2;    // SH.h:23
 constexpr double kAngleTolerance = 1e-4; // SH.h:24
 constexpr double kDotThreshold = 0.99;   // sh2peaks.cpp:25
 constexpr double kSqrt2 = 1.41421356237309504880;   // Math::sqrt2 (core/math/math.h:43)
+constexpr double kPi = 3.14159265358979323846;   // Math::pi (core/math/math.h)
 
 constexpr int nforl(int L) { return (L + 1) * (L + 2) / 2; }
 constexpr int nforl_mpos(int L) { return (L / 2 + 1) * (L / 2 + 1); }
@@ -107,15 +108,45 @@ __device__ __forceinline__ void plm_all(float (&AL)[nforl_mpos(LMAX)], float x, 
 // m >= 2 come from the m = 0 and m = 1 values by the recurrence X(m+1) = 2 cos(az) X(m) - X(m-1)
 // in double (its error, below m^2 double ulps, leaves the float values as the directly evaluated
 // ones but for rounding ties) instead of sixteen sin / cos calls.
+// SH.h:436-470 (PrecomputedAL::set / get, the -fast option): the values at elevation el by linear
+// interpolation between the two rows of tab ([kPeaksFastRows][nforl_mpos(LMAX)], row n at elevation
+// n pi / 511) around it; the first or last row alone outside [0, pi).
 template <int LMAX>
+__device__ __forceinline__ void fast_al(float (&AL)[nforl_mpos(LMAX)], const float* __restrict__ tab, float el) {
+  const float inc = (float)(kPi / (kPeaksFastRows - 1));
+  float f2 = el / inc, f1;
+  int i = (int)f2;
+  if (i < 0) {
+    i = 0; f1 = 1.0f; f2 = 0.0f;
+  } else if (i >= kPeaksFastRows - 1) {
+    i = kPeaksFastRows - 1; f1 = 1.0f; f2 = 0.0f;
+  } else {
+    f2 -= (float)i;
+    f1 = (float)(1.0 - (double)f2);
+  }
+  const float* p1 = tab + i * nforl_mpos(LMAX);
+  if (f2 != 0.0f) {
+    const float* p2 = p1 + nforl_mpos(LMAX);
+#pragma unroll
+    for (int k = 0; k < nforl_mpos(LMAX); ++k) AL[k] = __fmul_rn(f1, p1[k]) + __fmul_rn(f2, p2[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < nforl_mpos(LMAX); ++k) AL[k] = f1 * p1[k];
+  }
+}
+
+template <int LMAX, bool FAST = false>
 __device__ __forceinline__ void derivatives(const float (&c)[nforl(LMAX)], float sel, float cel, float saz1, float caz1,
-                                            float (&out)[6]) {
+                                            float (&out)[6], float el = 0.0f, const float* tab = nullptr) {
   const double twoc = 2.0 * (double)caz1;
   double cprev = kSqrt2, sprev = 0.0;                                   // m = 0
   double cm = kSqrt2 * (double)caz1, sm = kSqrt2 * (double)saz1;        // m = 1
   const bool atpole = (double)sel < 1e-4;
   float AL[nforl_mpos(LMAX)];
-  plm_all<LMAX>(AL, cel, sel * sel, std::make_integer_sequence<int, LMAX + 1>{});
+  if constexpr (FAST)
+    fast_al<LMAX>(AL, tab, el);
+  else
+    plm_all<LMAX>(AL, cel, sel * sel, std::make_integer_sequence<int, LMAX + 1>{});
   float dSH_del = 0.0f, dSH_daz = 0.0f, d2SH_del2 = 0.0f, d2SH_deldaz = 0.0f, d2SH_daz2 = 0.0f;
   float amplitude = c[0] * AL[0];
 #pragma unroll
@@ -191,15 +222,19 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {   // 
 
 // One iteration of SH.h:514-559 from (x, y, z): the amplitude at the current direction into amp,
 // the direction updated in place; true when the step was below the angle tolerance (converged).
-template <int LMAX>
-__device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, float& amp) {
+template <int LMAX, bool FAST>
+__device__ __forceinline__ bool newton_step(const float (&c)[nforl(LMAX)], float& x, float& y, float& z, float& amp,
+                                            const float* tab) {
   // the reference takes az = atan2(y, x), el = atan2(hypot(x, y), z) and their sines and cosines;
   // for the unit direction these are hypot(x, y), z, y / hypot and x / hypot (az = 0 at the pole)
   const float rho = hypotf(x, y);
   const float sel = rho, cel = z;
   const float caz = rho > 0.0f ? x / rho : 1.0f, saz = rho > 0.0f ? y / rho : 0.0f;
   float d[6];
-  derivatives<LMAX>(c, sel, cel, saz, caz, d);
+  if constexpr (FAST)
+    derivatives<LMAX, true>(c, sel, cel, saz, caz, d, atan2f(rho, z), tab);   // the table's row from el itself
+  else
+    derivatives<LMAX>(c, sel, cel, saz, caz, d);
   float del = sqrtf(d[1] * d[1] + d[2] * d[2]);
   float daz = 0.0f;
   if (del != 0.0f) {
@@ -244,7 +279,7 @@ static_assert(kGrabTail >= 64 && kGrab >= kGrabTail, "a reservation block holds 
 // Searches of voxels [v0, v0 + nv): item g = (v0 + g / nseeds, seed g % nseeds), result in work[g].
 // Up to lmax 8 the kernel is held to 168 registers (three wavefronts per SIMD, a few spills), which
 // beats two wavefronts without spills; at lmax 10 two fit, at lmax 12 one.
-template <int LMAX>
+template <int LMAX, bool FAST>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 3 : (LMAX == 10 ? 2 : 1)))) void peaks_search_kernel(PeaksLaunch L, size_t v0, size_t nv) {
   constexpr int NSH = nforl(LMAX);
   const int lane = threadIdx.x;
@@ -322,7 +357,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LMAX <= 8 ? 
     }
     if (busy) {
       float amp;
-      const bool conv = newton_step<LMAX>(c, x, y, z, amp);
+      const bool conv = newton_step<LMAX, FAST>(c, x, y, z, amp, L.fast_table);
       ++it;
       if (conv || it == 50) {
         L.work[item] = conv ? make_float4(amp, x, y, z) : make_float4(NAN, NAN, NAN, NAN);
@@ -497,11 +532,15 @@ hipError_t launch_lmax(const PeaksLaunch& L, hipStream_t stream) {
   for (size_t v0 = 0; v0 < L.nvox; v0 += chunk) {
     const size_t nv = L.nvox - v0 < chunk ? L.nvox - v0 : chunk;
     size_t g1 = 0, g2 = 0;
-    hipError_t e = persistent_grid(peaks_search_kernel<LMAX>, nv * (size_t)L.nseeds / 64 + 1, &g1);
+    hipError_t e = L.fast_table ? persistent_grid(peaks_search_kernel<LMAX, true>, nv * (size_t)L.nseeds / 64 + 1, &g1)
+                                : persistent_grid(peaks_search_kernel<LMAX, false>, nv * (size_t)L.nseeds / 64 + 1, &g1);
     if (e != hipSuccess) return e;
     if ((e = persistent_grid(peaks_select_kernel, nv, &g2)) != hipSuccess) return e;
     reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
-    peaks_search_kernel<LMAX><<<dim3((unsigned)g1), dim3(64), 0, stream>>>(L, v0, nv);
+    if (L.fast_table)
+      peaks_search_kernel<LMAX, true><<<dim3((unsigned)g1), dim3(64), 0, stream>>>(L, v0, nv);
+    else
+      peaks_search_kernel<LMAX, false><<<dim3((unsigned)g1), dim3(64), 0, stream>>>(L, v0, nv);
     peaks_select_kernel<<<dim3((unsigned)g2), dim3(64), 0, stream>>>(L, v0, nv);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }

@@ -5,8 +5,8 @@
 //   sh2peaks SH output [-num n] [-direction phi theta]... [-peaks image] [-threshold value]
 //            [-seeds file] [-mask image] [-fast] [-force] [-quiet] [-info] [-nthreads n]
 //
-// -fast (the reference's approximate Legendre lookup table) is accepted and the exact evaluation
-// is used; -nthreads is accepted and ignored.
+// -fast takes the Legendre values from the reference's lookup table (SH.h:402-470); -nthreads is
+// accepted and ignored.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,7 +73,7 @@ struct Args {
   std::string peaks, seeds, mask;
   std::vector<std::pair<float, float>> directions;
   int num = 3;
-  bool has_num = false, has_thr = false, force = false;
+  bool has_num = false, has_thr = false, force = false, fast = false;
   double thr = -INFINITY;
 };
 
@@ -97,7 +97,7 @@ Args parse_args(int argc, char** argv) {
       else if (o == "threshold") { a.has_thr = true; a.thr = std::stod(next()); }
       else if (o == "seeds") a.seeds = next();
       else if (o == "mask") a.mask = next();
-      else if (o == "fast") info("-fast: the exact evaluation is used");
+      else if (o == "fast") a.fast = true;
       else if (o == "nthreads") next();
       else if (o == "force") a.force = true;
       else if (o == "quiet") g_quiet = true;
@@ -174,6 +174,7 @@ int run(int argc, char** argv) {
   opt.n_sh = int(nsh);
   opt.npeaks = npeaks;
   opt.threshold = float(a.thr);
+  opt.fast = a.fast ? 1 : 0;
   if (!seeds.empty()) { opt.seeds = seeds.data(); opt.n_seeds = int(seeds.size() / 2); }
   if (!dirs.empty()) { opt.directions = dirs.data(); opt.n_directions = int(dirs.size() / 2); }
   mrpeaks_ctx* ctx = nullptr;

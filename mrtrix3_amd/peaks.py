@@ -24,7 +24,7 @@ _registered = False
 class Options(C.Structure):
     _fields_ = [("n_sh", C.c_int), ("npeaks", C.c_int), ("threshold", C.c_float),
                 ("seeds", C.POINTER(C.c_float)), ("n_seeds", C.c_int),
-                ("directions", C.POINTER(C.c_float)), ("n_directions", C.c_int)]
+                ("directions", C.POINTER(C.c_float)), ("n_directions", C.c_int), ("fast", C.c_int)]
 
 
 def _register():
@@ -45,7 +45,7 @@ def _is_torch(x):
 
 
 class SH2Peaks:
-    def __init__(self, n_sh, npeaks=3, threshold=-math.inf, seeds=None, directions=None):
+    def __init__(self, n_sh, npeaks=3, threshold=-math.inf, seeds=None, directions=None, fast=False):
         _register()
         lib = _lib.load()
         opt = Options()
@@ -53,6 +53,7 @@ class SH2Peaks:
         opt.n_sh = int(n_sh)
         opt.npeaks = int(npeaks)
         opt.threshold = float(threshold)
+        opt.fast = 1 if fast else 0
         self._keep = []
         if seeds is not None:
             s = np.ascontiguousarray(seeds, dtype=np.float32).reshape(-1, 2)

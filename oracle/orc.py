@@ -76,6 +76,10 @@ _SIG = {
     "orc_sh2peaks_voxel": (C.c_int, [FP, C.c_int, C.c_int, FP, C.c_int, C.c_int, C.c_float, FP, FP, I]),
     "orc_sh2peaks_run": (C.c_longlong, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_int, FP, C.c_int, C.c_int,
                                         C.c_float, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "orc_sh2peaks_run_fast": (C.c_longlong, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_int, FP, C.c_int, C.c_int,
+                                             C.c_float, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                             C.c_void_p]),
+    "orc_peaks_fast_table": (C.c_int, [C.c_int, C.c_void_p]),
 }
 
 
@@ -343,9 +347,18 @@ def sh_get_peak_f(sh, lmax, direction):
     return float(a), d, it.value
 
 
+def peaks_fast_table(lmax, generic=False):
+    """The -fast lookup table [512, (lmax/2+1)^2] float32 (SH.h:418-434)."""
+    nal = (lmax // 2 + 1) ** 2
+    tab = np.empty((512, nal), dtype=np.float32)
+    assert lib(False, generic).orc_peaks_fast_table(int(lmax), tab.ctypes.data) == nal
+    return tab
+
+
 def sh2peaks(sh, lmax=None, seeds=None, npeaks=3, threshold=-np.inf, directions=None, peaks_image=None,
-             mask=None, nthreads=0, generic=False):
+             mask=None, nthreads=0, generic=False, fast=False):
     """sh [nvox, nsh] float32 -> peaks [nvox, 3 npeaks] (amplitude x direction, NaN past the last).
+    fast: Legendre values from the 512-elevation lookup table (-fast).
     directions: [k, 3] directions reported for every voxel (-direction; npeaks = k);
     peaks_image: [nvox, 3 npeaks] per-voxel directions (-peaks).  Returns (peaks, Newton iterations)."""
     sh = _arr(np.atleast_2d(sh), np.float32)
@@ -365,7 +378,9 @@ def sh2peaks(sh, lmax=None, seeds=None, npeaks=3, threshold=-np.inf, directions=
         ld_match = m.shape[1]
     out = np.empty((nvox, 3 * npeaks), dtype=np.float32)
     mk = _arr(mask, np.uint8) if mask is not None else None
-    it = lib(False, generic).orc_sh2peaks_run(sh.ctypes.data, nsh, nsh, nvox, int(lmax), _f(seeds), seeds.shape[0], int(npeaks),
-                                float(threshold), m.ctypes.data if m is not None else None, ld_match,
-                                mk.ctypes.data if mk is not None else None, int(nthreads), out.ctypes.data)
+    tab = peaks_fast_table(int(lmax), generic) if fast else None
+    it = lib(False, generic).orc_sh2peaks_run_fast(sh.ctypes.data, nsh, nsh, nvox, int(lmax), _f(seeds), seeds.shape[0],
+                                int(npeaks), float(threshold), m.ctypes.data if m is not None else None, ld_match,
+                                mk.ctypes.data if mk is not None else None, int(nthreads),
+                                tab.ctypes.data if tab is not None else None, out.ctypes.data)
     return out, int(it)

@@ -8,6 +8,8 @@
  *   Math::SH::derivatives                         core/math/SH.h:568-647
  *   Math::SH::get_peak                            core/math/SH.h:514-559 (constants :23-24)
  *   sh2peaks Processor / check_input / Direction  cmd/sh2peaks.cpp:93-105, 160-300
+ *   -fast: Math::SH::PrecomputedAL (512 elevations) SH.h:402-470, Legendre::Plm_sph (x only)
+ *          legendre.h:98-122, its use in derivatives() SH.h:589-594
  */
 #define _GNU_SOURCE
 #include "peaks_oracle.h"
@@ -48,9 +50,74 @@ void orc_plm_sph_f(float* array, int lmax, int m, float x, float sin_x) {
   }
 }
 
+/* legendre.h:98-122: the same recursion from cos(elevation) alone (sin^2 as 1 - x^2) */
+static void plm_sph_x_f(float* array, int lmax, int m, float x) {
+  const float x2 = x * x;
+  if (m && (double)x2 >= 1.0) {
+    for (int n = m; n <= lmax; ++n) array[n] = 0.0f;
+    return;
+  }
+  array[m] = (float)0.282094791773878;
+  if (m) array[m] = (float)((double)array[m] * sqrt((double)(float)(2 * m + 1) * plm_helper(1.0 - (double)x2, 2.0 * m)));
+  if (m & 1) array[m] = -array[m];
+  if (lmax == m) return;
+  float f = sqrtf((float)(2 * m + 3));
+  array[m + 1] = x * f * array[m];
+  for (int n = m + 2; n <= lmax; n++) {
+    array[n] = x * array[n - 1] - array[n - 2] / f;
+    f = sqrtf((float)(4 * n * n - 1) / (float)(n * n - m * m));
+    array[n] *= f;
+  }
+}
+
+/* SH.h:418-434 (PrecomputedAL::init, 512 directions as sh2peaks constructs it): table
+ * [ORC_PEAKS_FAST_NDIR][(lmax/2+1)^2], row n the values at elevation n * pi / 511.  Returns the
+ * row length. */
+int orc_peaks_fast_table(int lmax, float* table) {
+  const int nAL = (lmax / 2 + 1) * (lmax / 2 + 1);
+  const float inc = (float)(M_PI / (ORC_PEAKS_FAST_NDIR - 1));
+  float buf[ORC_PEAKS_MAX_LMAX + 1];
+  for (int n = 0; n < ORC_PEAKS_FAST_NDIR; n++) {
+    float* p = table + (size_t)n * nAL;
+    const float cos_el = cosf((float)n * inc);
+    for (int m = 0; m <= lmax; m++) {
+      plm_sph_x_f(buf, lmax, m, cos_el);
+      for (int l = (m & 1) ? m + 1 : m; l <= lmax; l += 2) p[sh_index_mpos(l, m)] = buf[l];
+    }
+  }
+  return nAL;
+}
+
+/* SH.h:436-470 (set / get): linear interpolation between the two rows around the elevation,
+ * the first or last row alone outside [0, pi) */
+static void fast_al(const float* table, int lmax, float elevation, float* AL) {
+  const int nAL = (lmax / 2 + 1) * (lmax / 2 + 1);
+  const float inc = (float)(M_PI / (ORC_PEAKS_FAST_NDIR - 1));
+  float f2 = elevation / inc, f1;
+  int i = (int)f2;
+  if (i < 0) {
+    i = 0; f1 = 1.0f; f2 = 0.0f;
+  } else if (i >= ORC_PEAKS_FAST_NDIR - 1) {
+    i = ORC_PEAKS_FAST_NDIR - 1; f1 = 1.0f; f2 = 0.0f;
+  } else {
+    f2 -= (float)i;
+    f1 = (float)(1.0 - (double)f2);
+  }
+  const float* p1 = table + (size_t)i * nAL;
+  const float* p2 = p1 + nAL;
+  for (int l = 0; l <= lmax; l += 2)
+    for (int m = 0; m <= l; m++) {
+      const int k = sh_index_mpos(l, m);
+      float v = f1 * p1[k];
+      if (f2 != 0.0f) v += f2 * p2[k];
+      AL[k] = v;
+    }
+}
+
 /* SH.h:568-647: amplitude and first/second derivatives in elevation/azimuth.
  * out = {amplitude, dSH_del, dSH_daz, d2SH_del2, d2SH_deldaz, d2SH_daz2} */
-void orc_sh_derivatives_f(const float* sh, int lmax, float elevation, float azimuth, float* out) {
+static void derivatives_impl(const float* sh, int lmax, float elevation, float azimuth, const float* table,
+                             float* out) {
   const float sel = sinf(elevation);
   const float cel = cosf(elevation);
   const int atpole = (double)sel < 1e-4;
@@ -60,9 +127,13 @@ void orc_sh_derivatives_f(const float* sh, int lmax, float elevation, float azim
     for (int i = 0; i < 6; i++) out[i] = NAN;
     return;
   }
-  for (int m = 0; m <= lmax; m++) {
-    orc_plm_sph_f(buf, lmax, m, cel, sel);
-    for (int l = (m & 1) ? m + 1 : m; l <= lmax; l += 2) AL[sh_index_mpos(l, m)] = buf[l];
+  if (table) {
+    fast_al(table, lmax, elevation, AL);
+  } else {
+    for (int m = 0; m <= lmax; m++) {
+      orc_plm_sph_f(buf, lmax, m, cel, sel);
+      for (int l = (m & 1) ? m + 1 : m; l <= lmax; l += 2) AL[sh_index_mpos(l, m)] = buf[l];
+    }
   }
   float amplitude = sh[0] * AL[0];
   for (int l = 2; l <= lmax; l += 2) {
@@ -119,6 +190,10 @@ void orc_sh_derivatives_f(const float* sh, int lmax, float elevation, float azim
   out[5] = d2SH_daz2;
 }
 
+void orc_sh_derivatives_f(const float* sh, int lmax, float elevation, float azimuth, float* out) {
+  derivatives_impl(sh, lmax, elevation, azimuth, NULL, out);
+}
+
 /* Eigen's normalize(): divide by the norm when the squared norm is positive */
 static void normalize3(float* v) {
   const float z = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -132,12 +207,12 @@ static void normalize3(float* v) {
 
 /* SH.h:514-559: Gauss-Newton ascent from dir (unit, updated in place); returns the amplitude, or
  * NaN (and a NaN direction) after 50 iterations without convergence.  *iters gets the count. */
-float orc_sh_get_peak_f(const float* sh, int lmax, float* dir, int* iters) {
+static float get_peak_impl(const float* sh, int lmax, float* dir, int* iters, const float* table) {
   for (int i = 0; i < 50; i++) {
     const float az = atan2f(dir[1], dir[0]);
     const float el = atan2f(hypotf(dir[0], dir[1]), dir[2]);
     float d[6];
-    orc_sh_derivatives_f(sh, lmax, el, az, d);
+    derivatives_impl(sh, lmax, el, az, table, d);
     const float amplitude = d[0], dSH_del = d[1], dSH_daz = d[2], d2SH_del2 = d[3], d2SH_deldaz = d[4],
                 d2SH_daz2 = d[5];
     float del = sqrtf(dSH_del * dSH_del + dSH_daz * dSH_daz);
@@ -168,6 +243,10 @@ float orc_sh_get_peak_f(const float* sh, int lmax, float* dir, int* iters) {
   return NAN;
 }
 
+float orc_sh_get_peak_f(const float* sh, int lmax, float* dir, int* iters) {
+  return get_peak_impl(sh, lmax, dir, iters, NULL);
+}
+
 static float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 /* sh2peaks.cpp:266-286 (check_input): a NaN coefficient, or no non-zero coefficient past the
@@ -185,8 +264,8 @@ static int no_peaks_input(const float* sh, int nsh) {
  * (nullable): directions to report the closest peak to (-direction, or a voxel of -peaks), each
  * normalised here; out [3 npeaks] = amplitude * direction of each reported peak, NaN past the
  * last.  Returns the number of accepted peaks. */
-int orc_sh2peaks_voxel(const float* sh, int nsh, int lmax, const float* seeds, int nseeds, int npeaks,
-                       float threshold, const float* match, float* out, int* total_iters) {
+static int voxel_impl(const float* sh, int nsh, int lmax, const float* seeds, int nseeds, int npeaks,
+                      float threshold, const float* match, float* out, int* total_iters, const float* table) {
   for (int i = 0; i < 3 * npeaks; i++) out[i] = NAN;
   if (no_peaks_input(sh, nsh)) return 0;
   if (nseeds > ORC_PEAKS_MAX_SEEDS) nseeds = ORC_PEAKS_MAX_SEEDS;
@@ -196,7 +275,7 @@ int orc_sh2peaks_voxel(const float* sh, int nsh, int lmax, const float* seeds, i
     const float phi = seeds[2 * s], theta = seeds[2 * s + 1];
     float v[3] = {cosf(phi) * sinf(theta), sinf(phi) * sinf(theta), cosf(theta)};   /* sh2peaks.cpp:96 */
     int it = 0;
-    float a = orc_sh_get_peak_f(sh, lmax, v, &it);
+    float a = get_peak_impl(sh, lmax, v, &it, table);
     if (total_iters) *total_iters += it;
     if (isfinite(a)) {
       for (int j = 0; j < np; j++)
@@ -251,9 +330,15 @@ int orc_sh2peaks_voxel(const float* sh, int nsh, int lmax, const float* seeds, i
   return np;
 }
 
+int orc_sh2peaks_voxel(const float* sh, int nsh, int lmax, const float* seeds, int nseeds, int npeaks,
+                       float threshold, const float* match, float* out, int* total_iters) {
+  return voxel_impl(sh, nsh, lmax, seeds, nseeds, npeaks, threshold, match, out, total_iters, NULL);
+}
+
 typedef struct peaks_job {
   const float* sh;
   int ld, nsh, lmax, nseeds, npeaks, ld_match;
+  const float* table;   /* -fast lookup table or NULL */
   const float* seeds;
   float threshold;
   const float* match;
@@ -287,8 +372,8 @@ static void* peaks_worker(void* arg) {
         continue;
       }
       int it = 0;
-      orc_sh2peaks_voxel(J->sh + v * (size_t)J->ld, J->nsh, J->lmax, J->seeds, J->nseeds, J->npeaks, J->threshold,
-                         m, o, &it);
+      voxel_impl(J->sh + v * (size_t)J->ld, J->nsh, J->lmax, J->seeds, J->nseeds, J->npeaks, J->threshold, m, o, &it,
+                 J->table);
       iters += it;
     }
   }
@@ -301,10 +386,11 @@ static void* peaks_worker(void* arg) {
 /* The whole volume on nthreads workers (0: the calling thread).  sh [nvox][ld] (the first nsh
  * values of a row are the coefficients); match [nvox][ld_match] per voxel (ld_match 0: the same
  * directions for every voxel) or NULL; mask [nvox] or NULL.  Returns the Newton iterations run. */
-long long orc_sh2peaks_run(const float* sh, int ld, int nsh, size_t nvox, int lmax, const float* seeds, int nseeds,
-                           int npeaks, float threshold, const float* match, int ld_match, const uint8_t* mask,
-                           int nthreads, float* out) {
+long long orc_sh2peaks_run_fast(const float* sh, int ld, int nsh, size_t nvox, int lmax, const float* seeds, int nseeds,
+                                int npeaks, float threshold, const float* match, int ld_match, const uint8_t* mask,
+                                int nthreads, const float* table, float* out) {
   peaks_job J;
+  J.table = table;
   J.sh = sh; J.ld = ld; J.nsh = nsh; J.lmax = lmax; J.seeds = seeds; J.nseeds = nseeds; J.npeaks = npeaks;
   J.threshold = threshold; J.match = match; J.ld_match = ld_match; J.mask = mask; J.out = out;
   J.nvox = nvox; J.next = 0; J.iters = 0;
@@ -319,4 +405,11 @@ long long orc_sh2peaks_run(const float* sh, int ld, int nsh, size_t nvox, int lm
   }
   pthread_mutex_destroy(&J.mtx);
   return J.iters;
+}
+
+long long orc_sh2peaks_run(const float* sh, int ld, int nsh, size_t nvox, int lmax, const float* seeds, int nseeds,
+                           int npeaks, float threshold, const float* match, int ld_match, const uint8_t* mask,
+                           int nthreads, float* out) {
+  return orc_sh2peaks_run_fast(sh, ld, nsh, nvox, lmax, seeds, nseeds, npeaks, threshold, match, ld_match, mask,
+                               nthreads, NULL, out);
 }

@@ -204,3 +204,19 @@ def test_two_passes_many_seeds(torch_cuda, fods):
     frac, worst = compare(ref_tile[:256], r, 3, fods[:256])
     say("[gpu peaks] 512 seeds: tiles identical; oracle agree %.5f worst angle %.2e" % (frac, worst))
     assert frac >= 0.996
+
+
+def test_fast_table_matches_oracle(torch_cuda, fods):
+    """-fast (Legendre values from the 512-elevation table, SH.h:402-470) against the oracle's run
+    with the same table, at the exact path's tolerance."""
+    torch = torch_cuda
+    finder = peaks.SH2Peaks(45, fast=True)
+    say("[gpu peaks] -fast: %d voxels, lmax 8" % len(fods))
+    out = finder.run(torch.from_numpy(fods).cuda())
+    torch.cuda.synchronize()
+    ref, _ = orc.sh2peaks(fods, nthreads=8, fast=True)
+    exact, _ = orc.sh2peaks(fods[:2000], nthreads=8)
+    frac, worst = compare(out.cpu().numpy(), ref, 3, fods)
+    say("[gpu peaks] -fast agree %.5f worst angle %.2e rad" % (frac, worst))
+    assert frac >= 0.999
+    assert not np.array_equal(ref[:2000], exact)    # the table path, not the exact one

@@ -165,3 +165,51 @@ def test_threaded_run_equals_single_voxels():
     a, ita = orc.sh2peaks(vols, nthreads=4)
     b, itb = orc.sh2peaks(vols, nthreads=0)
     assert np.array_equal(a, b, equal_nan=True) and ita == itb
+
+
+def test_fast_table_against_legendre_closed_form():
+    # -fast table rows (SH.h:418-434): the SH-normalised associated Legendre functions with the
+    # Condon-Shortley phase, sqrt((2l+1)/4pi (l-m)!/(l+m)!) P_l^m(cos el), at el = n pi / 511
+    from math import factorial
+    from scipy.special import lpmv
+    for lmax in (2, 8, 12):
+        tab = orc.peaks_fast_table(lmax)
+        assert tab.shape == (512, (lmax // 2 + 1) ** 2)
+        x = np.cos(np.arange(512) * np.pi / 511)
+        for l in range(0, lmax + 1, 2):
+            for m in range(0, l + 1):
+                want = np.sqrt((2 * l + 1) / (4 * np.pi) * factorial(l - m) / factorial(l + m)) * lpmv(m, l, x)
+                got = tab[:, l * l // 4 + m].astype(np.float64)
+                # 1e-4: near the poles 1 - x^2 is formed in single precision, as the reference's
+                # one-argument Plm_sph does (up to 2.1e-5 at lmax 12); a wrong sign, index or
+                # normalisation is off by orders more
+                assert np.max(np.abs(got - want)) <= 1e-4 * max(1.0, np.max(np.abs(want))), (lmax, l, m)
+        # m > 0 rows vanish at the poles (x^2 >= 1 in single precision)
+        assert np.all(tab[0, [l * l // 4 + m for l in range(2, lmax + 1, 2) for m in range(1, l + 1)]] == 0.0)
+
+
+def test_fast_run_close_to_exact_run():
+    # the table's linear interpolation moves the peaks by far less than the search tolerance of the
+    # de-duplication (acos 0.99): the same peaks, directions within 5e-3 rad, amplitudes within 1e-3
+    # lobes at least 45 degrees apart: closer ones merge, and the second reported peak is then one
+    # of a ring of equal sidelobe maxima, whichever a search happens to reach
+    rng = np.random.default_rng(5)
+    dirs = rng.standard_normal((400, 2, 3))
+    dirs /= np.linalg.norm(dirs, axis=2, keepdims=True)
+    dirs = dirs[np.abs(np.sum(dirs[:, 0] * dirs[:, 1], axis=1)) < np.cos(np.pi / 4)][:200]
+    sh = np.stack([delta_fod(d, [1.0, 0.6]) for d in dirs]).astype(np.float32)
+    exact, _ = orc.sh2peaks(sh, npeaks=2, nthreads=4)
+    fast, _ = orc.sh2peaks(sh, npeaks=2, nthreads=4, fast=True)
+    assert not np.array_equal(exact, fast)    # the table is in use
+    ok = 0
+    for v in range(len(sh)):
+        e, f = exact[v].reshape(2, 3), fast[v].reshape(2, 3)
+        if not np.array_equal(np.isfinite(e[:, 0]), np.isfinite(f[:, 0])):
+            continue
+        good = True
+        for i in np.nonzero(np.isfinite(e[:, 0]))[0]:
+            ae, af = np.linalg.norm(e[i]), np.linalg.norm(f[i])
+            c = abs(e[i] @ f[i]) / (ae * af)
+            good &= np.arccos(min(1.0, c)) < 5e-3 and abs(ae - af) < 1e-3 * ae
+        ok += good
+    assert len(sh) == 200 and ok >= 0.995 * len(sh)
