@@ -144,14 +144,14 @@ def cpu_baseline_csd(ph, lmax, gpu_fod, sample):
                 single_thread_kernels=single, host=host_info()), rel, ref
 
 
-def cpu_baseline_icls(ph, shared, gpu_x, sample):
+def cpu_baseline_icls(ph, gpu_x, sample):
     from oracle import orc
     sample = min(sample, ph.nvox)
     sub = np.ascontiguousarray(ph.dw[:sample])
     nt = threads()
     res = {}
     for name, generic in (("native", False), ("generic", True)):
-        o = orc.IclsOracle(shared.matrix("C"), shared.matrix("A"), None, 0, 1e-10, 1e-10, generic=generic)
+        o = orc.MsmtOracle(ph.grad, ph.response, generic=generic)
         med, runs, (x, _) = median_rate(lambda: o.run_batch(sub, nthreads=nt), sample)
         res[name] = (med, runs, x)
     med, runs, x = res["native"]
@@ -507,7 +507,7 @@ def bench_msmt(args, torch, ph):
     if not args.no_cpu:
         sample = args.cpu_sample or CPU_SAMPLE["multi_shell"]
         print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
-        base, rel = cpu_baseline_icls(ph, shared, x.cpu().numpy(), sample)
+        base, rel = cpu_baseline_icls(ph, x.cpu().numpy(), sample)
         out["cpu_baseline"] = base
         out["parity_on_cpu_sample"] = {"max_rel_err": float(rel.max()),
                                        "p999_rel_err": float(np.quantile(rel, 0.999)),

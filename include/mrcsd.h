@@ -104,12 +104,21 @@ void mrcsd_destroy(mrcsd_ctx* ctx);
  * proposed negative set replaces the first amplitude evaluation.  A voxel whose negative set
  * reaches a fixed point there is accepted (for threshold 0 that is the unique minimiser, i.e. the
  * plain solver's answer); any other voxel is solved again from rconv*b with the full niter budget,
- * which reproduces the reference trajectory.  Ignored when niter is 0. */
+ * which reproduces the reference trajectory.  Ignored when niter is 0.
+ * The certificate needs threshold 0 and a positive definite Mt_M (norm_lambda > 0, or M of full
+ * column rank): mrcsd_run refuses a proposal otherwise (MRCSD_ERR_ARG; mrcsd_warm_supported tells).
+ * One difference from the reference remains by design: a voxel the plain loop would leave
+ * unconverged after niter iterations (csd.h:245-269 returns that iterate) is returned converged
+ * when its proposal certifies. */
 typedef struct mrcsd_warm {
   const float* F0;         /* device [nvox][n_sh] or NULL */
   const uint64_t* neg;     /* device [nvox][(n_hr+63)/64] proposed negative sets, or NULL */
   int budget;              /* iterate() calls allowed from the proposal (>= 2 to certify) */
 } mrcsd_warm;
+
+/* 1 when mrcsd_run accepts a warm-start proposal for this context (threshold 0 and Mt_M positive
+ * definite), else 0. */
+int mrcsd_warm_supported(const mrcsd_ctx* ctx);
 
 /* Optional per-voxel outputs beyond the FOD. */
 typedef struct mrcsd_outputs {

@@ -59,6 +59,7 @@ _SIG = {
     "mrcsd_create": (C.c_int, [C.POINTER(Params), C.POINTER(_P)]),
     "mrcsd_create_from_shared": (C.c_int, [_P, C.POINTER(_P)]),
     "mrcsd_destroy": (None, [_P]),
+    "mrcsd_warm_supported": (C.c_int, [_P]),
     "mrcsd_run": (C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, C.POINTER(Outputs), C.POINTER(Warm), _P]),
     "mrcsd_run_host": (C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P]),
 }

@@ -120,6 +120,11 @@ class CSD:
         obj.n_hr = HR.shape[0]
         return obj
 
+    def warm_supported(self):
+        """Whether the solver accepts a warm-start proposal (threshold 0 and a positive definite
+        normal matrix, where the certified fixed point is the plain solver's answer)."""
+        return bool(_lib.load().mrcsd_warm_supported(self._h))
+
     def run(self, dw, valid=None, iters=True, nneg=True, stats=None, warm_F0=None, fod=None,
             negmask=False, warm_neg=None, warm_budget=6):
         """Solve every row of dw ([nvox, nvol] float32).  Returns (fod, iters, nneg), plus the

@@ -31,7 +31,36 @@ struct mrcsd_ctx {
   mrcsd::DeviceBuffer<int> dwis;
   mrcsd::DeviceBuffer<unsigned long long> counter;
   int max_vol = 0;   // largest selected volume index (checked against ld_dw)
+  // The warm-start certificate (neg(F(S)) == S implies the plain answer) holds when the fixed point
+  // is the unique minimiser of the CSD objective: threshold 0 and a positive definite Mt_M.
+  bool warm_certifiable = false;
 };
+
+namespace {
+
+// Cholesky of the symmetric n x n matrix A; true when every pivot exceeds `rel` times the largest
+// diagonal entry (positive definite with margin above rounding).
+bool positive_definite(const double* A, int n, double rel) {
+  std::vector<double> L(size_t(n) * n, 0.0);
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i) dmax = std::max(dmax, A[size_t(i) * n + i]);
+  if (!(dmax > 0.0)) return false;
+  for (int j = 0; j < n; ++j) {
+    double d = A[size_t(j) * n + j];
+    for (int k = 0; k < j; ++k) d -= L[size_t(j) * n + k] * L[size_t(j) * n + k];
+    if (!(d > rel * dmax)) return false;
+    const double ljj = std::sqrt(d);
+    L[size_t(j) * n + j] = ljj;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[size_t(i) * n + j];
+      for (int k = 0; k < j; ++k) s -= L[size_t(i) * n + k] * L[size_t(j) * n + k];
+      L[size_t(i) * n + j] = s / ljj;
+    }
+  }
+  return true;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -156,6 +185,7 @@ int mrcsd_create(const mrcsd_params* p, mrcsd_ctx** out) {
     P.M = ctx->M.get(); P.rconvT = ctx->rconvT.get(); P.dwis = ctx->dwis.get();
     P.thr = p->threshold;
     P.n_dw = N; P.n_sh = n; P.n_init = ni; P.n_hr = R; P.n_hr_pad = Rpad; P.niter = p->niter;
+    ctx->warm_certifiable = p->threshold == 0.0 && positive_definite(p->MtM, n, 1e-12);
     *out = ctx.release();
     return MRCSD_OK;
   });
@@ -182,6 +212,8 @@ int mrcsd_create_from_shared(const mrcsd_shared* s, mrcsd_ctx** out) {
 
 void mrcsd_destroy(mrcsd_ctx* ctx) { delete ctx; }
 
+int mrcsd_warm_supported(const mrcsd_ctx* ctx) { return ctx && ctx->warm_certifiable ? 1 : 0; }
+
 int mrcsd_run(mrcsd_ctx* ctx, const float* dw, int ld_dw, const uint8_t* valid, size_t nvox,
               float* fod, const mrcsd_outputs* out, const mrcsd_warm* warm, void* stream) {
   return mrcsd::guarded([&]() -> int {
@@ -207,6 +239,10 @@ int mrcsd_run(mrcsd_ctx* ctx, const float* dw, int ld_dw, const uint8_t* valid, 
     L.warm_budget = warm ? warm->budget : 0;
     if (warm && (warm->F0 || warm->neg) && (warm->budget < 1 || warm->budget > 65534))
       return mrcsd::fail(MRCSD_ERR_ARG, "mrcsd_run: warm-start budget must be in [1, 65534]");
+    if (warm && (warm->F0 || warm->neg) && !ctx->warm_certifiable)
+      return mrcsd::fail(MRCSD_ERR_ARG,
+                         "mrcsd_run: a warm start needs threshold 0 and a positive definite normal matrix "
+                         "(norm_lambda > 0, or M of full column rank); solve without a proposal");
     L.counter = ctx->counter.get();
     L.max_groups = 0;
     MRCSD_HIP(mrcsd::csd_launch(ctx->P, L, static_cast<hipStream_t>(stream)));

@@ -196,7 +196,10 @@ def train_warm_start(net, solver, dw_train, dwis, b0, epochs=12, batch=4096, lr=
 def warm_solve(net, solver, dw, dwis, b0, valid=None, budget=6, stats=None, fod=None, negmask=False, fused=True):
     """Network proposal -> warm-started CSD (fallback to the plain start).  The solver starts from the
     proposed negative set, so the proposed coefficients are not needed; the fused kernel computes the
-    set directly from the DW rows."""
+    set directly from the DW rows.  When the solver's options void the certificate (threshold not 0,
+    or a singular normal matrix), the proposal is not made and every voxel takes the plain start."""
+    if not solver.warm_supported():
+        return solver.run(dw, valid=valid, stats=stats, fod=fod, negmask=negmask)
     if fused and net.fused_ok():
         _, neg = net.infer(dw, dwis, b0)
         F0 = None

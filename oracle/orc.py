@@ -384,3 +384,80 @@ def sh2peaks(sh, lmax=None, seeds=None, npeaks=3, threshold=-np.inf, directions=
                                 mk.ctypes.data if mk is not None else None, int(nthreads),
                                 tab.ctypes.data if tab is not None else None, out.ctypes.data)
     return out, int(it)
+
+
+# ------------------------------------------------------------------------------------------
+# MSMT_CSD::Shared (src/dwi/sdeconv/msmt_csd.h:100-218): C and A built by the oracle alone
+# ------------------------------------------------------------------------------------------
+def msmt_shells(grad, shells=None, bzero=10.0, eps=80.0):
+    """Shell index of each volume (-1: in no retained shell), shells sorted by mean b-value
+    (shells.cpp:300-342).  shells: the -shells b-values; each picks the b=0 shell (b <= bzero),
+    else the shell whose [min, max] holds it, else the one within 1 of its mean
+    (shells.cpp:115-180); None keeps every shell (msmt_csd.h:54)."""
+    g = _arr(grad)
+    b = g[:, 3]
+    _, cl = shells_cluster(b, bzero, eps)
+    ids = [c for c in np.unique(cl) if c > 0]
+    ids.sort(key=lambda c: b[cl == c].mean())
+    if shells is not None:
+        keep = []
+        for want in shells:
+            if want < 0:
+                raise ValueError("Cannot select shells corresponding to negative b-values")
+            hit = None
+            for c in ids:
+                bc = b[cl == c]
+                is_zero = bc.mean() <= bzero
+                if (want <= bzero and is_zero) or (want > bzero and bc.min() <= want <= bc.max()):
+                    hit = c
+                    break
+            if hit is None:
+                near = [c for c in ids if abs(b[cl == c].mean() - want) <= 1.0]
+                if len(near) != 1:
+                    raise ValueError("no unambiguous shell for b = %g" % want)
+                hit = near[0]
+            if hit in keep:
+                raise ValueError("shell selected more than once")
+            keep.append(hit)
+        ids = [c for c in ids if c in keep]
+    shell_of = np.array([ids.index(c) if c in ids else -1 for c in cl], dtype=np.int32)
+    return shell_of, len(ids)
+
+
+def msmt_matrices(grad, responses, lmax=None, shells=None, directions=None):
+    """(C [nvol, nparams], A [ncon, nparams], lmax list) of MSMT_CSD::Shared::init from the raw
+    gradient table and per-tissue response matrices [nshell, ncoef]."""
+    g, _ = normalise_grad(grad)
+    shell_of, nshell = msmt_shells(g, shells)
+    resp = [np.atleast_2d(_arr(r)) for r in responses]
+    if lmax is None:
+        lmax = []
+        for r in resp:   # trailing all-zero columns dropped (msmt_csd.h:237-255), capped at 8
+            nz = [j + 1 for j in range(r.shape[1]) if np.any(r[:, j] != 0)]
+            lmax.append(min(8, 2 * (max(nz, default=1) - 1)))
+    lmax = [int(l) for l in lmax]
+    for r in resp:
+        if r.shape[0] != nshell:
+            raise ValueError("number of rows in response functions must match number of b-value shells")
+    cat = _arr(np.concatenate([r.ravel() for r in resp]))
+    cols = np.array([r.shape[1] for r in resp], dtype=np.int32)
+    lm = np.array(lmax, dtype=np.int32)
+    hr = _arr(directions) if directions is not None else load_dirs300()
+    nparams = int(sum((l + 1) * (l + 2) // 2 for l in lmax))
+    ncon = int(sum(hr.shape[0] if l > 0 else 1 for l in lmax))
+    Cm = np.zeros((g.shape[0], nparams))
+    A = np.zeros((ncon, nparams))
+    nc, nr = C.c_int(), C.c_int()
+    lib().orc_msmt_build(_d(g), g.shape[0], shell_of.ctypes.data_as(I), nshell, _d(cat),
+                         cols.ctypes.data_as(I), len(resp), lm.ctypes.data_as(I), _d(hr), hr.shape[0],
+                         _d(Cm), _d(A), C.byref(nc), C.byref(nr))
+    return Cm, A, lmax
+
+
+class MsmtOracle(IclsOracle):
+    """MSMT_CSD::Shared::init + ICLS solves, every matrix built by the oracle."""
+
+    def __init__(self, grad, responses, lmax=None, shells=None, directions=None, norm_lambda=1e-10,
+                 neg_lambda=1e-10, **kw):
+        self.C, self.A, self.lmax = msmt_matrices(grad, responses, lmax, shells, directions)
+        super().__init__(self.C, self.A, None, 0, norm_lambda, neg_lambda, **kw)

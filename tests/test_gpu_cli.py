@@ -86,7 +86,7 @@ def test_cli_msmt_matches_oracle(tmp_path):
     got = [mif.read(p)[0] for p in outs]
     assert got[0].shape == shape + (45,) and got[1].shape[:3] == shape and got[2].shape[:3] == shape
     s = icls.MSMTShared(ph.grad, ph.response)
-    o = orc.IclsOracle(s.matrix("C"), s.matrix("A"), None, 0, 1e-10, 1e-10)
+    o = orc.MsmtOracle(ph.grad, ph.response)
     rel = []
     for idx in np.ndindex(shape):
         x, _, _ = o.solve(img[idx].astype(np.float64))
@@ -283,7 +283,7 @@ def test_cli_shells_option(tmp_path):
     got = [mif.read(p)[0] for p in outs]
     s = icls.MSMTShared(ph.grad, two, shells=[0, 1000, 3000])
     assert s.nshell == 3
-    o2 = orc.IclsOracle(s.matrix("C"), s.matrix("A"), None, 0, 1e-10, 1e-10)
+    o2 = orc.MsmtOracle(ph.grad, two, shells=[0, 1000, 3000])
     rel = []
     for idx in np.ndindex(shape):
         x, _, _ = o2.solve(img[idx].astype(np.float64))

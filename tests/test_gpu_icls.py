@@ -98,7 +98,7 @@ def test_msmt_small_volume(torch_cuda):
     x = x.cpu().numpy().astype(np.float64)
     it = it.cpu().numpy().astype(np.int64)
     say("[gpu msmt] stats", stats.cpu().numpy().tolist())
-    o = orc.IclsOracle(s.matrix("C"), s.matrix("A"), None, 0, 1e-10, 1e-10)
+    o = orc.MsmtOracle(ph.grad, ph.response)
     rel, same_it = [], []
     for v in range(0, ph.nvox, 3):
         xo, ito, _ = o.solve(ph.dw[v].astype(np.float64))

@@ -57,14 +57,16 @@ def test_gradients_match_reference(torch_cuda):
     (lc, ll), gref = mlp_ref.loss_and_grads(net.get_params(), dims(net), x.float().cpu().numpy(),
                                             tc.cpu().numpy().astype(np.float64), tn.cpu().numpy(), 1.0, 0.1)
     say("[gpu warm] loss gpu %.6g %.6g ref %.6g %.6g" % (loss[0], loss[1], lc, ll))
-    assert abs(loss[0] - lc) <= 1e-2 * lc and abs(loss[1] - ll) <= 1e-2 * ll
+    # bf16 operands emulated in the reference, fp32 accumulation on the GPU: measured per-block
+    # relative errors are 3e-5 to 9e-5, the loss agrees to 2e-6
+    assert abs(loss[0] - lc) <= 1e-4 * lc and abs(loss[1] - ll) <= 1e-4 * ll
     o = 0
     for fin, fout in zip(dims(net)[:-1], dims(net)[1:]):
         for size in (fin * fout, fout):
             a, b = g[o:o + size], gref[o:o + size]
             rel = np.linalg.norm(a - b) / np.linalg.norm(b)
             say("[gpu warm] grad block %d x %d size %d rel err %.3e" % (fin, fout, size, rel))
-            assert rel < 3e-2
+            assert rel < 3e-4
             o += size
 
 
@@ -156,3 +158,31 @@ def test_warm_solve_matches_plain_on_held_out_voxels(torch_cuda, trained):
         % (rel.max(), same.mean(), sp[1] / sp[0], sw[1] / sw[0], sw[4] / sw[0]))
     assert np.mean((rel <= 1e-4) & same) >= 0.999
     assert sw[1] < sp[1]
+
+
+def test_warm_start_refused_where_the_certificate_does_not_hold(torch_cuda, trained):
+    """neg(F(S)) == S certifies the plain answer only for threshold 0 and a positive definite normal
+    matrix.  With -threshold 0.1, or -norm_lambda 0 on an under-determined lmax-12 fit, the C ABI
+    refuses a proposal, and warm_solve takes the plain start for every voxel instead."""
+    torch = torch_cuda
+    from mrtrix3_amd._lib import MrcsdError
+    ph = phantom.single_shell_small(seed=99)
+    dw = torch.from_numpy(np.ascontiguousarray(ph.dw[:3000])).cuda()
+    assert trained["solver"].warm_supported()
+    # norm_lambda 0 with 60 directions at lmax 8: M has full column rank, the certificate holds
+    assert csd.CSD(csd.CSDShared(ph.grad, ph.response, norm_lambda=0.0)).warm_supported()
+    # under-determined lmax 12 (91 coefficients from 60 directions, no norm regulariser): refused
+    sing = csd.CSD(csd.CSDShared(ph.grad, ph.response, lmax=12, norm_lambda=0.0))
+    assert not sing.warm_supported()
+    thr = csd.CSD(csd.CSDShared(ph.grad, ph.response, threshold=0.1))
+    assert not thr.warm_supported()
+    neg = torch.zeros((dw.shape[0], (thr.n_hr + 63) // 64), dtype=torch.int64, device="cuda")
+    say("[gpu warm] certificate guard: threshold 0.1 and singular lmax 12")
+    for s in (thr, sing):
+        with pytest.raises(MrcsdError, match="warm start needs threshold 0"):
+            s.run(dw, warm_neg=torch.zeros((dw.shape[0], (s.n_hr + 63) // 64), dtype=torch.int64,
+                                           device="cuda"))
+    f_plain, it_plain, _ = thr.run(dw)
+    f_warm, it_warm, _ = warm.warm_solve(trained["net"], thr, dw, trained["dwis"], trained["b0"])
+    torch.cuda.synchronize()
+    assert torch.equal(f_plain, f_warm) and torch.equal(it_plain, it_warm)

@@ -1,7 +1,5 @@
 """Host-side MSMT_CSD::Shared set-up (libmrcsd.so, no GPU) against the oracle's restatement of
 msmt_csd.h:100-218."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
@@ -11,25 +9,8 @@ from oracle import orc
 
 
 def oracle_msmt(grad, responses, lmax):
-    g, _ = orc.normalise_grad(grad)
-    n, cl = orc.shells_cluster(g[:, 3])
-    ids = [c for c in np.unique(cl) if c > 0]
-    means = {c: g[cl == c, 3].mean() for c in ids}
-    order = sorted(ids, key=lambda c: means[c])
-    shell_of = np.array([order.index(c) if c > 0 else -1 for c in cl], dtype=np.int32)
-    resp = np.ascontiguousarray(np.concatenate([np.asarray(r, float).ravel() for r in responses]))
-    cols = np.array([np.atleast_2d(r).shape[1] for r in responses], dtype=np.int32)
-    lm = np.array(lmax, dtype=np.int32)
-    nparams = int(sum((l + 1) * (l + 2) // 2 for l in lmax))
-    ncon = int(sum(300 if l > 0 else 1 for l in lmax))
-    Cm = np.zeros((g.shape[0], nparams))
-    A = np.zeros((ncon, nparams))
-    nc, nr = C.c_int(), C.c_int()
-    hr = orc.load_dirs300()
-    orc.lib().orc_msmt_build(orc._d(g), g.shape[0], shell_of.ctypes.data_as(orc.I), len(order), orc._d(resp),
-                             cols.ctypes.data_as(orc.I), len(responses), lm.ctypes.data_as(orc.I), orc._d(hr),
-                             300, orc._d(Cm), orc._d(A), C.byref(nc), C.byref(nr))
-    return Cm, A
+    C, A, _ = orc.msmt_matrices(grad, responses, lmax=lmax)
+    return C, A
 
 
 @pytest.fixture(scope="module")

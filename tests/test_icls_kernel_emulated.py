@@ -57,7 +57,7 @@ def test_icls_kernel_under_wavefront_emulation(tmp_path):
     assert "EMU:" not in r.stderr, r.stderr[-3000:]
 
     s = icls.MSMTShared(ph.grad, ph.response)
-    o = orc.IclsOracle(s.matrix("C"), s.matrix("A"), None, 0, 1e-10, 1e-10)
+    o = orc.MsmtOracle(ph.grad, ph.response)
     n = s.nparams
     x = np.fromfile(d / "x.bin", dtype=np.float32).reshape(nv, n)
     stats = np.loadtxt(d / "iters.txt", dtype=np.int64).reshape(nv, 2)
