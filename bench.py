@@ -291,7 +291,7 @@ def bench_csd(args, torch, workload, ph):
     def roofline(st, kern_ms, leg):
         fl = csd_flops(shared.n_sh, shared.n_init, shared.n_dw, shared.n_hr, st)
         achieved = fl["total"] / (kern_ms * 1e-3) / 1e12
-        return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        return {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": load_traffic("csd_persistent_kernel_%s" % workload, leg),
                 "kernel": tag % ("true" if leg == "warm" else "false"), "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
@@ -490,9 +490,10 @@ def bench_msmt(args, torch, ph):
                                "10/30/60/90, WM lmax 8 + GM + CSF (%d parameters, %d constraints)"
                                % (ph.nvox, prob.n, prob.ncon),
                    "form": "msmt_csd: ICLS::Solver dual active set, FP64, persistent kernel"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic("icls_persistent_kernel_multi_shell"),
-                     "kernel": "icls_persistent_kernel", "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic("icls_active_set_kernel_multi_shell"),
+                     "kernel": "icls_active_set_kernel<%d, %d>" % (prob.capacity, 4 if prob.ncon <= 256 else 8 if prob.ncon <= 512 else 16),
+                     "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
                      "note": "FP64-bound; peak is the MI355X FP64 rate (vector = matrix)"},
         "icls": {"outer_per_voxel": float(st[1]) / max(1, st[0]), "inner_per_voxel": float(st[2]) / max(1, st[0]),
                  "mean_active": float(st[3]) / max(1, st[2]), "over_capacity": int(st[7])},
@@ -653,7 +654,7 @@ def bench_peaks(args, torch, ph):
         "data": "synthetic",
         "config": {"workload": "sh2peaks on the single_shell_brain FODs (646,128 voxels, lmax 8), 60 search "
                                "directions, 3 peaks; FODs from the plain GPU CSD, resident in HBM"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": load_traffic("peaks_kernel_single_shell_brain"),
                      "kernel": "peaks_search_kernel<%d> + peaks_select_kernel" % shared.lmax,
                      "kernel_ms": kern_ms, "flop_per_launch": fl,
@@ -729,7 +730,7 @@ def bench_response(args, torch, ph):
         "data": "synthetic",
         "config": {"workload": "amp2response on the single_shell_brain phantom (646,128 voxels), b=3000 shell "
                                "(64 directions), lmax 10, fibre directions from GPU CSD + sh2peaks, resident in HBM"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "resp_accumulate_kernel<10> + resp_sum_kernel (+ the 36-value copy back)",
                      "kernel_ms": kern_ms, "flop_per_launch": fl,

@@ -26,8 +26,8 @@ typedef struct mricls_ctx mricls_ctx;
 /* Problem(H, A, t, num_eq, sol_reg, con_reg, max_iter, tol, standard_form)
  * (constrained_least_squares.h:91-152).  H [nmeas][n] (or [n][n] with H^T H in the lower
  * triangle when standard_form), A [ncon][n] with the num_eq equality rows last, t [ncon] or NULL
- * (zeros).  Limits: n <= 64, nmeas <= 512, ncon <= 512, at most 64 simultaneously active
- * constraints (a voxel exceeding it reports iters 0xFFFE). */
+ * (zeros).  Limits: n <= 128, ncon <= 1024, at most 128 simultaneously active constraints (the
+ * reference has none; a voxel exceeding 128 reports iters 0xFFFE and mricls_solve_host fails). */
 int mricls_create(const double* H, int nmeas, int n, const double* A, int ncon, const double* t,
                   int num_eq, double sol_reg, double con_reg, int max_iter, double tol,
                   int standard_form, mricls_ctx** out);
@@ -35,19 +35,29 @@ void mricls_destroy(mricls_ctx* ctx);
 /* Dimensions and the effective max_niter (10 n when 0 was given, :105). */
 int mricls_dims(const mricls_ctx* ctx, int* nmeas, int* n, int* ncon, int* max_niter);
 
+/* Active-set slots of the first pass of mricls_solve (no counterpart in the reference, whose
+ * active set grows without bound): a voxel whose active set outgrows them is solved again, from
+ * the start and with the same arithmetic, with 64 and then 128 slots.  slots: 16, 32, 48, 64, 96
+ * or 128; 0 restores the default, the smallest above n + num_eq.  Fewer slots mean less on-chip
+ * memory per voxel and more voxels in flight. */
+int mricls_set_capacity(mricls_ctx* ctx, int slots);
+int mricls_get_capacity(const mricls_ctx* ctx);
+
 /* Solver::operator()(x, b) (:237-357) for nvox right-hand sides.  Device pointers:
  *   b       [nvox][ld_b] float, the first nmeas entries of each row
  *   valid   [nvox] uint8 or NULL: 0 -> x = 0 and iters = 0xFFFF
  *   x       [nvox][n] float
- *   iters   [nvox] uint16 or NULL: the returned niter (0xFFFE: active set over capacity)
+ *   iters   [nvox] uint16 or NULL: the returned niter (0xFFFE: more than 128 active constraints)
  *   nactive [nvox] uint16 or NULL: size of the final active set
  *   stats   uint64[8] or NULL, accumulated: {voxels, outer iterations, inner solves,
  *           sum k, sum k^2, sum k^3 over inner solves, sum k over constraint updates,
- *           voxels over capacity}
- * Asynchronous on `stream`. */
+ *           voxels over 128 active constraints}
+ * Asynchronous on `stream`.  The first call with more voxels than any before grows the context's
+ * voxel queues (a device allocation, after synchronising `stream`). */
 int mricls_solve(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                  float* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream);
-/* Host pointers (copies in, solves, copies out; synchronous). */
+/* Host pointers (copies in, solves, copies out; synchronous).  Fails with MRCSD_ERR_SETUP when a
+ * voxel needs more than 128 active constraints (its outputs are still written). */
 int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                       float* x, uint16_t* iters, uint16_t* nactive);
 

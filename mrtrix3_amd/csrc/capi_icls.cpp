@@ -22,7 +22,12 @@ struct mricls_ctx {
   int device = 0;
   mrcsd::IclsDeviceProblem P{};
   mrcsd::DeviceBuffer<double> b2d, Bt, t, G, L;
-  mrcsd::DeviceBuffer<unsigned long long> counter;
+  // scratch of a solve: work counters and queue lengths of the passes, and the two queues of
+  // voxels re-solved with more active-set slots (grown to the largest voxel count seen)
+  mrcsd::DeviceBuffer<unsigned long long> counters;
+  mrcsd::DeviceBuffer<uint32_t> queue;
+  size_t queue_cap = 0;
+  int capacity = 0;   // active-set slots of the first pass
 };
 
 struct mrmsmt_shared {
@@ -35,13 +40,13 @@ int make_ctx(const Problem& pr, mricls_ctx** out) {
   const int n = int(pr.num_parameters()), nm = pr.b2d.rows, ncon = pr.B.rows;
   const int num_eq = int(pr.num_eq);
   if (n < 1 || n > mrcsd::kIclsMaxParams)
-    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: number of parameters must be in [1, 64]");
-  if (nm < 1 || nm > mrcsd::kIclsMaxMeas)
-    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: number of measurements must be in [1, 512]");
+    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: number of parameters must be in [1, 128]");
+  if (nm < 1)
+    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: no measurements");
   if (ncon < 1 || ncon > mrcsd::kIclsMaxCon)
-    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: number of constraints must be in [1, 512]");
+    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: number of constraints must be in [1, 1024]");
   if (num_eq > mrcsd::kIclsMaxActive)
-    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: at most 64 equality constraints");
+    return mrcsd::fail(MRCSD_ERR_ARG, "mricls: at most 128 equality constraints");
   if (pr.max_niter > 65533) return mrcsd::fail(MRCSD_ERR_ARG, "mricls: max_iter too large");
   int rc = mrcsd::require_gpu();
   if (rc) return rc;
@@ -68,7 +73,8 @@ int make_ctx(const Problem& pr, mricls_ctx** out) {
   ctx->t.upload(t.data(), t.size());
   ctx->G.upload(G.data(), G.size());
   ctx->L.upload(Lf.data(), Lf.size());
-  ctx->counter.alloc(1);
+  ctx->counters.alloc(2 * mrcsd::kIclsMaxPasses);
+  ctx->capacity = mrcsd::icls_default_capacity(n, num_eq);
   mrcsd::IclsDeviceProblem& P = ctx->P;
   P.b2d = ctx->b2d.get(); P.Bt = ctx->Bt.get(); P.t = ctx->t.get(); P.G = ctx->G.get();
   P.L = ctx->L.get();
@@ -115,6 +121,22 @@ int mricls_dims(const mricls_ctx* ctx, int* nmeas, int* n, int* ncon, int* max_n
   return MRCSD_OK;
 }
 
+int mricls_set_capacity(mricls_ctx* ctx, int slots) {
+  if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_set_capacity: null context");
+  if (slots == 0) {
+    ctx->capacity = mrcsd::icls_default_capacity(ctx->P.n, ctx->P.num_eq);
+    return MRCSD_OK;
+  }
+  if (!mrcsd::icls_valid_capacity(slots))
+    return mrcsd::fail(MRCSD_ERR_ARG, "mricls_set_capacity: slots must be 0, 16, 32, 48, 64, 96 or 128");
+  if (slots < ctx->P.num_eq)
+    return mrcsd::fail(MRCSD_ERR_ARG, "mricls_set_capacity: fewer slots than equality constraints");
+  ctx->capacity = slots;
+  return MRCSD_OK;
+}
+
+int mricls_get_capacity(const mricls_ctx* ctx) { return ctx ? ctx->capacity : 0; }
+
 int mricls_solve(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                  float* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream) {
   return mrcsd::guarded([&]() -> int {
@@ -123,11 +145,22 @@ int mricls_solve(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid
     if (!b || !x) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: null b or x");
     if (ld_b < ctx->P.nmeas) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: ld_b smaller than nmeas");
     MRCSD_HIP(hipSetDevice(ctx->device));
+    if (nvox > size_t(UINT32_MAX)) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: more than 2^32 - 1 voxels");
+    if (ctx->capacity < mrcsd::kIclsMaxActive && ctx->queue_cap < nvox) {
+      // queues of voxels passed on to larger capacities: grown (synchronously) to this call's size
+      MRCSD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+      ctx->queue.alloc(2 * nvox);
+      ctx->queue_cap = nvox;
+    }
     mrcsd::IclsLaunch L{};
     L.b = b; L.ld_b = ld_b; L.valid = valid; L.nvox = nvox; L.x = x; L.iters = iters;
     L.nactive = nactive; L.stats = reinterpret_cast<unsigned long long*>(stats);
-    L.counter = ctx->counter.get();
-    MRCSD_HIP(mrcsd::icls_launch(ctx->P, L, static_cast<hipStream_t>(stream)));
+    mrcsd::IclsScratch S{};
+    S.counters = ctx->counters.get();
+    S.queue[0] = ctx->queue.get();
+    S.queue[1] = ctx->queue.get() ? ctx->queue.get() + ctx->queue_cap : nullptr;
+    S.capacity = ctx->queue_cap;
+    MRCSD_HIP(mrcsd::icls_launch(ctx->P, L, S, ctx->capacity, static_cast<hipStream_t>(stream)));
     return MRCSD_OK;
   });
 }
@@ -146,15 +179,21 @@ int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* 
     d_b.upload(b, nvox * size_t(ld_b));
     d_x.alloc(nvox * n);
     if (valid) d_v.upload(valid, nvox);
-    if (iters) d_it.alloc(nvox);
+    d_it.alloc(nvox);
     if (nactive) d_na.alloc(nvox);
-    int rc = mricls_solve(ctx, d_b.get(), ld_b, valid ? d_v.get() : nullptr, nvox, d_x.get(),
-                          iters ? d_it.get() : nullptr, nactive ? d_na.get() : nullptr, nullptr, nullptr);
+    int rc = mricls_solve(ctx, d_b.get(), ld_b, valid ? d_v.get() : nullptr, nvox, d_x.get(), d_it.get(),
+                          nactive ? d_na.get() : nullptr, nullptr, nullptr);
     if (rc) return rc;
     MRCSD_HIP(hipDeviceSynchronize());
+    std::vector<uint16_t> it(nvox);
+    d_it.download(it.data(), nvox);
     d_x.download(x, nvox * n);
-    if (iters) d_it.download(iters, nvox);
+    if (iters) std::memcpy(iters, it.data(), nvox * sizeof(uint16_t));
     if (nactive) d_na.download(nactive, nvox);
+    const size_t over = size_t(std::count(it.begin(), it.end(), uint16_t(0xFFFE)));
+    if (over)
+      return mrcsd::fail(MRCSD_ERR_SETUP, "mricls_solve_host: " + std::to_string(over) +
+                                              " voxel(s) need more than 128 active constraints (iters 0xFFFE)");
     return MRCSD_OK;
   });
 }

@@ -7,15 +7,36 @@
 // active inequality with the smallest lambda_prev / (lambda_prev - l) (lowest index on ties) while
 // any multiplier is negative, stop when the added constraint was already active or after
 // max_niter outer iterations.  What differs is how each step is computed:
-//   * active constraints live in slots, one per lane; B_act B_act^T is gathered from the
-//     precomputed Gram matrix G = B B^T instead of being re-formed from B (k^2 n flops saved);
-//   * the k x k system is solved by a Gauss-Jordan sweep, thread a owning row a;
-//   * constraint values are updated as c = c_u + G[:, act] l, which equals B x - t for
-//     x = y_u + B_act^T l;
-//   * the final x = L^-T (y_u + B_act^T l) is formed as the reference forms it: y gathered from
-//     B^T, then a back substitution with L^T, one row per step (a precomputed L^-1 would lose
-//     cond(L)^2 rather than cond(L) of accuracy on nearly singular problems).
+//
+//   * The active set lives in slots, slot s on lane s % 64 (register s / 64), in the order the
+//     constraints entered.  The Cholesky factor L of B_act B_act^T + lambda I is kept from one
+//     inner solve to the next instead of being re-factorised (the reference forms the matrix and
+//     its LLT afresh every time, k^2 n + k^3 / 3 flops): row s of L is a row of LDS owned by slot s
+//     (an indirection, so that deleting a slot moves no rows).
+//       - adding constraint j: L w = G[act, j] by forward substitution, new row [w, sqrt(G_jj +
+//         lambda - w.w)];
+//       - removing slot p: delete row p, then Givens rotations on column pairs (c, c+1),
+//         c = p..k-2, restore the triangle (the rotated-out last column is dropped);
+//       - z = L^-1 (-c_u[act]) is updated alongside (one more entry on an addition; the same
+//         rotations on a removal: z behaves as one more row of L), so each solve is one backward
+//         substitution L^T l = z.
+//     Both substitutions run one column per step: the lane that owns the pivot finishes its
+//     entry, a readlane hands it to the wavefront, and every lane updates its own entry with one
+//     fused multiply-add (the coefficient, with the pivot's reciprocal folded in, is computed off
+//     that chain).
+//   * Rows of B_act B_act^T come from the precomputed Gram matrix G = B B^T, and the constraint
+//     values as c = c_u + G[:, act] l, which equals B x - t for x = y_u + B_act^T l.
+//   * A re-solve with an unchanged active set (the most violated constraint already active) is
+//     the previous solve, whose multipliers are all non-negative; the loop stops there, as the
+//     reference does after repeating it.
+//   * The final x = L_H^-T (y_u + B_act^T l) is formed as the reference forms it: y gathered from
+//     B^T, then a back substitution with L_H^T, one row per step.
 // All arithmetic is FP64; results agree with the reference to rounding.
+//
+// Capacity: a pass holds at most KCAP active constraints (its LDS factor is KCAP rows).  A voxel
+// that needs more is queued for the next pass, which solves it again from the start with more
+// slots (the arithmetic does not depend on the capacity); after the largest capacity (128) a voxel
+// is reported with iters 0xFFFE.
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <stdint.h>
@@ -26,249 +47,548 @@
 namespace mrcsd {
 namespace {
 
-constexpr int KMAX = kIclsMaxActive;
-constexpr int kQ = kIclsMaxCon / 64;
+struct IclsPass {
+  const uint32_t* in_list;             // voxels of this pass (null: every voxel, 0..nvox-1)
+  const unsigned long long* in_count;  // their number (device)
+  uint32_t* out_list;                  // voxels that outgrow this pass (null: last pass)
+  unsigned long long* out_count;
+  unsigned long long* counter;         // work counter of this pass
+};
 
-__device__ __forceinline__ bool test_bit(const unsigned long long* w, int i) {
-  unsigned long long word = 0ull;
+// Slot s's value of a per-slot register array (s wave-uniform).
+template <int SPL>
+__device__ __forceinline__ double slot_get(const double (&a)[SPL], int s) {
+  double v = a[0];
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) word = (q == (i >> 6)) ? w[q] : word;
-  return (word >> (i & 63)) & 1ull;
+  for (int j = 1; j < SPL; ++j) v = ((s >> 6) == j) ? a[j] : v;
+  return lane_bcast(v, s & 63);
 }
 
-__device__ __forceinline__ void set_bit(unsigned long long* w, int i, bool on) {
+template <int SPL>
+__device__ __forceinline__ int slot_get(const int (&a)[SPL], int s) {
+  int v = a[0];
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) {
-    const unsigned long long m = 1ull << (i & 63);
-    if (q == (i >> 6)) w[q] = on ? (w[q] | m) : (w[q] & ~m);
+  for (int j = 1; j < SPL; ++j) v = ((s >> 6) == j) ? a[j] : v;
+  return lane_bcast(v, s & 63);
+}
+
+// Slots p..last-1 take the value of the next slot.
+template <int SPL, class T>
+__device__ __forceinline__ void shift_down(T (&a)[SPL], int lane, int p, int last) {
+  T nxt[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    nxt[j] = __shfl(a[j], (lane + 1) & 63);
+    const T wrap = (j + 1 < SPL) ? lane_bcast(a[j + 1 < SPL ? j + 1 : j], 0) : nxt[j];
+    if (lane == 63) nxt[j] = wrap;
+  }
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int s = lane + 64 * j;
+    if (s >= p && s < last) a[j] = nxt[j];
   }
 }
 
-__global__ __launch_bounds__(64) void icls_persistent_kernel(IclsDeviceProblem P, IclsLaunch L) {
-  __shared__ double s_b[kIclsMaxMeas];
-  __shared__ double s_y[64];
-  __shared__ double s_cu[kIclsMaxCon];   // c_u, indexed by constraint
-  __shared__ double s_l[KMAX];
-  __shared__ int s_act[KMAX];
-  __shared__ double s_col[2][64];
-  __shared__ unsigned long long s_st[8];
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
 
-  const int tid = threadIdx.x;
-  const int n = P.n, nm = P.nmeas, ncon = P.ncon, ldg = P.ncon_pad;
-  const int nq = P.ncon_pad / 64;   // host guarantees ncon_pad % 64 == 0, ncon_pad <= kIclsMaxCon
-  const int num_ineq = ncon - P.num_eq;
-  if (tid < 8) s_st[tid] = 0ull;
+// 1/sqrt(h): the hardware estimate refined by two Newton steps (relative error near 2^-53).
+__device__ __forceinline__ double rsqrt_refined(double h) {
+  double y = __builtin_amdgcn_rsq(h);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-h * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+template <int KCAP, int KQ>
+__global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P, IclsLaunch L, IclsPass Q) {
+  constexpr int SPL = (KCAP + 63) / 64;   // slots per lane
+  constexpr int LD = KCAP + 1;            // odd row stride: the column reads of a step hit distinct banks
+  __shared__ double s_L[KCAP * LD];
+
+  const int lane = threadIdx.x;
+  const int n = P.n, nm = P.nmeas, ldg = P.ncon_pad;
+  const int nq = ldg / 64;                // host guarantees ncon_pad % 64 == 0 and nq <= KQ
+  const int num_ineq = P.ncon - P.num_eq;
+  const double lam = P.lambda_min;
+
+  // statistics of this wavefront (wave-uniform)
+  unsigned long long st_vox = 0, st_outer = 0, st_inner = 0, st_k = 0, st_k2 = 0, st_k3 = 0, st_kc = 0,
+                     st_ovf = 0;
+
+  // LDS row of each slot: a permutation of 0..KCAP-1 (lanes past KCAP keep row 0, never used)
+  int phys[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) phys[j] = (lane + 64 * j < KCAP) ? lane + 64 * j : 0;
 
   for (;;) {
-    unsigned long long vox = 0;
-    if (tid == 0) vox = atomicAdd(L.counter, 1ull);
-    vox = uniform64(vox);
-    if (vox >= L.nvox) break;
-    if (L.valid && !L.valid[vox]) {
-      if (tid < n) L.x[vox * n + tid] = 0.0f;
-      if (tid == 0) {
-        if (L.iters) L.iters[vox] = 0xFFFF;
-        if (L.nactive) L.nactive[vox] = 0;
+    unsigned long long idx = 0;
+    if (lane == 0) idx = atomicAdd(Q.counter, 1ull);
+    idx = uniform64(idx);
+    size_t vox;
+    if (Q.in_list) {
+      if (idx >= *Q.in_count) break;
+      vox = (size_t)uniform32((int)Q.in_list[idx]);
+    } else {
+      if (idx >= L.nvox) break;
+      vox = (size_t)idx;
+      if (L.valid && !L.valid[vox]) {
+        for (int j = lane; j < n; j += 64) L.x[vox * n + j] = 0.0f;
+        if (lane == 0) {
+          if (L.iters) L.iters[vox] = 0xFFFF;
+          if (L.nactive) L.nactive[vox] = 0;
+        }
+        continue;
       }
-      continue;
     }
     const float* bsrc = L.b + vox * (size_t)L.ld_b;
-    for (int r = tid; r < nm; r += 64) s_b[r] = (double)bsrc[r];
-    wave_sync();
 
-    // ---- unconstrained solution and its constraint values (:244-248) ----
-    double yu = 0.0;
-    if (tid < n)
-      for (int r = 0; r < nm; ++r) yu = fma(P.b2d[r * n + tid], s_b[r], yu);
-    s_y[tid] = yu;
-    wave_sync();
-    double c[kQ];
+    // ---- unconstrained solution y_u = b2d^T b (:244), parameter j on lane j % 64 ----
+    double yu[2];
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-      c[q] = 0.0;
-      if (q < nq) {
-        const int r = tid + 64 * q;
-        const double* bp = P.Bt + r;
-        double acc = 0.0;
-        for (int j = 0; j < n; ++j) acc = fma(bp[(size_t)j * ldg], s_y[j], acc);
-        acc -= (r < ncon) ? P.t[r] : 0.0;
-        c[q] = acc;
-        s_cu[r] = acc;   // s_cu holds ncon_pad entries
+    for (int h = 0; h < 2; ++h) {
+      const int j = lane + 64 * h;
+      double acc = 0.0;
+      if (h == 0 || n > 64) {
+        const int jj = j < n ? j : 0;
+#pragma unroll 8
+        for (int r = 0; r < nm; ++r) acc = fma(P.b2d[(size_t)r * n + jj], (double)bsrc[r], acc);
       }
+      yu[h] = j < n ? acc : 0.0;
+    }
+    // ---- constraint values of y_u: c_u = B y_u - t (:246-248), constraint lane + 64 q ----
+    double cu[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) cu[q] = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double yj = slot_get<2>(yu, j);
+      const double* bt = P.Bt + (size_t)j * ldg + lane;
+#pragma unroll
+      for (int q = 0; q < KQ; ++q)
+        if (q < nq) cu[q] = fma(bt[64 * q], yj, cu[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < KQ; ++q)
+      if (q < nq) cu[q] -= P.t[lane + 64 * q];
+
+    // ---- active set: empty, equality constraints active from the start (:255-261) ----
+    int k = 0;
+    int aidx[SPL];
+    double lp[SPL], lv[SPL], zv[SPL], invd[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      aidx[j] = -1;
+      lp[j] = lv[j] = zv[j] = invd[j] = 0.0;
+    }
+    unsigned long long actw = 0ull;   // lane w: bit i of constraint 64 w + i
+
+    // Appends constraint j as slot k: forward substitution for the new row of L and the new
+    // entry of z.
+    auto append = [&](int j) __attribute__((always_inline)) {
+      const double* gj = P.G + (size_t)j * ldg;
+      double r[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const int s = lane + 64 * jj;
+        const double g = gj[aidx[jj] >= 0 ? aidx[jj] : 0];
+        r[jj] = s < k ? g : 0.0;
+      }
+      const double gjj = gj[j] + lam;
+      for (int m = 0; m < k; ++m) {
+        const double im = slot_get<SPL>(invd, m);
+        double coef[SPL];
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) {
+          const int s = lane + 64 * jj;
+          const double Lsm = s_L[phys[jj] * LD + m];
+          coef[jj] = (s > m && s < k) ? Lsm * im : 0.0;
+        }
+        const double rm = slot_get<SPL>(r, m);
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) r[jj] = fma(-coef[jj], rm, r[jj]);
+      }
+      const int pk = slot_get<SPL>(phys, k);
+      double ww = 0.0, wz = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const int s = lane + 64 * jj;
+        const double w = s < k ? r[jj] * invd[jj] : 0.0;
+        if (s < k) s_L[pk * LD + s] = w;
+        ww = fma(w, w, ww);
+        wz = fma(w, zv[jj], wz);
+      }
+      ww = wave_sum_d(ww);
+      wz = wave_sum_d(wz);
+      const double d2 = gjj - ww;
+      const double y = rsqrt_refined(d2);
+      // right-hand side -c_u[j] of the new row
+      // (each word read out by readlane and the wanted one selected in scalar registers: a select
+      // over the lane's words would be lowered to an indexed scratch array)
+      double cj = 0.0;
+#pragma unroll
+      for (int q = 0; q < KQ; ++q) {
+        const double w = lane_bcast(cu[q], j & 63);
+        cj = ((j >> 6) == q) ? w : cj;
+      }
+      const double rhs = -cj;
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        if (lane + 64 * jj == k) {
+          s_L[pk * LD + k] = d2 * y;
+          invd[jj] = y;
+          zv[jj] = (rhs - wz) * y;
+          aidx[jj] = j;
+          lp[jj] = 0.0;
+          lv[jj] = 0.0;
+        }
+      }
+      ++k;
+    };
+
+    // l = L^-T z by backward substitution.
+    auto solve = [&]() __attribute__((always_inline)) {
+      double r[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) r[jj] = (lane + 64 * jj < k) ? zv[jj] : 0.0;
+      for (int m = k - 1; m >= 0; --m) {
+        const int pm = slot_get<SPL>(phys, m);
+        const double im = slot_get<SPL>(invd, m);
+        double coef[SPL];
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) {
+          const int s = lane + 64 * jj;
+          const double Lms = s_L[pm * LD + (s < m ? s : 0)];
+          coef[jj] = s < m ? Lms * im : 0.0;
+        }
+        const double rm = slot_get<SPL>(r, m);
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) r[jj] = fma(-coef[jj], rm, r[jj]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) lv[jj] = (lane + 64 * jj < k) ? r[jj] * invd[jj] : 0.0;
+    };
+
+    // Removes slot p: later slots move down one, the factor is re-triangularised by Givens
+    // rotations of column pairs (c, c+1), and z with it.
+    auto remove = [&](int p) __attribute__((always_inline)) {
+      const int K = k;
+      const int freed = slot_get<SPL>(phys, p);
+      shift_down<SPL>(aidx, lane, p, K - 1);
+      shift_down<SPL>(phys, lane, p, K - 1);
+      shift_down<SPL>(lp, lane, p, K - 1);
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj)
+        if (lane + 64 * jj == K - 1) {
+          phys[jj] = freed;
+          aidx[jj] = -1;
+          lp[jj] = 0.0;
+          lv[jj] = 0.0;
+        }
+      double u[SPL], v[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const double* row = s_L + phys[jj] * LD;
+        u[jj] = row[p];
+        v[jj] = row[p + 1 < LD ? p + 1 : p];
+      }
+      double zc = slot_get<SPL>(zv, p);
+      for (int c = p; c <= K - 2; ++c) {
+        const double a = slot_get<SPL>(u, c);
+        const double b = slot_get<SPL>(v, c);
+        const double h = fma(a, a, b * b);
+        const double y = rsqrt_refined(h);
+        const double cs = a * y, sn = b * y;
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) {
+          const int s = lane + 64 * jj;
+          double* row = s_L + phys[jj] * LD;
+          const double vn = row[c + 2 <= K - 1 ? c + 2 : c];
+          if (s == c) {
+            row[c] = h * y;
+            invd[jj] = y;
+          } else if (s > c && s <= K - 2) {
+            row[c] = fma(cs, u[jj], sn * v[jj]);
+            u[jj] = fma(-sn, u[jj], cs * v[jj]);
+          }
+          v[jj] = vn;
+        }
+        const double zc1 = slot_get<SPL>(zv, c + 1);
+        const double nz = fma(cs, zc, sn * zc1);
+        zc = fma(-sn, zc, cs * zc1);
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj)
+          if (lane + 64 * jj == c) zv[jj] = nz;
+      }
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj)
+        if (lane + 64 * jj == K - 1) {
+          zv[jj] = 0.0;
+          invd[jj] = 0.0;
+        }
+      k = K - 1;
+    };
+
+    auto set_bit = [&](int r, bool on) __attribute__((always_inline)) {
+      if (lane == (r >> 6)) {
+        const unsigned long long m = 1ull << (r & 63);
+        actw = on ? (actw | m) : (actw & ~m);
+      }
+    };
+
+    bool overflow = false;
+    for (int e = 0; e < P.num_eq; ++e) {
+      if (k >= KCAP) { overflow = true; break; }
+      append(num_ineq + e);
+      set_bit(num_ineq + e, true);
     }
 
-    // ---- active set: equality constraints start (and stay) active (:259-261) ----
-    int k = P.num_eq;
-    int my_act = tid < k ? num_ineq + tid : -1;
-    double lamprev = 0.0, la = 0.0;
-    if (tid < k) s_act[tid] = my_act;
-    unsigned long long actw[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) actw[q] = 0ull;
-    for (int e = 0; e < P.num_eq; ++e) set_bit(actw, num_ineq + e, true);
     int niter = 0;
-    bool fail = false;
-    wave_sync();
-
-    for (;;) {
+    double c[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) c[q] = cu[q];
+    while (!overflow) {
       // most violated inequality, lowest index on ties (:270)
       double bv = INFINITY;
       int bi = INT_MAX;
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) {
-        const int r = tid + 64 * q;
+      for (int q = 0; q < KQ; ++q) {
+        const int r = lane + 64 * q;
         if (q < nq && r < num_ineq && c[q] < bv) { bv = c[q]; bi = r; }
       }
       wave_argmin(bv, bi);
       bi = uniform32(bi);
-      if (!(bv < -P.tol)) break;
-      bool changed = !test_bit(actw, bi);
-      if (changed) {
-        if (k >= KMAX) { fail = true; break; }
-        if (tid == k) { my_act = bi; lamprev = 0.0; s_act[k] = bi; }
-        ++k;
-        set_bit(actw, bi, true);
+      if (!uniform32(bv < -P.tol ? 1 : 0)) break;
+      const bool was_active = (readlane64(actw, bi >> 6) >> (bi & 63)) & 1ull;
+      if (was_active) {
+        // the reference solves once more with the same set, gets the same (feasible)
+        // multipliers, counts the iteration and stops (:272-346)
+        const unsigned long long kk = (unsigned long long)k;
+        ++st_inner; st_k += kk; st_k2 += kk * kk; st_k3 += kk * kk * kk;
+        ++niter;
+        break;
       }
+      if (k >= KCAP) { overflow = true; break; }
+      append(bi);
+      set_bit(bi, true);
 
       for (;;) {   // inner loop (:274-331)
-        wave_sync();
-        const bool slot = tid < k;
-        // row `tid` of B_act B_act^T + lambda I, gathered from G
-        const double* grow = P.G + (size_t)(slot ? my_act : 0) * ldg;
-        double A[KMAX];
-#pragma unroll
-        for (int bb = 0; bb < KMAX; ++bb) {
-          double v = 0.0;
-          if (bb < k) {
-            const double g = grow[s_act[bb]];
-            v = slot ? g : 0.0;
-          }
-          A[bb] = (bb == tid) ? v + P.lambda_min : v;
-        }
-        double rhs = slot ? -s_cu[my_act] : 0.0;
-        // Gauss-Jordan sweep (thread a owns row a); steps past k are skipped by uniform branches
-#pragma unroll
-        for (int kk = 0; kk < KMAX; ++kk) {
-          if (kk < k) {
-            const int buf = kk & 1;
-            s_col[buf][tid] = A[kk];
-            const double d = lane_bcast(A[kk], kk);
-            const double bk = lane_bcast(rhs, kk);
-            wave_sync();
-            const double cf = (tid != kk) ? A[kk] / d : 0.0;
-#pragma unroll
-            for (int j = kk + 1; j < KMAX; ++j) A[j] = fma(-cf, s_col[buf][j], A[j]);
-            rhs = fma(-cf, bk, rhs);
-          }
-        }
-        double di = 1.0;
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) di = (tid == j) ? A[j] : di;
-        la = slot ? rhs / di : 0.0;
-        if (tid == 0) {
+        solve();
+        {
           const unsigned long long kk = (unsigned long long)k;
-          s_st[2] += 1ull;
-          s_st[3] += kk;
-          s_st[4] += kk * kk;
-          s_st[5] += kk * kk * kk;
+          ++st_inner; st_k += kk; st_k2 += kk * kk; st_k3 += kk * kk * kk;
         }
-
         // worst offender among negative inequality multipliers (:297-313)
         double sv = INFINITY;
         int si = INT_MAX;
-        if (slot && my_act < num_ineq && la < 0.0) {
-          sv = lamprev / (lamprev - la);
-          si = my_act;
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) {
+          const int s = lane + 64 * jj;
+          if (s < k && aidx[jj] < num_ineq && lv[jj] < 0.0) {
+            const double sj = lp[jj] / (lp[jj] - lv[jj]);
+            if (sj < sv || (sj == sv && aidx[jj] < si)) { sv = sj; si = aidx[jj]; }
+          }
         }
         wave_argmin(sv, si);
-        if (!(sv < INFINITY)) break;   // all feasible: x = y_u + B_act^T l (:316-321)
         si = uniform32(si);
-        changed = true;                // si is active by construction (:327-329)
-        set_bit(actw, si, false);
-        const int pos = uniform32(__ffsll((long long)__ballot(slot && my_act == si)) - 1);
-        const int last = k - 1;
-        const int act_last = lane_bcast(my_act, last);
-        const double lp_last = lane_bcast(lamprev, last);
-        if (tid == pos) { my_act = act_last; lamprev = lp_last; }
-        if (tid == last) { my_act = -1; lamprev = 0.0; }
-        --k;
-        wave_sync();
-        if (tid < k) s_act[tid] = my_act;
+        if (si == INT_MAX) break;   // all feasible: x = y_u + B_act^T l (:316-321)
+        int p = 0;
+        {
+          unsigned long long hit[SPL];
+#pragma unroll
+          for (int jj = 0; jj < SPL; ++jj) hit[jj] = __ballot(lane + 64 * jj < k && aidx[jj] == si);
+#pragma unroll
+          for (int jj = SPL - 1; jj >= 0; --jj)
+            if (hit[jj]) p = 64 * jj + __ffsll((long long)hit[jj]) - 1;
+        }
+        set_bit(si, false);
+        remove(p);
       }
-      if (fail) break;
 
-      lamprev = la;          // lambda_prev = lambda (:334)
+      // lambda_prev = lambda (:334): the multipliers of the active set, zero elsewhere
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) lp[jj] = (lane + 64 * jj < k) ? lv[jj] : 0.0;
       ++niter;
-      if (!changed || niter > P.max_niter) break;
+      if (niter > P.max_niter) break;
 
       // c = B x - t = c_u + G[:, act] l (:349-351)
-      wave_sync();
-      s_l[tid] = la;
-      if (tid < k) s_act[tid] = my_act;
-      wave_sync();
-      if (tid == 0) s_st[6] += (unsigned long long)k;
+      st_kc += (unsigned long long)k;
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) c[q] = (q < nq) ? s_cu[tid + 64 * q] : 0.0;
-      for (int a = 0; a < k; ++a) {
-        const double lv = s_l[a];
-        const double* g = P.G + (size_t)s_act[a] * ldg + tid;
+      for (int q = 0; q < KQ; ++q) c[q] = cu[q];
+#pragma unroll 4
+      for (int s = 0; s < k; ++s) {
+        const int as = slot_get<SPL>(aidx, s);
+        const double ls = slot_get<SPL>(lv, s);
+        const double* g = P.G + (size_t)as * ldg + lane;
 #pragma unroll
-        for (int q = 0; q < kQ; ++q)
-          if (q < nq) c[q] = fma(g[64 * q], lv, c[q]);
+        for (int q = 0; q < KQ; ++q)
+          if (q < nq) c[q] = fma(g[64 * q], ls, c[q]);
       }
     }
 
-    // ---- x = y_u + B_act^T l (:319), then L^T x = y solved in place (:355) ----
-    wave_sync();
-    s_l[tid] = tid < k ? la : 0.0;
-    s_act[tid] = tid < k ? my_act : 0;
-    wave_sync();
-    double yv = 0.0;   // lane j < n: y_j, then x_j
-    if (tid < n) {
-      yv = s_y[tid];
-      for (int a = 0; a < k; ++a) yv = fma(P.Bt[(size_t)tid * ldg + s_act[a]], s_l[a], yv);
+    if (overflow && Q.out_list) {
+      // solved again, from the start, by the next pass
+      if (lane == 0) Q.out_list[atomicAdd(Q.out_count, 1ull)] = (uint32_t)vox;
+      continue;
+    }
+
+    // ---- x = y_u + B_act^T l (:319), then L_H^T x = y solved in place (:355) ----
+    double y[2] = {yu[0], yu[1]};
+    for (int s = 0; s < k; ++s) {
+      const int as = slot_get<SPL>(aidx, s);
+      const double ls = slot_get<SPL>(lv, s);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        if (h == 0 || n > 64) {
+          const double bt = P.Bt[(size_t)(j < n ? j : 0) * ldg + as];
+          if (j < n) y[h] = fma(bt, ls, y[h]);
+        }
+      }
     }
     // back substitution by columns: step i finalises x_i and removes it from the rows above
     for (int i = n - 1; i >= 0; --i) {
-      const double xi = lane_bcast(yv, i) / P.L[(size_t)i * n + i];
-      if (tid == i) yv = xi;
-      if (tid < i) yv = fma(-P.L[(size_t)i * n + tid], xi, yv);
+      const double xi = slot_get<2>(y, i) / P.L[(size_t)i * n + i];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        const double lij = P.L[(size_t)i * n + (j < i ? j : 0)];
+        if (j == i) y[h] = xi;
+        else if (j < i) y[h] = fma(-lij, xi, y[h]);
+      }
     }
-    if (tid < n) L.x[vox * n + tid] = (float)yv;
-    if (tid == 0) {
-      if (L.iters) L.iters[vox] = fail ? (uint16_t)0xFFFE : (uint16_t)niter;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = lane + 64 * h;
+      if (j < n) L.x[vox * n + j] = (float)y[h];
+    }
+    if (lane == 0) {
+      if (L.iters) L.iters[vox] = overflow ? (uint16_t)0xFFFE : (uint16_t)niter;
       if (L.nactive) L.nactive[vox] = (uint16_t)k;
     }
-    if (tid == 0) {
-      s_st[0] += 1ull;
-      s_st[1] += (unsigned long long)niter;
-      s_st[7] += fail ? 1ull : 0ull;
-    }
-    wave_sync();
+    ++st_vox;
+    st_outer += (unsigned long long)niter;
+    st_ovf += overflow ? 1ull : 0ull;
   }
-  wave_sync();
-  if (L.stats && tid < 8) atomicAdd(L.stats + tid, s_st[tid]);
-}
 
-__global__ void reset_counter_kernel(unsigned long long* c) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ull;
+  if (L.stats && lane == 0) {
+    const unsigned long long v[8] = {st_vox, st_outer, st_inner, st_k, st_k2, st_k3, st_kc, st_ovf};
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (v[i]) atomicAdd(L.stats + i, v[i]);
+  }
 }
 
 }  // namespace
 
-hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, hipStream_t stream) {
+int icls_default_capacity(int n, int num_eq) {
+  for (int i = 0; i < kIclsNumCapacities; ++i)
+    if (kIclsCapacities[i] > n + num_eq) return kIclsCapacities[i];
+  return kIclsCapacities[kIclsNumCapacities - 1];
+}
+
+bool icls_valid_capacity(int kcap) {
+  for (int i = 0; i < kIclsNumCapacities; ++i)
+    if (kIclsCapacities[i] == kcap) return true;
+  return false;
+}
+
+namespace {
+
+// The passes of a solve: the chosen capacity, then 64 and 128 slots for the voxels that outgrow
+// the previous one.  Returns their number.
+int plan_passes(int first, int (&caps)[kIclsMaxPasses]) {
+  int npass = 0;
+  caps[npass++] = first;
+  if (first < 64) caps[npass++] = 64;
+  if (first < 128) caps[npass++] = 128;
+  return npass;
+}
+
+// Queues and counters of pass i of npass: it reads the queue the previous pass wrote.
+IclsPass pass_args(const IclsScratch& S, int i, int npass) {
+  IclsPass Q{};
+  Q.counter = S.counters + i;
+  if (i > 0) {
+    Q.in_list = S.queue[(i - 1) & 1];
+    Q.in_count = S.counters + kIclsMaxPasses + (i - 1);
+  }
+  if (i + 1 < npass) {
+    Q.out_list = S.queue[i & 1];
+    Q.out_count = S.counters + kIclsMaxPasses + i;
+  }
+  return Q;
+}
+
+// ---- HIP launches ----
+
+__global__ void reset_counters_kernel(unsigned long long* c, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
+
+template <int KCAP, int KQ>
+hipError_t launch_pass(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q, size_t max_grid,
+                       hipStream_t stream) {
+  auto kern = icls_active_set_kernel<KCAP, KQ>;
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, icls_persistent_kernel, 64, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   int dev = 0, ncu = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
   size_t grid = (size_t)ncu * (size_t)per_cu;
-  if (grid > L.nvox) grid = L.nvox ? L.nvox : 1;
-  reset_counter_kernel<<<1, 64, 0, stream>>>(L.counter);
-  icls_persistent_kernel<<<dim3((unsigned)grid), dim3(64), 0, stream>>>(P, L);
+  if (grid > max_grid) grid = max_grid;
+  if (grid < 1) grid = 1;
+  kern<<<dim3((unsigned)grid), dim3(64), 0, stream>>>(P, L, Q);
   return hipGetLastError();
+}
+
+template <int KCAP>
+hipError_t launch_kq(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q, size_t max_grid,
+                     hipStream_t stream) {
+  const int nq = P.ncon_pad / 64;
+  if (nq <= 4) return launch_pass<KCAP, 4>(P, L, Q, max_grid, stream);
+  if (nq <= 8) return launch_pass<KCAP, 8>(P, L, Q, max_grid, stream);
+  return launch_pass<KCAP, 16>(P, L, Q, max_grid, stream);
+}
+
+hipError_t launch_capacity(int kcap, const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q,
+                           size_t max_grid, hipStream_t stream) {
+  switch (kcap) {
+    case 16: return launch_kq<16>(P, L, Q, max_grid, stream);
+    case 32: return launch_kq<32>(P, L, Q, max_grid, stream);
+    case 48: return launch_kq<48>(P, L, Q, max_grid, stream);
+    case 64: return launch_kq<64>(P, L, Q, max_grid, stream);
+    case 96: return launch_kq<96>(P, L, Q, max_grid, stream);
+    case 128: return launch_kq<128>(P, L, Q, max_grid, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S,
+                       int first_capacity, hipStream_t stream) {
+  if (!icls_valid_capacity(first_capacity) || P.ncon_pad % 64 != 0 || P.ncon_pad / 64 > 16)
+    return hipErrorInvalidValue;
+  int caps[kIclsMaxPasses];
+  const int npass = plan_passes(first_capacity, caps);
+  if (npass > 1 && S.capacity < L.nvox) return hipErrorInvalidValue;
+  reset_counters_kernel<<<1, 64, 0, stream>>>(S.counters, 2 * kIclsMaxPasses);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int i = 0; i < npass; ++i) {
+    // later passes see few voxels: a small grid that exits at once when its queue is empty
+    const size_t max_grid = i == 0 ? (L.nvox ? L.nvox : 1) : 256;
+    if ((e = launch_capacity(caps[i], P, L, pass_args(S, i, npass), max_grid, stream)) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace mrcsd

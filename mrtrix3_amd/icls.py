@@ -29,6 +29,8 @@ def _register():
     _lib.register("mricls_dims", C.c_int, [_P, _I, _I, _I, _I])
     _lib.register("mricls_solve", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P, _P, _P])
     _lib.register("mricls_solve_host", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P])
+    _lib.register("mricls_set_capacity", C.c_int, [_P, C.c_int])
+    _lib.register("mricls_get_capacity", C.c_int, [_P])
     _lib.register("mrmsmt_shared_create", C.c_int, [_D, C.c_int, _D, _I, _I, C.c_int, _I, C.c_int, _D,
                                                    C.c_int, C.c_double, C.c_double, _D, C.c_int, C.POINTER(_P)])
     _lib.register("mrmsmt_shared_destroy", None, [_P])
@@ -62,6 +64,16 @@ class IclsProblem:
         d = [C.c_int() for _ in range(4)]
         check(lib.mricls_dims(self._h, *[C.byref(v) for v in d]))
         self.nmeas, self.n, self.ncon, self.max_niter = [v.value for v in d]
+
+    @property
+    def capacity(self):
+        """Active-set slots of the solver's first pass (voxels needing more are re-solved with 64,
+        then 128 slots)."""
+        return int(_lib.load().mricls_get_capacity(self._h))
+
+    @capacity.setter
+    def capacity(self, slots):
+        check(_lib.load().mricls_set_capacity(self._h, int(slots)))
 
     def solve(self, b, valid=None, stats=None, x=None):
         """b: [nvox, >= nmeas] float32 (torch CUDA tensor or numpy).  Returns (x, iters, nactive)."""

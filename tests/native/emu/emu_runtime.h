@@ -55,6 +55,8 @@ extern long long g_collectives;
 #define __builtin_amdgcn_readlane(v, s) emu_readlane((v), (s), __FILE__, __LINE__)
 #define __builtin_amdgcn_readfirstlane(v) emu_readfirst((v), __FILE__, __LINE__)
 #define __shfl_xor(v, o) emu_shfl_xor((v), (o), __FILE__, __LINE__)
+#define __shfl(v, s) emu_shfl((v), (s), __FILE__, __LINE__)
+#define __builtin_amdgcn_rsq(x) (1.0 / std::sqrt((double)(x)))
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 #define __ffsll(x) __builtin_ffsll(x)
 #define __popcll(x) __builtin_popcountll(x)
@@ -90,6 +92,18 @@ inline T emu_shfl_xor(T v, int off, const char* f, int l) {
   const uint64_t* s = emu::collective(u, emu::K_SHFL, f, l);
   T r;
   const uint64_t w = s[(emu::g_lane ^ off) & 63];
+  std::memcpy(&r, &w, sizeof(T));
+  return r;
+}
+// each lane reads the value of lane `src` (per lane, any lane of the wavefront)
+template <class T>
+inline T emu_shfl(T v, int src, const char* f, int l) {
+  static_assert(sizeof(T) <= 8, "");
+  uint64_t u = 0;
+  std::memcpy(&u, &v, sizeof(T));
+  const uint64_t* s = emu::collective(u, emu::K_SHFL, f, l);
+  T r;
+  const uint64_t w = s[src & 63];
   std::memcpy(&r, &w, sizeof(T));
   return r;
 }

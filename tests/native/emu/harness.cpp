@@ -2,6 +2,7 @@
 // ICLS kernel source compiled for the CPU against emu_runtime.h (KERNEL_SOURCE: the kernel file
 // without its launcher) and run one workgroup at a time.
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -10,10 +11,39 @@
 #include KERNEL_SOURCE
 
 namespace mrcsd {
-hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, hipStream_t) {
-  *L.counter = 0ull;
+// The passes of icls_launch, each as two emulated persistent workgroups run one after the other.
+template <int KCAP, int KQ>
+void run_pass(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q) {
   const unsigned grid = 2;
-  for (unsigned b = 0; b < grid; ++b) emu::run_workgroup([&]() { icls_persistent_kernel(P, L); }, b, grid);
+  for (unsigned b = 0; b < grid; ++b)
+    emu::run_workgroup([&]() { icls_active_set_kernel<KCAP, KQ>(P, L, Q); }, b, grid);
+}
+template <int KCAP>
+void run_kq(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q) {
+  const int nq = P.ncon_pad / 64;
+  if (nq <= 4) run_pass<KCAP, 4>(P, L, Q);
+  else if (nq <= 8) run_pass<KCAP, 8>(P, L, Q);
+  else run_pass<KCAP, 16>(P, L, Q);
+}
+hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S, int first,
+                       hipStream_t) {
+  int caps[kIclsMaxPasses];
+  const int np = plan_passes(first, caps);
+  for (int i = 0; i < 2 * kIclsMaxPasses; ++i) S.counters[i] = 0ull;
+  for (int i = 0; i < np; ++i) {
+    const IclsPass Q = pass_args(S, i, np);
+    switch (caps[i]) {
+      case 16: run_kq<16>(P, L, Q); break;
+      case 32: run_kq<32>(P, L, Q); break;
+      case 48: run_kq<48>(P, L, Q); break;
+      case 64: run_kq<64>(P, L, Q); break;
+      case 96: run_kq<96>(P, L, Q); break;
+      case 128: run_kq<128>(P, L, Q); break;
+      default: return hipErrorInvalidValue;
+    }
+    std::fprintf(stderr, "pass %d: capacity %d, queued for the next pass %llu\n", i, caps[i],
+                 i + 1 < np ? (unsigned long long)S.counters[kIclsMaxPasses + i] : 0ull);
+  }
   return hipSuccess;
 }
 }  // namespace mrcsd
@@ -30,9 +60,38 @@ std::vector<T> load(const std::string& p, size_t n) {
   return v;
 }
 
+// A plain ICLS problem (generic.txt: nmeas n ncon nvox con_reg; H.bin, A.bin, b.bin).
+int run_generic(const std::string& d, int capacity) {
+  int nm, n, nc, nvox;
+  double con_reg;
+  std::ifstream h(d + "/generic.txt");
+  h >> nm >> n >> nc >> nvox >> con_reg;
+  auto H = load<double>(d + "/H.bin", size_t(nm) * size_t(n));
+  auto A = load<double>(d + "/A.bin", size_t(nc) * size_t(n));
+  auto b = load<float>(d + "/b.bin", size_t(nvox) * size_t(nm));
+  mricls_ctx* ctx = nullptr;
+  if (mricls_create(H.data(), nm, n, A.data(), nc, nullptr, 0, 0.0, con_reg, 0, 0.0, 0, &ctx) ||
+      (capacity && mricls_set_capacity(ctx, capacity))) {
+    std::fprintf(stderr, "icls: %s\n", mrcsd_last_error());
+    return 2;
+  }
+  std::vector<float> x(size_t(nvox) * size_t(n));
+  std::vector<unsigned short> it(static_cast<size_t>(nvox)), na(static_cast<size_t>(nvox));
+  if (mricls_solve_host(ctx, b.data(), nm, nullptr, size_t(nvox), x.data(), it.data(), na.data())) {
+    std::fprintf(stderr, "solve: %s\n", mrcsd_last_error());
+    return 2;
+  }
+  std::ofstream out(d + "/x.bin", std::ios::binary);
+  out.write((const char*)x.data(), std::streamsize(x.size() * sizeof(float)));
+  std::ofstream o2(d + "/iters.txt");
+  for (int v = 0; v < nvox; ++v) o2 << it[size_t(v)] << " " << na[size_t(v)] << "\n";
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   const std::string d = argv[1];
+  if (std::ifstream(d + "/generic.txt").good()) return run_generic(d, argc > 2 ? std::atoi(argv[2]) : 0);
   int nvol, nvox, ld, nt;
   std::ifstream h(d + "/case.txt");
   h >> nvol >> nvox >> ld >> nt;
@@ -55,6 +114,10 @@ int main(int argc, char** argv) {
   mricls_ctx* ctx = nullptr;
   if (mrmsmt_icls_create(sh, &ctx)) {
     std::fprintf(stderr, "icls: %s\n", mrcsd_last_error());
+    return 2;
+  }
+  if (argc > 2 && mricls_set_capacity(ctx, std::atoi(argv[2]))) {
+    std::fprintf(stderr, "capacity: %s\n", mrcsd_last_error());
     return 2;
   }
   int nm, n, nc, mi;

@@ -126,3 +126,53 @@ def test_msmt_masked_voxels(torch_cuda):
     it = it.cpu().numpy().view(np.uint16)
     assert np.all(x[::5] == 0) and np.all(it[::5] == 0xFFFF)
     assert np.all(it[1::5] < 0xFFFE)
+
+
+def test_capacity_passes_give_identical_answers(torch_cuda):
+    """The multi-shell volume with a 16-slot first pass: every voxel outgrows it and is solved again
+    with 64 slots.  The arithmetic does not depend on the capacity, so the answers are the default
+    pass's bit for bit."""
+    torch = torch_cuda
+    ph = phantom.multi_shell(seed=4, shape=(16, 16, 8), semi=None)
+    p = icls.MSMTShared(ph.grad, ph.response).problem()
+    assert p.capacity == 48
+    b = torch.from_numpy(ph.dw).cuda()
+    valid = torch.from_numpy((np.arange(ph.nvox) % 7 != 3).astype(np.uint8)).cuda()
+    x0, it0, na0 = p.solve(b, valid=valid)
+    p.capacity = 16
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    x1, it1, na1 = p.solve(b, valid=valid, stats=st)
+    torch.cuda.synchronize()
+    say("[gpu icls] capacity 16 -> 64: stats", st.cpu().numpy().tolist())
+    assert torch.equal(x0, x1) and torch.equal(it0, it1) and torch.equal(na0, na1)
+    assert int(st[0]) == int(valid.sum()) and int(st[7]) == 0
+    assert int(na1.max()) > 16
+    p.capacity = 0
+    assert p.capacity == 48
+
+
+def test_large_active_sets_against_oracle(torch_cuda):
+    """Non-negative least squares with 100 unknowns whose answers keep about 80 of the 100
+    constraints active: the 16-slot first pass hands every voxel to the 64-slot pass and most on to
+    the 128-slot pass.  Against the oracle, which has no capacity."""
+    torch = torch_cuda
+    rng = np.random.default_rng(21)
+    n, m, nvox = 100, 160, 96
+    H = rng.standard_normal((m, n))
+    A = np.eye(n)
+    xt = rng.standard_normal((nvox, n)) - 0.9            # mostly negative: most bounds end active
+    B = (xt @ H.T + 0.01 * rng.standard_normal((nvox, m))).astype(np.float32)
+    p = icls.IclsProblem(H, A, con_reg=1e-10)
+    assert p.capacity == 128
+    p.capacity = 16
+    x, it, na = p.solve(torch.from_numpy(B).cuda())
+    torch.cuda.synchronize()
+    x, it, na = x.cpu().numpy(), it.cpu().numpy().view(np.uint16), na.cpu().numpy()
+    o = orc.IclsOracle(H, A, None, con_reg=1e-10)
+    bad = 0
+    for v in range(nvox):
+        xo, ito, acto = o.solve(B[v].astype(np.float64))
+        if not (np.linalg.norm(x[v] - xo) <= 1e-5 * np.linalg.norm(xo) and it[v] == ito and na[v] == acto.sum()):
+            bad += 1
+    say("[gpu icls] 100 unknowns: active %d..%d, mismatches %d / %d" % (na.min(), na.max(), bad, nvox))
+    assert na.max() > 64 and bad == 0

@@ -21,11 +21,14 @@ CSRC = os.path.join(ROOT, "mrtrix3_amd", "csrc")
 EMU = os.path.join(HERE, "native", "emu")
 
 
-@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
-def test_icls_kernel_under_wavefront_emulation(tmp_path):
+@pytest.fixture(scope="module")
+def emu_exe(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    tmp_path = tmp_path_factory.mktemp("emu")
     # the kernel file without its launcher (the harness launches the emulated workgroups)
     src = open(os.path.join(CSRC, "kernels", "icls_solve.hip")).read()
-    cut = src.index("__global__ void reset_counter_kernel")
+    cut = src.index("// ---- HIP launches ----")
     prep = tmp_path / "icls_kernel_emu.hip"
     prep.write_text(src[:cut] + "}  // namespace\n}  // namespace mrcsd\n")
     exe = str(tmp_path / "icls_emu")
@@ -36,6 +39,14 @@ def test_icls_kernel_under_wavefront_emulation(tmp_path):
            '-DKERNEL_SOURCE="%s"' % prep, "-x", "c++", os.path.join(EMU, "harness.cpp"),
            os.path.join(EMU, "emu_runtime.cpp")] + host + ["-o", exe]
     subprocess.run(cmd, check=True, timeout=600)
+    return exe
+
+
+@pytest.mark.parametrize("capacity", [0, 16])
+def test_icls_kernel_under_wavefront_emulation(tmp_path, emu_exe, capacity):
+    """capacity 0: the default first pass (48 slots for 47 parameters); 16: every voxel outgrows
+    the first pass and is queued for the 64-slot pass, which must give the same answers."""
+    exe = emu_exe
 
     ph = phantom.multi_shell(seed=6, shape=(4, 4, 4), semi=None)
     nv = 24
@@ -52,9 +63,13 @@ def test_icls_kernel_under_wavefront_emulation(tmp_path):
     np.concatenate([r.ravel() for r in resp]).tofile(d / "resp.bin")
     np.ascontiguousarray(ph.dw[:nv], dtype=np.float32).tofile(d / "dw.bin")
     valid.tofile(d / "valid.bin")
-    r = subprocess.run([exe, str(d)], capture_output=True, text=True, timeout=900)
+    r = subprocess.run([exe, str(d), str(capacity)], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "EMU:" not in r.stderr, r.stderr[-3000:]
+    if capacity == 16:
+        import re
+        queued = int(re.search(r"pass 0: capacity 16, queued for the next pass (\d+)", r.stderr).group(1))
+        assert queued >= 10   # the valid voxels whose active set passes 16 constraints
 
     s = icls.MSMTShared(ph.grad, ph.response)
     o = orc.MsmtOracle(ph.grad, ph.response)
@@ -67,4 +82,32 @@ def test_icls_kernel_under_wavefront_emulation(tmp_path):
             continue
         xo, it, act = o.solve(ph.dw[v].astype(np.float64))
         assert stats[v, 0] == it and stats[v, 1] == int(act.sum())
+        assert np.linalg.norm(x[v] - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_two_slots_per_lane_under_emulation(tmp_path, emu_exe):
+    """Non-negative least squares with 100 unknowns and 71-83 active bounds: passes of 16, 64 and
+    128 slots (two slots per lane in the last), against the oracle."""
+    rng = np.random.default_rng(21)
+    n, m, nvox = 100, 160, 6
+    H = rng.standard_normal((m, n))
+    A = np.eye(n)
+    xt = rng.standard_normal((nvox, n)) - 0.9
+    B = (xt @ H.T + 0.01 * rng.standard_normal((nvox, m))).astype(np.float32)
+    d = tmp_path / "generic"
+    d.mkdir()
+    (d / "generic.txt").write_text("%d %d %d %d %.17g\n" % (m, n, n, nvox, 1e-10))
+    H.astype(np.float64).tofile(d / "H.bin")
+    A.astype(np.float64).tofile(d / "A.bin")
+    B.tofile(d / "b.bin")
+    r = subprocess.run([emu_exe, str(d), "16"], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "EMU:" not in r.stderr, r.stderr[-3000:]
+    assert "pass 2: capacity 128" in r.stderr
+    x = np.fromfile(d / "x.bin", dtype=np.float32).reshape(nvox, n)
+    stats = np.loadtxt(d / "iters.txt", dtype=np.int64).reshape(nvox, 2)
+    o = orc.IclsOracle(H, A, None, con_reg=1e-10)
+    for v in range(nvox):
+        xo, it, act = o.solve(B[v].astype(np.float64))
+        assert stats[v, 0] == it and stats[v, 1] == int(act.sum()) and stats[v, 1] > 64
         assert np.linalg.norm(x[v] - xo) <= 1e-6 * np.linalg.norm(xo)
