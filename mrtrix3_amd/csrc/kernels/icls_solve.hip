@@ -108,9 +108,11 @@ __device__ __forceinline__ double rsqrt_refined(double h) {
 
 template <int KCAP, int KQ>
 __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P, IclsLaunch L, IclsPass Q) {
-  constexpr int SPL = (KCAP + 63) / 64;   // slots per lane
-  constexpr int LD = KCAP + 1;            // odd row stride: the column reads of a step hit distinct banks
-  __shared__ double s_L[KCAP * LD];
+  constexpr int SPL = (KCAP + 63) / 64;           // slots per lane
+  __shared__ double s_W[KCAP * (KCAP + 1) / 2];   // inverse factor W = L^-1, packed lower rows
+  __shared__ double s_v[KCAP];                    // a per-slot vector staged for broadcast reads
+  __shared__ double s_cs[KCAP], s_sn[KCAP];       // rotation parameters of a removal
+  __shared__ int s_a[KCAP];                       // constraint index of each slot (staged)
 
   const int lane = threadIdx.x;
   const int n = P.n, nm = P.nmeas, ldg = P.ncon_pad;
@@ -122,10 +124,8 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
   unsigned long long st_vox = 0, st_outer = 0, st_inner = 0, st_k = 0, st_k2 = 0, st_k3 = 0, st_kc = 0,
                      st_ovf = 0;
 
-  // LDS row of each slot: a permutation of 0..KCAP-1 (lanes past KCAP keep row 0, never used)
-  int phys[SPL];
-#pragma unroll
-  for (int j = 0; j < SPL; ++j) phys[j] = (lane + 64 * j < KCAP) ? lane + 64 * j : 0;
+  // start of packed row r
+  auto T = [](int r) __attribute__((always_inline)) { return r * (r + 1) / 2; };
 
   for (;;) {
     unsigned long long idx = 0;
@@ -180,70 +180,106 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
     // ---- active set: empty, equality constraints active from the start (:255-261) ----
     int k = 0;
     int aidx[SPL];
-    double lp[SPL], lv[SPL], zv[SPL], invd[SPL];
+    double lp[SPL], lv[SPL], zv[SPL], rhs[SPL];
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
       aidx[j] = -1;
-      lp[j] = lv[j] = zv[j] = invd[j] = 0.0;
+      lp[j] = lv[j] = zv[j] = rhs[j] = 0.0;
     }
     unsigned long long actw = 0ull;   // lane w: bit i of constraint 64 w + i
 
-    // Appends constraint j as slot k: forward substitution for the new row of L and the new
-    // entry of z.
-    auto append = [&](int j) __attribute__((always_inline)) {
-      const double* gj = P.G + (size_t)j * ldg;
-      double r[SPL];
+    // Stages x[slot] for slots < kk in s_v.
+    auto stage = [&](const double (&x)[SPL], int kk) __attribute__((always_inline)) {
+      wave_sync();   // earlier readers of s_v are done
 #pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) {
-        const int s = lane + 64 * jj;
-        const double g = gj[aidx[jj] >= 0 ? aidx[jj] : 0];
-        r[jj] = s < k ? g : 0.0;
-      }
-      const double gjj = gj[j] + lam;
-      for (int m = 0; m < k; ++m) {
-        const double im = slot_get<SPL>(invd, m);
-        double coef[SPL];
+      for (int jj = 0; jj < SPL; ++jj)
+        if (lane + 64 * jj < kk) s_v[lane + 64 * jj] = x[jj];
+      wave_sync();
+    };
+    // out_i = sum_{m <= i} W[i][m] s_v[m] for rows i < kk (a row of W per slot lane).
+    auto mv_rows = [&](double (&out)[SPL], int kk) __attribute__((always_inline)) {
+      double acc[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) acc[jj] = 0.0;
+#pragma unroll 4
+      for (int m = 0; m < kk; ++m) {
+        const double xm = s_v[m];
 #pragma unroll
         for (int jj = 0; jj < SPL; ++jj) {
-          const int s = lane + 64 * jj;
-          const double Lsm = s_L[phys[jj] * LD + m];
-          coef[jj] = (s > m && s < k) ? Lsm * im : 0.0;
+          const int i = lane + 64 * jj;
+          const bool on = m <= i && i < kk;
+          const double w = s_W[on ? T(i) + m : 0];
+          acc[jj] = fma(on ? w : 0.0, xm, acc[jj]);
         }
-        const double rm = slot_get<SPL>(r, m);
-#pragma unroll
-        for (int jj = 0; jj < SPL; ++jj) r[jj] = fma(-coef[jj], rm, r[jj]);
       }
-      const int pk = slot_get<SPL>(phys, k);
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) out[jj] = acc[jj];
+    };
+    // out_c = sum_{m >= c} W[m][c] s_v[m] for columns c < kk (W^T x; rows read contiguously).
+    auto mv_cols = [&](double (&out)[SPL], int kk) __attribute__((always_inline)) {
+      double acc[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) acc[jj] = 0.0;
+#pragma unroll 4
+      for (int m = 0; m < kk; ++m) {
+        const double xm = s_v[m];
+        const int base = T(m);
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) {
+          const int c = lane + 64 * jj;
+          const bool on = c <= m;
+          const double w = s_W[on ? base + c : 0];
+          acc[jj] = fma(on ? w : 0.0, xm, acc[jj]);
+        }
+      }
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) out[jj] = (lane + 64 * jj < kk) ? acc[jj] : 0.0;
+    };
+
+    // Appends constraint j as slot k.  With L w = g (g = G[act, j]), the new factor row is
+    // [w, d], d = sqrt(G_jj + lambda - w.w); its inverse gains the row [-(w^T W) / d, 1 / d], and z
+    // the entry (rhs_j - w.z) / d.
+    auto append = [&](int j) __attribute__((always_inline)) {
+      const double* gj = P.G + (size_t)j * ldg;
+      double g[SPL], w[SPL], nr[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const double v = gj[aidx[jj] >= 0 ? aidx[jj] : 0];
+        g[jj] = (lane + 64 * jj < k) ? v : 0.0;
+      }
+      const double gjj = gj[j] + lam;
+      stage(g, k);
+      mv_rows(w, k);                       // w = W g
       double ww = 0.0, wz = 0.0;
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
-        const int s = lane + 64 * jj;
-        const double w = s < k ? r[jj] * invd[jj] : 0.0;
-        if (s < k) s_L[pk * LD + s] = w;
-        ww = fma(w, w, ww);
-        wz = fma(w, zv[jj], wz);
+        if (lane + 64 * jj >= k) w[jj] = 0.0;
+        ww = fma(w[jj], w[jj], ww);
+        wz = fma(w[jj], zv[jj], wz);
       }
       ww = wave_sum_d(ww);
       wz = wave_sum_d(wz);
-      const double d2 = gjj - ww;
-      const double y = rsqrt_refined(d2);
-      // right-hand side -c_u[j] of the new row
-      // (each word read out by readlane and the wanted one selected in scalar registers: a select
-      // over the lane's words would be lowered to an indexed scratch array)
+      const double y = rsqrt_refined(gjj - ww);   // 1 / d
+      // right-hand side -c_u[j] (each word read out by readlane and the wanted one selected in
+      // scalar registers: a select over the lane's words would be lowered to an indexed scratch array)
       double cj = 0.0;
 #pragma unroll
       for (int q = 0; q < KQ; ++q) {
-        const double w = lane_bcast(cu[q], j & 63);
-        cj = ((j >> 6) == q) ? w : cj;
+        const double v = lane_bcast(cu[q], j & 63);
+        cj = ((j >> 6) == q) ? v : cj;
       }
-      const double rhs = -cj;
+      stage(w, k);
+      mv_cols(nr, k);                      // w^T W
+      const int base = T(k);
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
-        if (lane + 64 * jj == k) {
-          s_L[pk * LD + k] = d2 * y;
-          invd[jj] = y;
-          zv[jj] = (rhs - wz) * y;
+        const int c = lane + 64 * jj;
+        if (c < k) s_W[base + c] = -y * nr[jj];
+        if (c == k) {
+          s_W[base + c] = y;
           aidx[jj] = j;
+          rhs[jj] = -cj;
+          zv[jj] = (-cj - wz) * y;
           lp[jj] = 0.0;
           lv[jj] = 0.0;
         }
@@ -251,87 +287,97 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       ++k;
     };
 
-    // l = L^-T z by backward substitution.
+    // l = L^-T z = W^T z.
     auto solve = [&]() __attribute__((always_inline)) {
-      double r[SPL];
-#pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) r[jj] = (lane + 64 * jj < k) ? zv[jj] : 0.0;
-      for (int m = k - 1; m >= 0; --m) {
-        const int pm = slot_get<SPL>(phys, m);
-        const double im = slot_get<SPL>(invd, m);
-        double coef[SPL];
-#pragma unroll
-        for (int jj = 0; jj < SPL; ++jj) {
-          const int s = lane + 64 * jj;
-          const double Lms = s_L[pm * LD + (s < m ? s : 0)];
-          coef[jj] = s < m ? Lms * im : 0.0;
-        }
-        const double rm = slot_get<SPL>(r, m);
-#pragma unroll
-        for (int jj = 0; jj < SPL; ++jj) r[jj] = fma(-coef[jj], rm, r[jj]);
-      }
-#pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) lv[jj] = (lane + 64 * jj < k) ? r[jj] * invd[jj] : 0.0;
+      stage(zv, k);
+      mv_cols(lv, k);
     };
 
-    // Removes slot p: later slots move down one, the factor is re-triangularised by Givens
-    // rotations of column pairs (c, c+1), and z with it.
+    // Removes slot p.  Deleting row p of L and rotating column pairs (c, c+1), c = p..k-2, back to a
+    // triangle is, for W, moving column p to the end and rotating rows (c, c+1) so that the moved
+    // column vanishes: the rotation of step c takes a_c (the carried row's last entry; W[p][p] at
+    // the first step) and b_c = W[c+1][p], and leaves r_c = sqrt(a_c^2 + b_c^2) as the next a.  So
+    // r_c^2 is a prefix sum of W[p..c+1][p]^2 and every rotation is known before any is applied.
     auto remove = [&](int p) __attribute__((always_inline)) {
       const int K = k;
-      const int freed = slot_get<SPL>(phys, p);
-      shift_down<SPL>(aidx, lane, p, K - 1);
-      shift_down<SPL>(phys, lane, p, K - 1);
-      shift_down<SPL>(lp, lane, p, K - 1);
-#pragma unroll
-      for (int jj = 0; jj < SPL; ++jj)
-        if (lane + 64 * jj == K - 1) {
-          phys[jj] = freed;
-          aidx[jj] = -1;
-          lp[jj] = 0.0;
-          lv[jj] = 0.0;
-        }
-      double u[SPL], v[SPL];
+      double wp[SPL], q[SPL];
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
-        const double* row = s_L + phys[jj] * LD;
-        u[jj] = row[p];
-        v[jj] = row[p + 1 < LD ? p + 1 : p];
+        const int m = lane + 64 * jj;
+        const bool on = m >= p && m < K;
+        const double v = s_W[on ? T(m) + p : 0];
+        wp[jj] = on ? v : 0.0;
+        q[jj] = wp[jj] * wp[jj];
       }
-      double zc = slot_get<SPL>(zv, p);
-      for (int c = p; c <= K - 2; ++c) {
-        const double a = slot_get<SPL>(u, c);
-        const double b = slot_get<SPL>(v, c);
-        const double h = fma(a, a, b * b);
-        const double y = rsqrt_refined(h);
-        const double cs = a * y, sn = b * y;
+      // inclusive prefix sums of q over the slots
 #pragma unroll
-        for (int jj = 0; jj < SPL; ++jj) {
-          const int s = lane + 64 * jj;
-          double* row = s_L + phys[jj] * LD;
-          const double vn = row[c + 2 <= K - 1 ? c + 2 : c];
-          if (s == c) {
-            row[c] = h * y;
-            invd[jj] = y;
-          } else if (s > c && s <= K - 2) {
-            row[c] = fma(cs, u[jj], sn * v[jj]);
-            u[jj] = fma(-sn, u[jj], cs * v[jj]);
-          }
-          v[jj] = vn;
+      for (int jj = 0; jj < SPL; ++jj) {
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const double t = __shfl(q[jj], (lane - off) & 63);
+          if (lane >= off) q[jj] += t;
         }
-        const double zc1 = slot_get<SPL>(zv, c + 1);
-        const double nz = fma(cs, zc, sn * zc1);
-        zc = fma(-sn, zc, cs * zc1);
-#pragma unroll
-        for (int jj = 0; jj < SPL; ++jj)
-          if (lane + 64 * jj == c) zv[jj] = nz;
       }
+#pragma unroll
+      for (int jj = 1; jj < SPL; ++jj) q[jj] += lane_bcast(q[jj - 1], 63);
+      double qn[SPL], wn[SPL];   // the next slot's values
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        qn[jj] = q[jj];
+        wn[jj] = wp[jj];
+      }
+      shift_down<SPL>(qn, lane, 0, KCAP);
+      shift_down<SPL>(wn, lane, 0, KCAP);
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const int c = lane + 64 * jj;
+        if (c >= p && c <= K - 2) {
+          const double y = rsqrt_refined(qn[jj]);                            // 1 / r_c
+          const double a = (c == p) ? wp[jj] : q[jj] * rsqrt_refined(q[jj]);  // a_c
+          s_cs[c] = wn[jj] * y;
+          s_sn[c] = a * y;
+        }
+      }
+      shift_down<SPL>(aidx, lane, p, K - 1);
+      shift_down<SPL>(lp, lane, p, K - 1);
+      shift_down<SPL>(rhs, lane, p, K - 1);
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj)
         if (lane + 64 * jj == K - 1) {
-          zv[jj] = 0.0;
-          invd[jj] = 0.0;
+          aidx[jj] = -1;
+          lp[jj] = lv[jj] = zv[jj] = rhs[jj] = 0.0;
         }
+      wave_sync();
+      // rows p..K-2 of the new inverse factor, column j (old column j, or j + 1 past p) per lane;
+      // new row c is written over old row c, already read
+      double cur[SPL];
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const int j = lane + 64 * jj;
+        const double v = s_W[j < p ? T(p) + j : 0];
+        cur[jj] = j < p ? v : 0.0;
+      }
+      for (int c = p; c <= K - 2; ++c) {
+        const double cs = s_cs[c], sn = s_sn[c];
+        const int rb = T(c + 1), wb = T(c);
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) {
+          const int j = lane + 64 * jj;
+          const bool in = j <= c;
+          const double v = s_W[in ? rb + (j < p ? j : j + 1) : 0];
+          const double nx = in ? v : 0.0;
+          const double fin = fma(cs, cur[jj], -sn * nx);
+          if (in) s_W[wb + j] = fin;
+          cur[jj] = fma(sn, cur[jj], cs * nx);
+        }
+      }
       k = K - 1;
+      // z = W rhs for the new factor
+      stage(rhs, k);
+      mv_rows(zv, k);
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj)
+        if (lane + 64 * jj >= k) zv[jj] = 0.0;
     };
 
     auto set_bit = [&](int r, bool on) __attribute__((always_inline)) {
@@ -418,12 +464,17 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
 
       // c = B x - t = c_u + G[:, act] l (:349-351)
       st_kc += (unsigned long long)k;
+      stage(lv, k);
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj)
+        if (lane + 64 * jj < k) s_a[lane + 64 * jj] = aidx[jj];
+      wave_sync();
 #pragma unroll
       for (int q = 0; q < KQ; ++q) c[q] = cu[q];
 #pragma unroll 4
       for (int s = 0; s < k; ++s) {
-        const int as = slot_get<SPL>(aidx, s);
-        const double ls = slot_get<SPL>(lv, s);
+        const int as = s_a[s];
+        const double ls = s_v[s];
         const double* g = P.G + (size_t)as * ldg + lane;
 #pragma unroll
         for (int q = 0; q < KQ; ++q)
@@ -438,10 +489,15 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
     }
 
     // ---- x = y_u + B_act^T l (:319), then L_H^T x = y solved in place (:355) ----
+    stage(lv, k);
+#pragma unroll
+    for (int jj = 0; jj < SPL; ++jj)
+      if (lane + 64 * jj < k) s_a[lane + 64 * jj] = aidx[jj];
+    wave_sync();
     double y[2] = {yu[0], yu[1]};
     for (int s = 0; s < k; ++s) {
-      const int as = slot_get<SPL>(aidx, s);
-      const double ls = slot_get<SPL>(lv, s);
+      const int as = s_a[s];
+      const double ls = s_v[s];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int j = lane + 64 * h;
