@@ -180,11 +180,11 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
     // ---- active set: empty, equality constraints active from the start (:255-261) ----
     int k = 0;
     int aidx[SPL];
-    double lp[SPL], lv[SPL], zv[SPL], rhs[SPL];
+    double lp[SPL], lv[SPL];   // lambda_prev and the current multipliers l = A^-1 (-c_u[act]) of each slot
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
       aidx[j] = -1;
-      lp[j] = lv[j] = zv[j] = rhs[j] = 0.0;
+      lp[j] = lv[j] = 0.0;
     }
     unsigned long long actw = 0ull;   // lane w: bit i of constraint 64 w + i
 
@@ -196,69 +196,62 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
         if (lane + 64 * jj < kk) s_v[lane + 64 * jj] = x[jj];
       wave_sync();
     };
-    // out_i = sum_{m <= i} W[i][m] s_v[m] for rows i < kk (a row of W per slot lane).
+    // out_i = sum_{m <= i} W[i][m] s_v[m] for rows i < kk (a row of W per slot lane; lanes past
+    // kk give 0).
     auto mv_rows = [&](double (&out)[SPL], int kk) __attribute__((always_inline)) {
-      double acc[SPL];
 #pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) acc[jj] = 0.0;
+      for (int jj = 0; jj < SPL; ++jj) {
+        const int i = lane + 64 * jj;
+        const double* row = s_W + (i < kk ? T(i) : 0);
+        double acc = 0.0;
+        if (i < kk) {
 #pragma unroll 4
-      for (int m = 0; m < kk; ++m) {
-        const double xm = s_v[m];
-#pragma unroll
-        for (int jj = 0; jj < SPL; ++jj) {
-          const int i = lane + 64 * jj;
-          const bool on = m <= i && i < kk;
-          const double w = s_W[on ? T(i) + m : 0];
-          acc[jj] = fma(on ? w : 0.0, xm, acc[jj]);
+          for (int m = 0; m <= i; ++m) acc = fma(row[m], s_v[m], acc);
         }
+        out[jj] = acc;
       }
-#pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) out[jj] = acc[jj];
     };
-    // out_c = sum_{m >= c} W[m][c] s_v[m] for columns c < kk (W^T x; rows read contiguously).
+    // out_c = sum_{m >= c} W[m][c] s_v[m] for columns c < kk (W^T x; rows read contiguously;
+    // lanes past kk give 0).
     auto mv_cols = [&](double (&out)[SPL], int kk) __attribute__((always_inline)) {
       double acc[SPL];
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) acc[jj] = 0.0;
+      int tm = 0;   // start of row m
 #pragma unroll 4
       for (int m = 0; m < kk; ++m) {
         const double xm = s_v[m];
-        const int base = T(m);
 #pragma unroll
-        for (int jj = 0; jj < SPL; ++jj) {
-          const int c = lane + 64 * jj;
-          const bool on = c <= m;
-          const double w = s_W[on ? base + c : 0];
-          acc[jj] = fma(on ? w : 0.0, xm, acc[jj]);
-        }
+        for (int jj = 0; jj < SPL; ++jj)
+          if (lane + 64 * jj <= m) acc[jj] = fma(s_W[tm + lane + 64 * jj], xm, acc[jj]);
+        tm += m + 1;
       }
 #pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) out[jj] = (lane + 64 * jj < kk) ? acc[jj] : 0.0;
+      for (int jj = 0; jj < SPL; ++jj) out[jj] = acc[jj];
     };
 
     // Appends constraint j as slot k.  With L w = g (g = G[act, j]), the new factor row is
-    // [w, d], d = sqrt(G_jj + lambda - w.w); its inverse gains the row [-(w^T W) / d, 1 / d], and z
-    // the entry (rhs_j - w.z) / d.
+    // [w, d], d = sqrt(G_jj + lambda - w.w); the inverse factor gains the row u = [-(w^T W) / d,
+    // 1 / d].  With z = W rhs (never formed) the new entry of z is (rhs_j - w.z) / d, where
+    // w.z = g^T A^-1 rhs = g.l, and l = W^T z changes by u^T z_k.
     auto append = [&](int j) __attribute__((always_inline)) {
       const double* gj = P.G + (size_t)j * ldg;
       double g[SPL], w[SPL], nr[SPL];
+      double gl = 0.0;
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
         const double v = gj[aidx[jj] >= 0 ? aidx[jj] : 0];
         g[jj] = (lane + 64 * jj < k) ? v : 0.0;
+        gl = fma(g[jj], lv[jj], gl);
       }
       const double gjj = gj[j] + lam;
       stage(g, k);
       mv_rows(w, k);                       // w = W g
-      double ww = 0.0, wz = 0.0;
+      double ww = 0.0;
 #pragma unroll
-      for (int jj = 0; jj < SPL; ++jj) {
-        if (lane + 64 * jj >= k) w[jj] = 0.0;
-        ww = fma(w[jj], w[jj], ww);
-        wz = fma(w[jj], zv[jj], wz);
-      }
+      for (int jj = 0; jj < SPL; ++jj) ww = fma(w[jj], w[jj], ww);
       ww = wave_sum_d(ww);
-      wz = wave_sum_d(wz);
+      gl = wave_sum_d(gl);
       const double y = rsqrt_refined(gjj - ww);   // 1 / d
       // right-hand side -c_u[j] (each word read out by readlane and the wanted one selected in
       // scalar registers: a select over the lane's words would be lowered to an indexed scratch array)
@@ -268,29 +261,26 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
         const double v = lane_bcast(cu[q], j & 63);
         cj = ((j >> 6) == q) ? v : cj;
       }
+      const double zk = (-cj - gl) * y;
       stage(w, k);
       mv_cols(nr, k);                      // w^T W
       const int base = T(k);
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
         const int c = lane + 64 * jj;
-        if (c < k) s_W[base + c] = -y * nr[jj];
+        const double u = -y * nr[jj];
+        if (c < k) {
+          s_W[base + c] = u;
+          lv[jj] = fma(u, zk, lv[jj]);
+        }
         if (c == k) {
           s_W[base + c] = y;
           aidx[jj] = j;
-          rhs[jj] = -cj;
-          zv[jj] = (-cj - wz) * y;
           lp[jj] = 0.0;
-          lv[jj] = 0.0;
+          lv[jj] = y * zk;
         }
       }
       ++k;
-    };
-
-    // l = L^-T z = W^T z.
-    auto solve = [&]() __attribute__((always_inline)) {
-      stage(zv, k);
-      mv_cols(lv, k);
     };
 
     // Removes slot p.  Deleting row p of L and rotating column pairs (c, c+1), c = p..k-2, back to a
@@ -298,9 +288,11 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
     // column vanishes: the rotation of step c takes a_c (the carried row's last entry; W[p][p] at
     // the first step) and b_c = W[c+1][p], and leaves r_c = sqrt(a_c^2 + b_c^2) as the next a.  So
     // r_c^2 is a prefix sum of W[p..c+1][p]^2 and every rotation is known before any is applied.
+    // The multipliers of the smaller system are l' = l - (l_p / B_pp) B e_p (B = A^-1 = W^T W,
+    // B e_p = W^T W e_p, B_pp = |W e_p|^2), slot p dropped.
     auto remove = [&](int p) __attribute__((always_inline)) {
       const int K = k;
-      double wp[SPL], q[SPL];
+      double wp[SPL], q[SPL], bp[SPL];
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
         const int m = lane + 64 * jj;
@@ -309,6 +301,8 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
         wp[jj] = on ? v : 0.0;
         q[jj] = wp[jj] * wp[jj];
       }
+      stage(wp, K);
+      mv_cols(bp, K);                      // B e_p
       // inclusive prefix sums of q over the slots
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
@@ -320,6 +314,13 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       }
 #pragma unroll
       for (int jj = 1; jj < SPL; ++jj) q[jj] += lane_bcast(q[jj - 1], 63);
+      {
+        const double lpv = slot_get<SPL>(lv, p);
+        const double bpp = slot_get<SPL>(q, K - 1);   // |W e_p|^2
+        const double f = lpv / bpp;
+#pragma unroll
+        for (int jj = 0; jj < SPL; ++jj) lv[jj] = fma(-f, bp[jj], lv[jj]);
+      }
       double qn[SPL], wn[SPL];   // the next slot's values
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
@@ -340,12 +341,12 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       }
       shift_down<SPL>(aidx, lane, p, K - 1);
       shift_down<SPL>(lp, lane, p, K - 1);
-      shift_down<SPL>(rhs, lane, p, K - 1);
+      shift_down<SPL>(lv, lane, p, K - 1);
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj)
         if (lane + 64 * jj == K - 1) {
           aidx[jj] = -1;
-          lp[jj] = lv[jj] = zv[jj] = rhs[jj] = 0.0;
+          lp[jj] = lv[jj] = 0.0;
         }
       wave_sync();
       // rows p..K-2 of the new inverse factor, column j (old column j, or j + 1 past p) per lane;
@@ -357,27 +358,22 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
         const double v = s_W[j < p ? T(p) + j : 0];
         cur[jj] = j < p ? v : 0.0;
       }
+      int wb = T(p);
       for (int c = p; c <= K - 2; ++c) {
         const double cs = s_cs[c], sn = s_sn[c];
-        const int rb = T(c + 1), wb = T(c);
+        const int rb = wb + c + 1;   // start of old row c + 1
 #pragma unroll
         for (int jj = 0; jj < SPL; ++jj) {
           const int j = lane + 64 * jj;
-          const bool in = j <= c;
-          const double v = s_W[in ? rb + (j < p ? j : j + 1) : 0];
-          const double nx = in ? v : 0.0;
-          const double fin = fma(cs, cur[jj], -sn * nx);
-          if (in) s_W[wb + j] = fin;
-          cur[jj] = fma(sn, cur[jj], cs * nx);
+          if (j <= c) {
+            const double nx = s_W[rb + (j < p ? j : j + 1)];
+            s_W[wb + j] = fma(cs, cur[jj], -sn * nx);
+            cur[jj] = fma(sn, cur[jj], cs * nx);
+          }
         }
+        wb = rb;
       }
       k = K - 1;
-      // z = W rhs for the new factor
-      stage(rhs, k);
-      mv_rows(zv, k);
-#pragma unroll
-      for (int jj = 0; jj < SPL; ++jj)
-        if (lane + 64 * jj >= k) zv[jj] = 0.0;
     };
 
     auto set_bit = [&](int r, bool on) __attribute__((always_inline)) {
@@ -423,8 +419,7 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       append(bi);
       set_bit(bi, true);
 
-      for (;;) {   // inner loop (:274-331)
-        solve();
+      for (;;) {   // inner loop (:274-331); l is the solution for the current set
         {
           const unsigned long long kk = (unsigned long long)k;
           ++st_inner; st_k += kk; st_k2 += kk * kk; st_k3 += kk * kk * kk;
