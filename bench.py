@@ -492,7 +492,7 @@ def bench_msmt(args, torch, ph):
                    "form": "msmt_csd: ICLS::Solver dual active set, FP64, persistent kernel"},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic("icls_active_set_kernel_multi_shell"),
-                     "kernel": "icls_active_set_kernel<%d, %d>" % (prob.capacity, 4 if prob.ncon <= 256 else 8 if prob.ncon <= 512 else 16),
+                     "kernel": "icls_active_set_kernel<%d, %d>" % (prob.capacity, next(w for w in (4, 5, 8, 16) if 64 * w >= prob.ncon)),
                      "kernel_ms": kern_ms, "flop_per_launch": fl["total"],
                      "note": "FP64-bound; peak is the MI355X FP64 rate (vector = matrix)"},
         "icls": {"outer_per_voxel": float(st[1]) / max(1, st[0]), "inner_per_voxel": float(st[2]) / max(1, st[0]),

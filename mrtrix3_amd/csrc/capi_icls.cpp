@@ -50,7 +50,7 @@ int make_ctx(const Problem& pr, mricls_ctx** out) {
   if (pr.max_niter > 65533) return mrcsd::fail(MRCSD_ERR_ARG, "mricls: max_iter too large");
   int rc = mrcsd::require_gpu();
   if (rc) return rc;
-  const int ldg = (ncon + 63) / 64 * 64;
+  const int ldg = 64 * mrcsd::icls_constraint_words(ncon);
   std::vector<double> Bt(size_t(n) * ldg, 0.0), t(size_t(ldg), 0.0), G(size_t(ncon) * ldg, 0.0);
   for (int r = 0; r < ncon; ++r)
     for (int j = 0; j < n; ++j) Bt[size_t(j) * ldg + r] = pr.B(r, j);

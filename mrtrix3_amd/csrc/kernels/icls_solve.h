@@ -52,6 +52,9 @@ constexpr int kIclsMaxCon = 1024;
 // The capacity the first pass uses when none is chosen: the smallest above n + num_eq.
 int icls_default_capacity(int n, int num_eq);
 bool icls_valid_capacity(int kcap);
+// Words of 64 constraints a launch is compiled for (4, 5, 8 or 16): ncon_pad = 64 times this,
+// the padding constraints zero rows of B and G (their values stay 0 and never enter the argmin).
+int icls_constraint_words(int ncon);
 
 hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S,
                        int first_capacity, hipStream_t stream);

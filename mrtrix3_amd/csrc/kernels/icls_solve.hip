@@ -110,13 +110,14 @@ template <int KCAP, int KQ>
 __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P, IclsLaunch L, IclsPass Q) {
   constexpr int SPL = (KCAP + 63) / 64;           // slots per lane
   __shared__ double s_W[KCAP * (KCAP + 1) / 2];   // inverse factor W = L^-1, packed lower rows
-  __shared__ double s_v[KCAP];                    // a per-slot vector staged for broadcast reads
-  __shared__ double s_cs[KCAP], s_sn[KCAP];       // rotation parameters of a removal
-  __shared__ int s_a[KCAP];                       // constraint index of each slot (staged)
+  // per-slot values staged for broadcast reads: a vector (s_v[s]), rotation parameters
+  // (s_v[2c], s_v[2c + 1]) or (multiplier, constraint index) pairs; with the factor, under 10 KB
+  // at 48 slots, so that four wavefronts share each SIMD
+  __shared__ __attribute__((aligned(16))) double s_v[2 * KCAP];
 
   const int lane = threadIdx.x;
   const int n = P.n, nm = P.nmeas, ldg = P.ncon_pad;
-  const int nq = ldg / 64;                // host guarantees ncon_pad % 64 == 0 and nq <= KQ
+  // constraint r on lane r % 64, word r / 64: ncon_pad = 64 KQ (host), padding rows zero
   const int num_ineq = P.ncon - P.num_eq;
   const double lam = P.lambda_min;
 
@@ -171,11 +172,11 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       const double* bt = P.Bt + (size_t)j * ldg + lane;
 #pragma unroll
       for (int q = 0; q < KQ; ++q)
-        if (q < nq) cu[q] = fma(bt[64 * q], yj, cu[q]);
+        cu[q] = fma(bt[64 * q], yj, cu[q]);
     }
 #pragma unroll
     for (int q = 0; q < KQ; ++q)
-      if (q < nq) cu[q] -= P.t[lane + 64 * q];
+      cu[q] -= P.t[lane + 64 * q];
 
     // ---- active set: empty, equality constraints active from the start (:255-261) ----
     int k = 0;
@@ -194,6 +195,19 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj)
         if (lane + 64 * jj < kk) s_v[lane + 64 * jj] = x[jj];
+      wave_sync();
+    };
+    // (multiplier, constraint index) of each active slot in s_v
+    auto stage_pairs = [&]() __attribute__((always_inline)) {
+      wave_sync();
+#pragma unroll
+      for (int jj = 0; jj < SPL; ++jj) {
+        const int s = lane + 64 * jj;
+        if (s < k) {
+          s_v[2 * s] = lv[jj];
+          s_v[2 * s + 1] = (double)aidx[jj];
+        }
+      }
       wave_sync();
     };
     // out_i = sum_{m <= i} W[i][m] s_v[m] for rows i < kk (a row of W per slot lane; lanes past
@@ -321,6 +335,7 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
 #pragma unroll
         for (int jj = 0; jj < SPL; ++jj) lv[jj] = fma(-f, bp[jj], lv[jj]);
       }
+      wave_sync();   // mv_cols has read the staged column
       double qn[SPL], wn[SPL];   // the next slot's values
 #pragma unroll
       for (int jj = 0; jj < SPL; ++jj) {
@@ -335,8 +350,8 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
         if (c >= p && c <= K - 2) {
           const double y = rsqrt_refined(qn[jj]);                            // 1 / r_c
           const double a = (c == p) ? wp[jj] : q[jj] * rsqrt_refined(q[jj]);  // a_c
-          s_cs[c] = wn[jj] * y;
-          s_sn[c] = a * y;
+          s_v[2 * c] = wn[jj] * y;
+          s_v[2 * c + 1] = a * y;
         }
       }
       shift_down<SPL>(aidx, lane, p, K - 1);
@@ -360,7 +375,7 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       }
       int wb = T(p);
       for (int c = p; c <= K - 2; ++c) {
-        const double cs = s_cs[c], sn = s_sn[c];
+        const double cs = s_v[2 * c], sn = s_v[2 * c + 1];
         const int rb = wb + c + 1;   // start of old row c + 1
 #pragma unroll
         for (int jj = 0; jj < SPL; ++jj) {
@@ -401,7 +416,7 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
 #pragma unroll
       for (int q = 0; q < KQ; ++q) {
         const int r = lane + 64 * q;
-        if (q < nq && r < num_ineq && c[q] < bv) { bv = c[q]; bi = r; }
+        if (r < num_ineq && c[q] < bv) { bv = c[q]; bi = r; }
       }
       wave_argmin(bv, bi);
       bi = uniform32(bi);
@@ -459,21 +474,17 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
 
       // c = B x - t = c_u + G[:, act] l (:349-351)
       st_kc += (unsigned long long)k;
-      stage(lv, k);
-#pragma unroll
-      for (int jj = 0; jj < SPL; ++jj)
-        if (lane + 64 * jj < k) s_a[lane + 64 * jj] = aidx[jj];
-      wave_sync();
+      stage_pairs();
 #pragma unroll
       for (int q = 0; q < KQ; ++q) c[q] = cu[q];
 #pragma unroll 4
       for (int s = 0; s < k; ++s) {
-        const int as = s_a[s];
-        const double ls = s_v[s];
+        const int as = (int)s_v[2 * s + 1];
+        const double ls = s_v[2 * s];
         const double* g = P.G + (size_t)as * ldg + lane;
 #pragma unroll
         for (int q = 0; q < KQ; ++q)
-          if (q < nq) c[q] = fma(g[64 * q], ls, c[q]);
+          c[q] = fma(g[64 * q], ls, c[q]);
       }
     }
 
@@ -484,15 +495,11 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
     }
 
     // ---- x = y_u + B_act^T l (:319), then L_H^T x = y solved in place (:355) ----
-    stage(lv, k);
-#pragma unroll
-    for (int jj = 0; jj < SPL; ++jj)
-      if (lane + 64 * jj < k) s_a[lane + 64 * jj] = aidx[jj];
-    wave_sync();
+    stage_pairs();
     double y[2] = {yu[0], yu[1]};
     for (int s = 0; s < k; ++s) {
-      const int as = s_a[s];
-      const double ls = s_v[s];
+      const int as = (int)s_v[2 * s + 1];
+      const double ls = s_v[2 * s];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int j = lane + 64 * h;
@@ -541,6 +548,11 @@ int icls_default_capacity(int n, int num_eq) {
   for (int i = 0; i < kIclsNumCapacities; ++i)
     if (kIclsCapacities[i] > n + num_eq) return kIclsCapacities[i];
   return kIclsCapacities[kIclsNumCapacities - 1];
+}
+
+int icls_constraint_words(int ncon) {
+  const int w = (ncon + 63) / 64;
+  return w <= 4 ? 4 : w <= 5 ? 5 : w <= 8 ? 8 : 16;
 }
 
 bool icls_valid_capacity(int kcap) {
@@ -603,10 +615,13 @@ hipError_t launch_pass(const IclsDeviceProblem& P, const IclsLaunch& L, const Ic
 template <int KCAP>
 hipError_t launch_kq(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q, size_t max_grid,
                      hipStream_t stream) {
-  const int nq = P.ncon_pad / 64;
-  if (nq <= 4) return launch_pass<KCAP, 4>(P, L, Q, max_grid, stream);
-  if (nq <= 8) return launch_pass<KCAP, 8>(P, L, Q, max_grid, stream);
-  return launch_pass<KCAP, 16>(P, L, Q, max_grid, stream);
+  switch (P.ncon_pad / 64) {
+    case 4: return launch_pass<KCAP, 4>(P, L, Q, max_grid, stream);
+    case 5: return launch_pass<KCAP, 5>(P, L, Q, max_grid, stream);
+    case 8: return launch_pass<KCAP, 8>(P, L, Q, max_grid, stream);
+    case 16: return launch_pass<KCAP, 16>(P, L, Q, max_grid, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_capacity(int kcap, const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q,
@@ -626,7 +641,7 @@ hipError_t launch_capacity(int kcap, const IclsDeviceProblem& P, const IclsLaunc
 
 hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S,
                        int first_capacity, hipStream_t stream) {
-  if (!icls_valid_capacity(first_capacity) || P.ncon_pad % 64 != 0 || P.ncon_pad / 64 > 16)
+  if (!icls_valid_capacity(first_capacity) || P.ncon_pad != 64 * icls_constraint_words(P.ncon))
     return hipErrorInvalidValue;
   int caps[kIclsMaxPasses];
   const int npass = plan_passes(first_capacity, caps);

@@ -20,10 +20,12 @@ void run_pass(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q
 }
 template <int KCAP>
 void run_kq(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q) {
-  const int nq = P.ncon_pad / 64;
-  if (nq <= 4) run_pass<KCAP, 4>(P, L, Q);
-  else if (nq <= 8) run_pass<KCAP, 8>(P, L, Q);
-  else run_pass<KCAP, 16>(P, L, Q);
+  switch (P.ncon_pad / 64) {
+    case 4: run_pass<KCAP, 4>(P, L, Q); break;
+    case 5: run_pass<KCAP, 5>(P, L, Q); break;
+    case 8: run_pass<KCAP, 8>(P, L, Q); break;
+    default: run_pass<KCAP, 16>(P, L, Q); break;
+  }
 }
 hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S, int first,
                        hipStream_t) {
