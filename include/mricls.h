@@ -56,6 +56,16 @@ int mricls_get_capacity(const mricls_ctx* ctx);
  * voxel queues (a device allocation, after synchronising `stream`). */
 int mricls_solve(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                  float* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream);
+/* mricls_solve without any device allocation (no counterpart in the reference: for callers that
+ * cannot allocate -- code under stream capture, serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as mricls_solve; its scratch (pass counters and the voxel
+ * queues between capacities, 8 MiB per device, held until the process ends) is a fixed device
+ * array shared by every call on the device: calls that overlap on one device race, so serialise
+ * them.  At most MRICLS_NOALLOC_MAX_VOXELS voxels per call (MRCSD_ERR_ARG beyond). */
+#define MRICLS_NOALLOC_MAX_VOXELS (1u << 20)
+int mricls_solve_noalloc(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
+                         float* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream);
+
 /* Host pointers (copies in, solves, copies out; synchronous).  Fails with MRCSD_ERR_SETUP when a
  * voxel needs more than 128 active constraints (its outputs are still written). */
 int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,

@@ -45,6 +45,7 @@ LIB_SOURCES = [
     "capi_warm.cpp",
     "capi_peaks.cpp",
     "capi_resp.cpp",
+    "icls_noalloc.hip",
 ]
 
 

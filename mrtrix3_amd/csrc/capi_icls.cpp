@@ -10,6 +10,7 @@
 
 #include "../../include/mricls.h"
 #include "capi_common.h"
+#include "capi_icls_ctx.h"
 #include "host/gradient.h"
 #include "host/icls.h"
 #include "host/msmt.h"
@@ -17,18 +18,6 @@
 
 using MR::Math::ICLS::Problem;
 using mrcsd::Matrix;
-
-struct mricls_ctx {
-  int device = 0;
-  mrcsd::IclsDeviceProblem P{};
-  mrcsd::DeviceBuffer<double> b2d, Bt, t, G, L;
-  // scratch of a solve: work counters and queue lengths of the passes, and the two queues of
-  // voxels re-solved with more active-set slots (grown to the largest voxel count seen)
-  mrcsd::DeviceBuffer<unsigned long long> counters;
-  mrcsd::DeviceBuffer<uint32_t> queue;
-  size_t queue_cap = 0;
-  int capacity = 0;   // active-set slots of the first pass
-};
 
 struct mrmsmt_shared {
   std::unique_ptr<MR::DWI::SDeconv::MSMT_CSD::Shared> shared;
