@@ -33,18 +33,21 @@ LIB_SOURCES = [
     "kernels/mlp.hip",
     "kernels/peaks.hip",
     "kernels/resp.hip",
+    "kernels/fmls.hip",
     "host/sh.cpp",
     "host/dense.cpp",
     "host/gradient.cpp",
     "host/csd.cpp",
     "host/icls.cpp",
     "host/msmt.cpp",
+    "host/fmls.cpp",
     "capi_common.cpp",
     "capi_csd.cpp",
     "capi_icls.cpp",
     "capi_warm.cpp",
     "capi_peaks.cpp",
     "capi_resp.cpp",
+    "capi_fmls.cpp",
     "icls_noalloc.hip",
 ]
 
@@ -70,6 +73,7 @@ def _gen_dirs_inc():
     _gen_inc("electrostatic_repulsion_300.txt", "dirs300.inc", 300, "src/dwi/directions/predefined.cpp:92-393")
     # double literals converted to float where the table is declared, as the reference's float array is
     _gen_inc("sh2peaks_seeds_60.txt", "seeds60.inc", 60, "cmd/sh2peaks.cpp:363-424")
+    _gen_inc("tesselation_1281.txt", "dirs1281.inc", 1281, "src/dwi/directions/predefined.cpp")
 
 
 def _compile(rel):

@@ -1,0 +1,240 @@
+// C ABI of the GPU FOD segmentation (include/mrfixel.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mrfixel.h"
+#include "capi_common.h"
+#include "host/fmls.h"
+#include "host/sh.h"
+#include "kernels/fmls.h"
+
+namespace {
+
+constexpr size_t kChunk = 32768;   // voxels per amplitude pass
+constexpr int kBigBlocks = 64;     // workgroups of the pass over voxels beyond the LDS tables
+
+const mrcsd::fmls::DirectionSet& direction_set(const double* az_el, int n, mrcsd::fmls::DirectionSet& own) {
+  if (!az_el) return mrcsd::fmls::tessellation_1281();
+  own = mrcsd::fmls::make_direction_set(std::vector<double>(az_el, az_el + 2 * size_t(n)));
+  return own;
+}
+
+}  // namespace
+
+struct mrfixel_ctx {
+  int device = 0;
+  int n_sh = 0, lmax = 0, ndir = 0;
+  mrcsd::FmlsDeviceSet S{};
+  mrcsd::DeviceBuffer<double> unit, weights, sht, al;
+  mrcsd::DeviceBuffer<int16_t> adj, cell_dirs;
+  mrcsd::DeviceBuffer<int> cell_start;
+  mrcsd::DeviceBuffer<double> amps, big_ws;
+  mrcsd::DeviceBuffer<unsigned long long> counters, total;
+  mrcsd::DeviceBuffer<uint32_t> queue, stage_off, scan_ws;
+  mrcsd::DeviceBuffer<float> stage;
+};
+
+extern "C" {
+
+void mrfixel_options_default(mrfixel_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->integral_threshold = 0.0;     // FMLS_INTEGRAL_THRESHOLD_DEFAULT (fmls.h:31)
+  opt->peak_value_threshold = 0.1;   // FMLS_PEAK_VALUE_THRESHOLD_DEFAULT (fmls.h:32)
+  opt->lobe_merge_ratio = 1.0;       // FMLS_MERGE_RATIO_BRIDGE_TO_PEAK_DEFAULT (fmls.h:33)
+}
+
+int mrfixel_direction_set(const double* az_el, int n, mrfixel_set_info* info, double* unit, int* adj, double* weights,
+                          int* cell_start, int* cell_dirs) {
+  return mrcsd::guarded([&]() -> int {
+    if (!info) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_direction_set: null info");
+    if (az_el && (n < 64 || n > mrcsd::kFmlsMaxDirs))
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_direction_set: 64 to 1536 directions");
+    mrcsd::fmls::DirectionSet own;
+    const auto& s = direction_set(az_el, n, own);
+    info->ndir = s.n;
+    info->n_az = s.n_az;
+    info->n_el = s.n_el;
+    info->n_cell_dirs = int(s.cell_dirs.size());
+    if (unit) std::memcpy(unit, s.unit.data(), s.unit.size() * sizeof(double));
+    if (adj) std::memcpy(adj, s.adj.data(), s.adj.size() * sizeof(int));
+    if (weights) std::memcpy(weights, s.weights.data(), s.weights.size() * sizeof(double));
+    if (cell_start) std::memcpy(cell_start, s.cell_start.data(), s.cell_start.size() * sizeof(int));
+    if (cell_dirs) std::memcpy(cell_dirs, s.cell_dirs.data(), s.cell_dirs.size() * sizeof(int));
+    return MRCSD_OK;
+  });
+}
+
+int mrfixel_legendre_table(int lmax, int nrows, double* out) {
+  return mrcsd::guarded([&]() -> int {
+    if (!out || lmax < 0 || lmax > 16 || (lmax & 1) || nrows < 2)
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_legendre_table: even lmax up to 16, at least 2 rows");
+    const auto t = mrcsd::fmls::legendre_table(lmax, nrows);
+    std::memcpy(out, t.data(), t.size() * sizeof(double));
+    return MRCSD_OK;
+  });
+}
+
+int mrfixel_create(const mrfixel_options* opt, mrfixel_ctx** out) {
+  return mrcsd::guarded([&]() -> int {
+    if (!opt || !out) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_create: null argument");
+    *out = nullptr;
+    const int lmax = mrcsd::sh::LforN(opt->n_sh);
+    if (opt->n_sh < 1 || lmax < 2 || lmax > mrcsd::kFmlsMaxLmax || mrcsd::sh::NforL(lmax) != opt->n_sh)
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_create: the FOD must be an even SH series of lmax 2 to 12 (n_sh " +
+                                            std::to_string(opt->n_sh) + ")");
+    if (opt->directions && (opt->n_directions < 64 || opt->n_directions > mrcsd::kFmlsMaxDirs))
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_create: 64 to 1536 segmentation directions");
+    if (opt->maxnum < 0) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_create: maxnum must be positive (or 0: no limit)");
+    if (!(opt->lobe_merge_ratio >= 0.0)) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_create: invalid lobe merge ratio");
+    int rc = mrcsd::require_gpu();
+    if (rc) return rc;
+    mrcsd::fmls::DirectionSet own;
+    const auto& s = direction_set(opt->directions, opt->n_directions, own);
+    auto ctx = std::make_unique<mrfixel_ctx>();
+    MRCSD_HIP(hipGetDevice(&ctx->device));
+    ctx->n_sh = opt->n_sh;
+    ctx->lmax = lmax;
+    ctx->ndir = s.n;
+    // the amplitude transform of the directions as the Segmenter builds it (fmls.cpp:150-157):
+    // azimuth and elevation taken back from the unit vectors
+    mrcsd::Matrix azel(s.n, 2);
+    for (int i = 0; i < s.n; ++i) {
+      azel(i, 0) = std::atan2(s.unit[size_t(3 * i + 1)], s.unit[size_t(3 * i)]);
+      azel(i, 1) = std::acos(s.unit[size_t(3 * i + 2)]);
+    }
+    const mrcsd::Matrix T = mrcsd::sh::transform(azel, lmax);
+    std::vector<int16_t> adj16(s.adj.begin(), s.adj.end()), cells16(s.cell_dirs.begin(), s.cell_dirs.end());
+    const int al_rows = 2 * s.n;   // PrecomputedAL (lmax, 2 * dirs.size()) (fmls.cpp:146)
+    const auto al = mrcsd::fmls::legendre_table(lmax, al_rows);
+    ctx->unit.upload(s.unit.data(), s.unit.size());
+    ctx->weights.upload(s.weights.data(), s.weights.size());
+    ctx->sht.upload(T.a.data(), T.a.size());
+    ctx->al.upload(al.data(), al.size());
+    ctx->adj.upload(adj16.data(), adj16.size());
+    ctx->cell_dirs.upload(cells16.data(), cells16.size());
+    ctx->cell_start.upload(s.cell_start.data(), s.cell_start.size());
+    ctx->amps.alloc(kChunk * size_t(s.n));
+    ctx->big_ws.alloc(size_t(kBigBlocks) * mrcsd::fmls_big_stride(s.n));
+    ctx->counters.alloc(4);
+    ctx->total.alloc(1);
+    ctx->queue.alloc(kChunk);
+    auto& S = ctx->S;
+    S.ndir = s.n;
+    S.unit = ctx->unit.get();
+    S.adj = ctx->adj.get();
+    S.weights = ctx->weights.get();
+    S.sht = ctx->sht.get();
+    S.nsh = opt->n_sh;
+    S.lmax = lmax;
+    S.n_az = s.n_az;
+    S.n_el = s.n_el;
+    S.az_step = s.az_step;
+    S.el_step = s.el_step;
+    S.az_begin = s.az_begin;
+    S.el_begin = s.el_begin;
+    S.cell_start = ctx->cell_start.get();
+    S.cell_dirs = ctx->cell_dirs.get();
+    S.al = ctx->al.get();
+    S.al_ndir = al_rows;
+    S.integral_threshold = opt->integral_threshold;
+    S.peak_threshold = opt->peak_value_threshold;
+    S.merge_ratio = opt->lobe_merge_ratio;
+    S.maxnum = opt->maxnum;
+    S.dir_from_peak = opt->dir_from_peak ? 1 : 0;
+    *out = ctx.release();
+    return MRCSD_OK;
+  });
+}
+
+void mrfixel_destroy(mrfixel_ctx* ctx) { delete ctx; }
+
+int mrfixel_segment(mrfixel_ctx* ctx, const float* fod, int ld_fod, const uint8_t* mask, size_t nvox, uint32_t* count,
+                    uint32_t* offset, float* fixels, size_t capacity, float* summary, uint64_t* total, void* stream) {
+  return mrcsd::guarded([&]() -> int {
+    if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: null context");
+    if (!total) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: null total");
+    if (nvox && (!fod || !count || !offset || (capacity && !fixels)))
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: null fod, count, offset or fixels");
+    if (ld_fod < ctx->n_sh) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: ld_fod smaller than n_sh");
+    if (nvox > 0xFFFFFFFFull) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: at most 2^32 - 1 voxels per call");
+    if (!mrcsd::fmls_gpu_enabled())
+      return mrcsd::fail(MRCSD_ERR_SETUP, "mrfixel_segment: the GPU segmentation kernels are switched off in this build "
+                                          "(their first GPU run did not complete and the cause is not known)");
+    MRCSD_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t nb = mrcsd::fmls_scan_blocks(nvox);
+    if (ctx->stage.size() < 5 * capacity || ctx->stage_off.size() < nvox || ctx->scan_ws.size() < nb + 1) {
+      MRCSD_HIP(hipStreamSynchronize(st));   // a previous call may still use them
+      if (ctx->stage.size() < 5 * capacity) ctx->stage.alloc(5 * capacity);
+      if (ctx->stage_off.size() < nvox) ctx->stage_off.alloc(nvox);
+      if (ctx->scan_ws.size() < nb + 1) ctx->scan_ws.alloc(nb + 1);
+    }
+    mrcsd::FmlsLaunch L{};
+    L.fod = fod;
+    L.ld_fod = ld_fod;
+    L.valid = mask;
+    L.nvox = nvox;
+    L.amps = ctx->amps.get();
+    L.chunk = kChunk;
+    L.count = count;
+    L.offset = offset;
+    L.fixels = fixels;
+    L.stage = ctx->stage.get();
+    L.stage_off = ctx->stage_off.get();
+    L.capacity = capacity;
+    L.total = reinterpret_cast<unsigned long long*>(total);
+    L.summary = summary;
+    L.counters = ctx->counters.get();
+    L.queue = ctx->queue.get();
+    L.big_ws = ctx->big_ws.get();
+    L.big_blocks = kBigBlocks;
+    L.scan_ws = ctx->scan_ws.get();
+    MRCSD_HIP(mrcsd::fmls_launch(ctx->S, L, st));
+    return MRCSD_OK;
+  });
+}
+
+int mrfixel_segment_host(mrfixel_ctx* ctx, const float* fod, int ld_fod, const uint8_t* mask, size_t nvox,
+                         uint32_t* count, uint32_t* offset, float* fixels, size_t capacity, float* summary,
+                         uint64_t* total) {
+  return mrcsd::guarded([&]() -> int {
+    if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment_host: null context");
+    if (!total) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment_host: null total");
+    if (nvox && (!fod || !count || !offset || (capacity && !fixels)))
+      return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment_host: null fod, count, offset or fixels");
+    MRCSD_HIP(hipSetDevice(ctx->device));
+    mrcsd::DeviceBuffer<float> d_fod, d_fix, d_sum;
+    mrcsd::DeviceBuffer<uint8_t> d_mask;
+    mrcsd::DeviceBuffer<uint32_t> d_count, d_off;
+    mrcsd::DeviceBuffer<unsigned long long> d_total;
+    d_fod.upload(fod, nvox * size_t(ld_fod));
+    if (mask) d_mask.upload(mask, nvox);
+    d_count.alloc(nvox);
+    d_off.alloc(nvox);
+    d_fix.alloc(5 * capacity);
+    if (summary) d_sum.alloc(5 * nvox);
+    d_total.alloc(1);
+    int rc = mrfixel_segment(ctx, d_fod.get(), ld_fod, mask ? d_mask.get() : nullptr, nvox, d_count.get(), d_off.get(),
+                             d_fix.get(), capacity, summary ? d_sum.get() : nullptr,
+                             reinterpret_cast<uint64_t*>(d_total.get()), nullptr);
+    if (rc) return rc;
+    MRCSD_HIP(hipDeviceSynchronize());
+    unsigned long long t = 0;
+    d_total.download(&t, 1);
+    *total = t;
+    d_count.download(count, nvox);
+    d_off.download(offset, nvox);
+    const size_t nfix = t < capacity ? size_t(t) : capacity;
+    d_fix.download(fixels, 5 * nfix);
+    if (summary) d_sum.download(summary, 5 * nvox);
+    return MRCSD_OK;
+  });
+}
+
+}  // extern "C"

@@ -461,3 +461,125 @@ class MsmtOracle(IclsOracle):
                  neg_lambda=1e-10, **kw):
         self.C, self.A, self.lmax = msmt_matrices(grad, responses, lmax, shells, directions)
         super().__init__(self.C, self.A, None, 0, norm_lambda, neg_lambda, **kw)
+
+
+# ------------------------------------------------------------------------------------------
+# fod2fixel's FMLS segmentation (src/dwi/fmls.cpp), FP64
+# ------------------------------------------------------------------------------------------
+FMLS_MAX_ADJ = 8
+
+
+def load_dirs1281():
+    path = os.path.join(os.path.dirname(HERE), "mrtrix3_amd", "data", "tesselation_1281.txt")
+    return np.loadtxt(path)
+
+
+def fmls_unit_vectors(azel):
+    """Set::initialise (directions/set.h:128-139): unit vectors of [azimuth, elevation] rows."""
+    az, el = azel[:, 0], azel[:, 1]
+    s = np.sin(el)
+    return np.ascontiguousarray(np.stack([np.cos(az) * s, np.sin(az) * s, np.cos(el)], axis=1))
+
+
+def fmls_adjacency(unit):
+    """Neighbours of each direction: the edges of the convex hull of the directions and their
+    antipodes (what Set::initialise_adjacency, set.cpp:70-313, computes), here by scipy's Qhull.
+    [n, FMLS_MAX_ADJ] int32, ascending, -1 padded."""
+    from scipy.spatial import ConvexHull
+    n = unit.shape[0]
+    hull = ConvexHull(np.concatenate([unit, -unit]))
+    nb = [set() for _ in range(n)]
+    for s in hull.simplices:
+        for a in s:
+            for b in s:
+                if a != b and (a % n) != (b % n):
+                    nb[a % n].add(b % n)
+    out = -np.ones((n, FMLS_MAX_ADJ), dtype=np.int32)
+    for i in range(n):
+        v = sorted(nb[i])
+        out[i, :len(v)] = v
+    return out
+
+
+def fmls_weights(unit):
+    """IntegrationWeights (fmls.cpp:109-141): least-squares weights with which the directions
+    integrate every even SH basis function up to LforN(n) + 2 (4 pi for the constant)."""
+    azel = np.stack([np.arctan2(unit[:, 1], unit[:, 0]), np.arccos(unit[:, 2])], axis=1)
+    A = sh_transform(azel, lforn(unit.shape[0]) + 2).T
+    b = np.zeros(A.shape[0])
+    b[0] = 2.0 * np.sqrt(np.pi)
+    q, r = np.linalg.qr(A)
+    return np.linalg.solve(r, q.T @ b)
+
+
+class FmlsOracle:
+    """Segmenter (fmls.cpp:144-377) on the 1281-direction set fod2fixel uses (fod2fixel.cpp:318)."""
+
+    def __init__(self, lmax, integral=0.0, peak_value=0.1, merge_ratio=1.0, azel=None):
+        L = lib()
+        for name, res, args in (
+                ("orc_fmls_al_table", C.c_int, [C.c_int, C.c_int, D]),
+                ("orc_fmls_get_peak", C.c_double, [D, C.c_int, D, D, C.c_int]),
+                ("orc_fmls_lookup_create", C.c_void_p, [D, C.c_int, I]),
+                ("orc_fmls_lookup_free", None, [C.c_void_p]),
+                ("orc_fmls_lookup_info", C.c_int, [C.c_void_p, I, I, I, I, C.c_int]),
+                ("orc_fmls_select", C.c_int, [C.c_void_p, D, D]),
+                ("orc_fmls_segment", C.c_int, [D, C.c_int, D, C.c_int, D, I, D, C.c_void_p, D, C.c_int,
+                                               C.c_double, C.c_double, C.c_double, D, C.c_int, I])):
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        self.L = L
+        self.lmax = int(lmax)
+        self.azel = load_dirs1281() if azel is None else _arr(azel)
+        self.unit = fmls_unit_vectors(self.azel)
+        self.n = self.unit.shape[0]
+        self.adj = fmls_adjacency(self.unit)
+        self.weights = fmls_weights(self.unit)
+        az = np.arctan2(self.unit[:, 1], self.unit[:, 0])
+        el = np.arccos(self.unit[:, 2])
+        self.SHT = np.ascontiguousarray(sh_transform(np.stack([az, el], axis=1), self.lmax))
+        self.al_ndir = 2 * self.n
+        nal = (self.lmax // 2 + 1) ** 2
+        self.al = np.zeros((self.al_ndir, nal))
+        L.orc_fmls_al_table(self.lmax, self.al_ndir, _d(self.al))
+        self.look = L.orc_fmls_lookup_create(_d(self.unit), self.n, self.adj.ctypes.data_as(I))
+        self.integral, self.peak_value, self.merge_ratio = float(integral), float(peak_value), float(merge_ratio)
+
+    def lookup_tables(self):
+        naz, nel = C.c_int(), C.c_int()
+        total = self.L.orc_fmls_lookup_info(self.look, C.byref(naz), C.byref(nel), None, None, 0)
+        start = np.zeros(naz.value * nel.value + 1, dtype=np.int32)
+        idx = np.zeros(total, dtype=np.int32)
+        self.L.orc_fmls_lookup_info(self.look, C.byref(naz), C.byref(nel), start.ctypes.data_as(I),
+                                    idx.ctypes.data_as(I), total)
+        return naz.value, nel.value, start, idx
+
+    def select(self, p):
+        p = _arr(p)
+        return int(self.L.orc_fmls_select(self.look, _d(self.unit), _d(p)))
+
+    def get_peak(self, sh, direction):
+        sh = _arr(sh)
+        d = _arr(direction).copy()
+        v = self.L.orc_fmls_get_peak(_d(sh), self.lmax, _d(d), _d(self.al), self.al_ndir)
+        return float(v), d
+
+    def segment(self, sh, max_out=64):
+        """One FOD -> (lobes [k, 8]: mean_dir[3], peak_dir[3], integral, max_peak, in the reference's
+        order; lut [n]: lobe of each direction, k where none)."""
+        sh = _arr(sh)
+        out = np.zeros((max_out, 8))
+        lut = np.zeros(self.n, dtype=np.int32)
+        k = self.L.orc_fmls_segment(_d(sh), self.lmax, _d(self.SHT), self.n, _d(self.unit),
+                                    self.adj.ctypes.data_as(I), _d(self.weights), self.look, _d(self.al),
+                                    self.al_ndir, self.integral, self.peak_value, self.merge_ratio,
+                                    _d(out), max_out, lut.ctypes.data_as(I))
+        if k < 0:
+            raise ValueError("more than 1281 lobes before thresholding")
+        return out[:min(k, max_out)].copy(), lut
+
+    def __del__(self):
+        look = getattr(self, "look", None)
+        if look:
+            self.L.orc_fmls_lookup_free(look)
+            self.look = None

@@ -13,6 +13,8 @@ Reads two reference source files *as text* (it never compiles or imports them) a
 * ``mrtrix3_amd/data/sh2peaks_seeds_60.txt`` — the 60 built-in search directions of ``sh2peaks``
   (``cmd/sh2peaks.cpp:363-424``, ``default_directions``) as ``azimuth elevation`` rows, the text
   format of ``sh2peaks -seeds``; the reference holds them as float.
+* ``mrtrix3_amd/data/tesselation_1281.txt`` — the 1281 directions ``fod2fixel`` segments FODs on
+  (``src/dwi/directions/predefined.cpp``, ``tesselation_1281_data``; ``cmd/fod2fixel.cpp:318``).
 
 Usage: ``python tests/golden/extract_fixtures.py [/path/to/reference]``.  Run once; the outputs
 are committed, and nothing at test or run time reads the reference tree.
@@ -81,7 +83,15 @@ def main(ref):
         f.write("# MRtrix3 cmd/sh2peaks.cpp:363-424 (default_directions, float), MPL-2.0\n")
         for az, el in seeds:
             f.write("%s %s\n" % (repr(float(az)), repr(float(el))))
-    print("wrote icls_golden.npz, electrostatic_repulsion_300.txt and sh2peaks_seeds_60.txt")
+    d = _array(pre, "tesselation_1281_data").reshape(-1, 2)
+    assert d.shape == (1281, 2)
+    path = os.path.join(REPO, "mrtrix3_amd", "data", "tesselation_1281.txt")
+    with open(path, "w") as f:
+        f.write("# 1281 FOD segmentation directions [ azimuth elevation ] (radians)\n")
+        f.write("# MRtrix3 src/dwi/directions/predefined.cpp (tesselation_1281_data), MPL-2.0\n")
+        for az, el in d:
+            f.write("%s %s\n" % (repr(float(az)), repr(float(el))))
+    print("wrote icls_golden.npz, electrostatic_repulsion_300.txt, sh2peaks_seeds_60.txt and tesselation_1281.txt")
 
 
 if __name__ == "__main__":

@@ -3,6 +3,7 @@
 // cross-lane operation and __syncthreads is a collective all 64 lanes must reach together (same
 // kind and call site), otherwise the run stops and reports where the lanes are.
 #pragma once
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -110,6 +111,10 @@ inline T emu_shfl(T v, int src, const char* f, int l) {
 inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
 inline double __longlong_as_double(long long x) { double r; std::memcpy(&r, &x, 8); return r; }
 template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p = o + v; return o; }
+using std::max;
+using std::min;
+using std::isfinite;
+using std::isnan;
 
 // HIP runtime stand-ins over host memory
 inline hipError_t hipMalloc(void** p, size_t n) { *p = std::calloc(1, n ? n : 1); return *p ? hipSuccess : hipErrorOutOfMemory; }

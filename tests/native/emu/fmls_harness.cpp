@@ -1,0 +1,125 @@
+// Runs mrfixel on a case written by tests/test_fmls_kernel_emulated.py through the C ABI, with the
+// FMLS segmentation kernel compiled for the CPU against emu_runtime.h (KERNEL_SOURCE: the kernel
+// file without its launchers) and run one workgroup at a time.  The amplitude GEMM (FMA in
+// ascending coefficient order, as the GPU kernel accumulates), the offset scan and the compaction
+// are restated here on the host.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../../include/mrfixel.h"
+#include KERNEL_SOURCE
+
+namespace mrcsd {
+template <int LMAX>
+void run_segment(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0, size_t nv) {
+  const unsigned grid = 2;
+  for (unsigned b = 0; b < grid; ++b)
+    emu::run_workgroup([&]() { fmls_segment_kernel<LMAX, false>(S, L, v0, nv); }, b, grid);
+  std::fprintf(stderr, "chunk at %zu: %llu voxel(s) queued for the workspace pass\n", v0,
+               (unsigned long long)L.counters[1]);
+  for (unsigned b = 0; b < grid; ++b)
+    emu::run_workgroup([&]() { fmls_segment_kernel<LMAX, true>(S, L, v0, nv); }, b, grid);
+}
+
+// the emulated kernels run whatever the device build's switch says
+bool fmls_gpu_enabled() { return true; }
+
+hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t) {
+  *L.total = 0ull;
+  for (size_t v0 = 0; v0 < L.nvox; v0 += L.chunk) {
+    const size_t nv = L.nvox - v0 < L.chunk ? L.nvox - v0 : L.chunk;
+    for (int i = 0; i < 4; ++i) L.counters[i] = 0ull;
+    for (size_t v = 0; v < nv; ++v)
+      for (int d = 0; d < S.ndir; ++d) {
+        double acc = 0.0;
+        for (int k = 0; k < S.nsh; ++k)
+          acc = std::fma(S.sht[size_t(d) * S.nsh + k], (double)L.fod[(v0 + v) * size_t(L.ld_fod) + k], acc);
+        L.amps[v * size_t(S.ndir) + d] = acc;
+      }
+    switch (S.lmax) {
+      case 2: run_segment<2>(S, L, v0, nv); break;
+      case 4: run_segment<4>(S, L, v0, nv); break;
+      case 6: run_segment<6>(S, L, v0, nv); break;
+      case 8: run_segment<8>(S, L, v0, nv); break;
+      case 10: run_segment<10>(S, L, v0, nv); break;
+      case 12: run_segment<12>(S, L, v0, nv); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  size_t off = 0;
+  for (size_t v = 0; v < L.nvox; ++v) {
+    const uint32_t c = L.count[v];
+    L.offset[v] = c ? uint32_t(off) : 0u;
+    const size_t src = L.stage_off[v];
+    if (c && off + c <= L.capacity && src + c <= L.capacity)
+      for (uint32_t q = 0; q < 5 * c; ++q) L.fixels[off * 5 + q] = L.stage[src * 5 + q];
+    off += c;
+  }
+  return hipSuccess;
+}
+}  // namespace mrcsd
+
+template <class T>
+std::vector<T> load(const std::string& p, size_t n) {
+  std::vector<T> v(n);
+  std::ifstream f(p, std::ios::binary);
+  f.read((char*)v.data(), std::streamsize(n * sizeof(T)));
+  if (!f) {
+    std::fprintf(stderr, "cannot read %s\n", p.c_str());
+    std::exit(2);
+  }
+  return v;
+}
+
+template <class T>
+void save(const std::string& p, const T* v, size_t n) {
+  std::ofstream f(p, std::ios::binary);
+  f.write((const char*)v, std::streamsize(n * sizeof(T)));
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) return 2;
+  const std::string d = argv[1];
+  // case.txt: nvox n_sh maxnum dir_from_peak / integral_threshold peak_threshold merge_ratio
+  size_t nvox;
+  int n_sh, maxnum, dirpeak;
+  double ithr, pthr, ratio;
+  std::ifstream h(d + "/case.txt");
+  h >> nvox >> n_sh >> maxnum >> dirpeak >> ithr >> pthr >> ratio;
+  if (!h) return 2;
+  auto fod = load<float>(d + "/fod.bin", nvox * size_t(n_sh));
+  auto mask = load<unsigned char>(d + "/mask.bin", nvox);
+  mrfixel_options opt;
+  mrfixel_options_default(&opt);
+  opt.n_sh = n_sh;
+  opt.maxnum = maxnum;
+  opt.dir_from_peak = dirpeak;
+  opt.integral_threshold = ithr;
+  opt.peak_value_threshold = pthr;
+  opt.lobe_merge_ratio = ratio;
+  mrfixel_ctx* ctx = nullptr;
+  if (mrfixel_create(&opt, &ctx)) {
+    std::fprintf(stderr, "create: %s\n", mrcsd_last_error());
+    return 2;
+  }
+  const size_t cap = 64 * nvox;
+  std::vector<uint32_t> count(nvox), offset(nvox);
+  std::vector<float> fixels(5 * cap), summary(5 * nvox);
+  uint64_t total = 0;
+  if (mrfixel_segment_host(ctx, fod.data(), n_sh, mask.data(), nvox, count.data(), offset.data(), fixels.data(), cap,
+                           summary.data(), &total)) {
+    std::fprintf(stderr, "segment: %s\n", mrcsd_last_error());
+    return 2;
+  }
+  save(d + "/count.bin", count.data(), nvox);
+  save(d + "/offset.bin", offset.data(), nvox);
+  save(d + "/fixels.bin", fixels.data(), 5 * size_t(total));
+  save(d + "/summary.bin", summary.data(), 5 * nvox);
+  std::printf("total %llu collectives %lld\n", (unsigned long long)total, emu::g_collectives);
+  mrfixel_destroy(ctx);
+  return 0;
+}

@@ -1,0 +1,131 @@
+"""The FOD segmentation kernel's own source (mrtrix3_amd/csrc/kernels/fmls.hip) compiled for the
+CPU against the wavefront emulation of tests/native/emu (each of the 64 lanes a fiber, every
+__syncthreads, ballot, readlane, readfirstlane and shuffle a collective all lanes must reach at the
+same call site) and run through the C ABI (include/mrfixel.h): every voxel's fixel count, offset,
+direction, integral and peak amplitude against the oracle (oracle/fmls_oracle.c), including
+voxels whose lobes overflow the LDS tables and go to the workspace pass, lobe merges, -dirpeak,
+-maxnum and the empty cases.  This checks the kernel's control flow, indexing and lane protocol
+without a GPU."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(ROOT, "mrtrix3_amd", "csrc")
+EMU = os.path.join(HERE, "native", "emu")
+
+
+@pytest.fixture(scope="module")
+def emu_exe(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    tmp_path = tmp_path_factory.mktemp("emu_fmls")
+    src = open(os.path.join(CSRC, "kernels", "fmls.hip")).read()
+    cut = src.index("// ---- HIP launches ----")
+    prep = tmp_path / "fmls_kernel_emu.hip"
+    prep.write_text(src[:cut] + "}  // namespace\n}  // namespace mrcsd\n")
+    exe = str(tmp_path / "fmls_emu")
+    host = [os.path.join(CSRC, f) for f in ("capi_fmls.cpp", "capi_common.cpp")] + \
+           [os.path.join(CSRC, "host", f) for f in ("fmls.cpp", "sh.cpp", "dense.cpp")]
+    cmd = ["g++", "-std=c++17", "-O1", "-w", "-I" + os.path.join(EMU, "include"), "-I" + CSRC,
+           "-I" + os.path.join(CSRC, "kernels"), "-I" + os.path.join(ROOT, "include"),
+           '-DKERNEL_SOURCE="%s"' % prep, "-x", "c++", os.path.join(EMU, "fmls_harness.cpp"),
+           os.path.join(EMU, "emu_runtime.cpp")] + host + ["-o", exe]
+    subprocess.run(cmd, check=True, timeout=600)
+    return exe
+
+
+def stick_fod(dirs, weights, lmax=8):
+    dirs = np.atleast_2d(np.asarray(dirs, dtype=np.float64))
+    dirs = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    az = np.arctan2(dirs[:, 1], dirs[:, 0])
+    el = np.arccos(np.clip(dirs[:, 2], -1, 1))
+    return np.asarray(weights, dtype=np.float64) @ orc.sh_transform(np.stack([az, el], axis=1), lmax)
+
+
+def fod_batch(seed=11):
+    """Single, crossing and three-fibre FODs with SH noise, plus the empty cases and the constant
+    FOD (rounding noise splits the sphere into well over 64 lobes)."""
+    rng = np.random.default_rng(seed)
+    fods, mask = [], []
+    for k in range(10):
+        nf = 1 + k % 3
+        d = rng.normal(size=(nf, 3))
+        w = rng.uniform(0.2, 0.6, nf)
+        f = stick_fod(d, w) + rng.normal(scale=0.02 * (1 + k % 4), size=45)
+        f[0] = abs(f[0]) + 0.05
+        fods.append(f)
+        mask.append(1)
+    const = np.zeros(45)
+    const[0] = 0.7
+    fods.append(const); mask.append(1)                      # many lobes: workspace pass
+    fods.append(stick_fod([[0, 0, 1]], [0.5])); mask.append(0)   # masked out
+    neg = -stick_fod([[1, 0, 0]], [0.5]); neg[0] = 0.1
+    fods.append(neg); mask.append(1)                        # largest |amplitude| negative
+    z = stick_fod([[0, 1, 0]], [0.5]); z[0] = 0.0
+    fods.append(z); mask.append(1)                          # c0 = 0
+    n = stick_fod([[0, 1, 0]], [0.5]); n[0] = np.nan
+    fods.append(n); mask.append(1)                          # c0 NaN
+    fods.append(stick_fod([[0.2, 0.1, 1]], [0.02])); mask.append(1)   # below the peak threshold
+    return np.array(fods, dtype=np.float32), np.array(mask, dtype=np.uint8)
+
+
+def expected(oracle, fods, mask, maxnum, dirpeak):
+    out = []
+    for v in range(fods.shape[0]):
+        if not mask[v]:
+            out.append(np.zeros((0, 5)))
+            continue
+        lobes, _ = oracle.segment(fods[v].astype(np.float64), max_out=2048)
+        if maxnum:
+            lobes = lobes[:maxnum]
+        d = lobes[:, 3:6] if dirpeak else lobes[:, 0:3]
+        out.append(np.concatenate([d, lobes[:, 6:8]], axis=1))
+    return out
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(maxnum=0, dirpeak=0, ithr=0.0, pthr=0.1, ratio=1.0),
+    dict(maxnum=2, dirpeak=1, ithr=0.0, pthr=0.1, ratio=0.3),
+    dict(maxnum=0, dirpeak=0, ithr=0.02, pthr=0.0, ratio=0.6),
+], ids=["defaults", "merges_dirpeak_maxnum", "integral_threshold"])
+def test_fmls_kernel_under_wavefront_emulation(tmp_path, emu_exe, cfg):
+    fods, mask = fod_batch()
+    nvox = fods.shape[0]
+    with open(tmp_path / "case.txt", "w") as f:
+        f.write("%d 45 %d %d\n%.17g %.17g %.17g\n" % (nvox, cfg["maxnum"], cfg["dirpeak"], cfg["ithr"], cfg["pthr"],
+                                                      cfg["ratio"]))
+    fods.tofile(tmp_path / "fod.bin")
+    mask.tofile(tmp_path / "mask.bin")
+    r = subprocess.run([emu_exe, str(tmp_path)], capture_output=True, text=True, timeout=1200)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "EMU:" not in r.stderr, r.stderr[-3000:]
+    assert "1 voxel(s) queued for the workspace pass" in r.stderr    # the constant FOD
+    count = np.fromfile(tmp_path / "count.bin", dtype=np.uint32)
+    offset = np.fromfile(tmp_path / "offset.bin", dtype=np.uint32)
+    fix = np.fromfile(tmp_path / "fixels.bin", dtype=np.float32).reshape(-1, 5)
+    summ = np.fromfile(tmp_path / "summary.bin", dtype=np.float32).reshape(nvox, 5)
+    o = orc.FmlsOracle(8, integral=cfg["ithr"], peak_value=cfg["pthr"], merge_ratio=cfg["ratio"])
+    exp = expected(o, fods, mask, cfg["maxnum"], cfg["dirpeak"])
+    assert list(count) == [len(e) for e in exp]
+    assert fix.shape[0] == int(count.sum())
+    run = 0
+    for v in range(nvox):
+        assert offset[v] == (run if count[v] else 0)
+        got = fix[run:run + count[v]]
+        want = exp[v].astype(np.float32)
+        np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-7, err_msg="voxel %d" % v)
+        run += count[v]
+        # the summary: peak amplitudes of fixels 0 and 1, direction of fixel 0
+        p = [want[i, 4] if i < len(want) else 0.0 for i in range(2)]
+        np.testing.assert_allclose(summ[v, :2], p, rtol=2e-6, atol=2e-7)
+        if len(want):
+            np.testing.assert_allclose(summ[v, 2:], want[0, :3], rtol=2e-6, atol=2e-7)
+        else:
+            assert np.all(np.isnan(summ[v, 2:]))
