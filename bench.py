@@ -516,7 +516,7 @@ def bench_msmt(args, torch, ph):
                  "mean_active": float(st[3]) / max(1, st[2]), "over_capacity": int(st[7])},
     }
     print("[bench] host-buffer leg", file=sys.stderr, flush=True)
-    host_b = np.ascontiguousarray(ph.dw)
+    host_b = np.ascontiguousarray(ph.dw, dtype=np.float32)
     med, runs, _ = median_rate(lambda: prob.solve(host_b), ph.nvox)
     out["host_buffers"] = {"icls_host_pointer_entry": {
         "value": med, "runs": runs,

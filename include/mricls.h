@@ -71,6 +71,15 @@ int mricls_solve_noalloc(mricls_ctx* ctx, const float* b, int ld_b, const uint8_
 int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                       float* x, uint16_t* iters, uint16_t* nactive);
 
+/* mricls_solve and mricls_solve_host with double-precision measurements and solutions, as
+ * Solver<double>::operator()(x, b) takes and returns them (:237-357; the solve itself is FP64 in
+ * both forms).  For right-hand sides that are not single-precision data, e.g. amp2response's
+ * accumulated normal equations (cmd/amp2response.cpp:357-361). */
+int mricls_solve_f64(mricls_ctx* ctx, const double* b, int ld_b, const uint8_t* valid, size_t nvox,
+                     double* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream);
+int mricls_solve_host_f64(mricls_ctx* ctx, const double* b, int ld_b, const uint8_t* valid, size_t nvox,
+                          double* x, uint16_t* iters, uint16_t* nactive);
+
 typedef struct mrmsmt_shared mrmsmt_shared;
 
 /* MSMT_CSD::Shared(header) + parse_cmdline_options() + set_responses() + init().

@@ -41,6 +41,7 @@ LIB_SOURCES = [
     "host/icls.cpp",
     "host/msmt.cpp",
     "host/fmls.cpp",
+    "host/voxel.cpp",
     "capi_common.cpp",
     "capi_csd.cpp",
     "capi_icls.cpp",
@@ -113,6 +114,17 @@ def build(verbose=False):
         if r.returncode != 0:
             raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
         os.replace(exe + ".tmp", exe)
+    # the test driver of the per-voxel classes (runs on the GPU box, which does not build)
+    tdir = os.path.join(REPO, "tests", "native")
+    tbin = os.path.join(tdir, "_bin")
+    os.makedirs(tbin, exist_ok=True)
+    exe = os.path.join(tbin, "voxel_classes")
+    cmd = [HIPCC] + HIPFLAGS + [os.path.join(tdir, "voxel_classes.cpp"), "-o", exe + ".tmp", "-L" + LIB,
+                                "-lmrcsd", "-Wl,-rpath,$ORIGIN/" + os.path.relpath(LIB, tbin), "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(exe + ".tmp", exe)
     if verbose:
         print("built", so)
     return so

@@ -12,6 +12,7 @@
 
 #include "../../include/mrcsd.h"
 #include "capi_common.h"
+#include "capi_csd_ctx.h"
 #include "host/csd.h"
 #include "host/gradient.h"
 #include "kernels/csd_solve.h"
@@ -22,18 +23,6 @@ using mrcsd::Matrix;
 
 struct mrcsd_shared {
   std::unique_ptr<CSD::Shared> shared;
-};
-
-struct mrcsd_ctx {
-  int device = 0;
-  mrcsd::CsdDeviceProblem P{};
-  mrcsd::DeviceBuffer<double> MtM, HR, HRt, M, rconvT;
-  mrcsd::DeviceBuffer<int> dwis;
-  mrcsd::DeviceBuffer<unsigned long long> counter;
-  int max_vol = 0;   // largest selected volume index (checked against ld_dw)
-  // The warm-start certificate (neg(F(S)) == S implies the plain answer) holds when the fixed point
-  // is the unique minimiser of the CSD objective: threshold 0 and a positive definite Mt_M.
-  bool warm_certifiable = false;
 };
 
 namespace {

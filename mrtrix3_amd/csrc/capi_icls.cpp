@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mricls.h"
@@ -23,9 +24,9 @@ struct mrmsmt_shared {
   std::unique_ptr<MR::DWI::SDeconv::MSMT_CSD::Shared> shared;
 };
 
-namespace {
+namespace mrcsd {
 
-int make_ctx(const Problem& pr, mricls_ctx** out) {
+int icls_ctx_from_problem(const Problem& pr, mricls_ctx** out) {
   const int n = int(pr.num_parameters()), nm = pr.b2d.rows, ncon = pr.B.rows;
   const int num_eq = int(pr.num_eq);
   if (n < 1 || n > mrcsd::kIclsMaxParams)
@@ -72,6 +73,83 @@ int make_ctx(const Problem& pr, mricls_ctx** out) {
   P.nmeas = nm; P.n = n; P.ncon = ncon; P.ncon_pad = ldg; P.num_eq = num_eq;
   P.max_niter = int(pr.max_niter);
   *out = ctx.release();
+  return MRCSD_OK;
+}
+
+int icls_solve_impl(mricls_ctx* ctx, const float* b, const double* b64, int ld_b, const uint8_t* valid,
+                    size_t nvox, float* x, double* x64, uint16_t* iters, uint16_t* nactive, uint64_t* stats,
+                    const IclsScratch* scratch, void* stream, const char* who) {
+  const std::string fn(who);
+  if (!ctx) return fail(MRCSD_ERR_ARG, fn + ": null context");
+  if (nvox == 0) return MRCSD_OK;
+  if ((!b && !b64) || (!x && !x64)) return fail(MRCSD_ERR_ARG, fn + ": null b or x");
+  if (ld_b < ctx->P.nmeas) return fail(MRCSD_ERR_ARG, fn + ": ld_b smaller than nmeas");
+  if (nvox > size_t(UINT32_MAX)) return fail(MRCSD_ERR_ARG, fn + ": more than 2^32 - 1 voxels");
+  MRCSD_HIP(hipSetDevice(ctx->device));
+  IclsScratch S{};
+  if (scratch) {
+    S = *scratch;
+  } else {
+    if (ctx->capacity < kIclsMaxActive && ctx->queue_cap < nvox) {
+      // queues of voxels passed on to larger capacities: grown (synchronously) to this call's size
+      MRCSD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+      ctx->queue.alloc(2 * nvox);
+      ctx->queue_cap = nvox;
+    }
+    S.counters = ctx->counters.get();
+    S.queue[0] = ctx->queue.get();
+    S.queue[1] = ctx->queue.get() ? ctx->queue.get() + ctx->queue_cap : nullptr;
+    S.capacity = ctx->queue_cap;
+  }
+  IclsLaunch L{};
+  L.b = b; L.b64 = b64; L.ld_b = ld_b; L.valid = valid; L.nvox = nvox; L.x = x; L.x64 = x64; L.iters = iters;
+  L.nactive = nactive; L.stats = reinterpret_cast<unsigned long long*>(stats);
+  MRCSD_HIP(icls_launch(ctx->P, L, S, ctx->capacity, static_cast<hipStream_t>(stream)));
+  return MRCSD_OK;
+}
+
+}  // namespace mrcsd
+
+namespace {
+
+int make_ctx(const Problem& pr, mricls_ctx** out) { return mrcsd::icls_ctx_from_problem(pr, out); }
+
+// mricls_solve_host and its double-precision twin: copies in, solves, copies out.
+template <class T>
+int solve_host(mricls_ctx* ctx, const T* b, int ld_b, const uint8_t* valid, size_t nvox, T* x, uint16_t* iters,
+               uint16_t* nactive, const char* who) {
+  const std::string fn(who);
+  if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, fn + ": null context");
+  if (nvox == 0) return MRCSD_OK;
+  if (!b || !x) return mrcsd::fail(MRCSD_ERR_ARG, fn + ": null b or x");
+  MRCSD_HIP(hipSetDevice(ctx->device));
+  const size_t n = size_t(ctx->P.n);
+  mrcsd::DeviceBuffer<T> d_b, d_x;
+  mrcsd::DeviceBuffer<uint8_t> d_v;
+  mrcsd::DeviceBuffer<uint16_t> d_it, d_na;
+  d_b.upload(b, nvox * size_t(ld_b));
+  d_x.alloc(nvox * n);
+  if (valid) d_v.upload(valid, nvox);
+  d_it.alloc(nvox);
+  if (nactive) d_na.alloc(nvox);
+  constexpr bool f64 = std::is_same<T, double>::value;
+  int rc = mrcsd::icls_solve_impl(ctx, f64 ? nullptr : reinterpret_cast<const float*>(d_b.get()),
+                                  f64 ? reinterpret_cast<const double*>(d_b.get()) : nullptr, ld_b,
+                                  valid ? d_v.get() : nullptr, nvox,
+                                  f64 ? nullptr : reinterpret_cast<float*>(d_x.get()),
+                                  f64 ? reinterpret_cast<double*>(d_x.get()) : nullptr, d_it.get(),
+                                  nactive ? d_na.get() : nullptr, nullptr, nullptr, nullptr, who);
+  if (rc) return rc;
+  MRCSD_HIP(hipDeviceSynchronize());
+  std::vector<uint16_t> it(nvox);
+  d_it.download(it.data(), nvox);
+  d_x.download(x, nvox * n);
+  if (iters) std::memcpy(iters, it.data(), nvox * sizeof(uint16_t));
+  if (nactive) d_na.download(nactive, nvox);
+  const size_t over = size_t(std::count(it.begin(), it.end(), uint16_t(0xFFFE)));
+  if (over)
+    return mrcsd::fail(MRCSD_ERR_SETUP, fn + ": " + std::to_string(over) +
+                                            " voxel(s) need more than 128 active constraints (iters 0xFFFE)");
   return MRCSD_OK;
 }
 
@@ -129,61 +207,30 @@ int mricls_get_capacity(const mricls_ctx* ctx) { return ctx ? ctx->capacity : 0;
 int mricls_solve(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                  float* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream) {
   return mrcsd::guarded([&]() -> int {
-    if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: null context");
-    if (nvox == 0) return MRCSD_OK;
-    if (!b || !x) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: null b or x");
-    if (ld_b < ctx->P.nmeas) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: ld_b smaller than nmeas");
-    MRCSD_HIP(hipSetDevice(ctx->device));
-    if (nvox > size_t(UINT32_MAX)) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve: more than 2^32 - 1 voxels");
-    if (ctx->capacity < mrcsd::kIclsMaxActive && ctx->queue_cap < nvox) {
-      // queues of voxels passed on to larger capacities: grown (synchronously) to this call's size
-      MRCSD_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-      ctx->queue.alloc(2 * nvox);
-      ctx->queue_cap = nvox;
-    }
-    mrcsd::IclsLaunch L{};
-    L.b = b; L.ld_b = ld_b; L.valid = valid; L.nvox = nvox; L.x = x; L.iters = iters;
-    L.nactive = nactive; L.stats = reinterpret_cast<unsigned long long*>(stats);
-    mrcsd::IclsScratch S{};
-    S.counters = ctx->counters.get();
-    S.queue[0] = ctx->queue.get();
-    S.queue[1] = ctx->queue.get() ? ctx->queue.get() + ctx->queue_cap : nullptr;
-    S.capacity = ctx->queue_cap;
-    MRCSD_HIP(mrcsd::icls_launch(ctx->P, L, S, ctx->capacity, static_cast<hipStream_t>(stream)));
-    return MRCSD_OK;
+    return mrcsd::icls_solve_impl(ctx, b, nullptr, ld_b, valid, nvox, x, nullptr, iters, nactive, stats, nullptr,
+                                  stream, "mricls_solve");
+  });
+}
+
+int mricls_solve_f64(mricls_ctx* ctx, const double* b, int ld_b, const uint8_t* valid, size_t nvox,
+                     double* x, uint16_t* iters, uint16_t* nactive, uint64_t* stats, void* stream) {
+  return mrcsd::guarded([&]() -> int {
+    return mrcsd::icls_solve_impl(ctx, nullptr, b, ld_b, valid, nvox, nullptr, x, iters, nactive, stats, nullptr,
+                                  stream, "mricls_solve_f64");
   });
 }
 
 int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                       float* x, uint16_t* iters, uint16_t* nactive) {
   return mrcsd::guarded([&]() -> int {
-    if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve_host: null context");
-    if (nvox == 0) return MRCSD_OK;
-    if (!b || !x) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_solve_host: null b or x");
-    MRCSD_HIP(hipSetDevice(ctx->device));
-    const size_t n = size_t(ctx->P.n);
-    mrcsd::DeviceBuffer<float> d_b, d_x;
-    mrcsd::DeviceBuffer<uint8_t> d_v;
-    mrcsd::DeviceBuffer<uint16_t> d_it, d_na;
-    d_b.upload(b, nvox * size_t(ld_b));
-    d_x.alloc(nvox * n);
-    if (valid) d_v.upload(valid, nvox);
-    d_it.alloc(nvox);
-    if (nactive) d_na.alloc(nvox);
-    int rc = mricls_solve(ctx, d_b.get(), ld_b, valid ? d_v.get() : nullptr, nvox, d_x.get(), d_it.get(),
-                          nactive ? d_na.get() : nullptr, nullptr, nullptr);
-    if (rc) return rc;
-    MRCSD_HIP(hipDeviceSynchronize());
-    std::vector<uint16_t> it(nvox);
-    d_it.download(it.data(), nvox);
-    d_x.download(x, nvox * n);
-    if (iters) std::memcpy(iters, it.data(), nvox * sizeof(uint16_t));
-    if (nactive) d_na.download(nactive, nvox);
-    const size_t over = size_t(std::count(it.begin(), it.end(), uint16_t(0xFFFE)));
-    if (over)
-      return mrcsd::fail(MRCSD_ERR_SETUP, "mricls_solve_host: " + std::to_string(over) +
-                                              " voxel(s) need more than 128 active constraints (iters 0xFFFE)");
-    return MRCSD_OK;
+    return solve_host<float>(ctx, b, ld_b, valid, nvox, x, iters, nactive, "mricls_solve_host");
+  });
+}
+
+int mricls_solve_host_f64(mricls_ctx* ctx, const double* b, int ld_b, const uint8_t* valid, size_t nvox,
+                          double* x, uint16_t* iters, uint16_t* nactive) {
+  return mrcsd::guarded([&]() -> int {
+    return solve_host<double>(ctx, b, ld_b, valid, nvox, x, iters, nactive, "mricls_solve_host_f64");
   });
 }
 

@@ -6,6 +6,8 @@
 #include "capi_common.h"
 #include "kernels/icls_solve.h"
 
+namespace MR { namespace Math { namespace ICLS { class Problem; } } }
+
 struct mricls_ctx {
   int device = 0;
   mrcsd::IclsDeviceProblem P{};
@@ -17,3 +19,13 @@ struct mricls_ctx {
   size_t queue_cap = 0;
   int capacity = 0;   // active-set slots of the first pass
 };
+
+namespace mrcsd {
+// A solver context for a problem set up on the host (mricls_create's second half); status code as
+// the C ABI returns it.
+int icls_ctx_from_problem(const MR::Math::ICLS::Problem& pr, mricls_ctx** out);
+// mricls_solve / mricls_solve_f64 with the caller's scratch (per-thread callers) and input type.
+int icls_solve_impl(mricls_ctx* ctx, const float* b, const double* b64, int ld_b, const uint8_t* valid,
+                    size_t nvox, float* x, double* x64, uint16_t* iters, uint16_t* nactive, uint64_t* stats,
+                    const IclsScratch* scratch, void* stream, const char* who);
+}  // namespace mrcsd

@@ -154,13 +154,11 @@ int mrresp_solve(int lmax, const double* M, const double* b, int constrained, do
     mricls_ctx* icls = nullptr;
     int rc = mricls_create(M, n, n, A.data(), 2 * kAngles, nullptr, 0, 1e-10, 1e-10, 0, 0.0, 1, &icls);
     if (rc) return rc;
-    std::vector<float> bf(static_cast<size_t>(n)), x(static_cast<size_t>(n));
-    for (int r = 0; r < n; ++r) bf[size_t(r)] = float(b[r]);
+    // b and the response in double, as Problem<default_type> takes and returns them (:357-361)
     uint16_t it = 0, na = 0;
-    rc = mricls_solve_host(icls, bf.data(), n, nullptr, 1, x.data(), &it, &na);
+    rc = mricls_solve_host_f64(icls, b, n, nullptr, 1, rf, &it, &na);
     mricls_destroy(icls);
     if (rc) return rc;
-    for (int r = 0; r < n; ++r) rf[r] = double(x[size_t(r)]);
     if (iterations) *iterations = int(it);
     return MRCSD_OK;
   });

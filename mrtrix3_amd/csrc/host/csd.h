@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,49 @@ class CSD {
     uint32_t lmax_response, lmax_data, lmax_cmdline, lmax;
     size_t niter;
   };
+
+  // The per-voxel solver (csd.h:214-271) with the reference's call surface: set(), iterate()
+  // until it returns true (at most shared.niter calls), FOD().  The fit itself runs on the GPU:
+  // the first iterate() after set() solves the voxel there (one voxel per launch, up to niter
+  // iterations) and the following calls replay its outcome, so that iterate() call c returns
+  // true exactly when the reference's would (its negative set unchanged at call c) and FOD()
+  // after call c is the reference's F after min(c, n) solves (recomputed on the GPU if asked for
+  // mid-way).  Signals are taken as dwi2fod's load_data hands them over (dwi2fod.cpp:156-173):
+  // single precision, negatives as 0, a non-finite value giving a zero FOD.  Copies (one per
+  // thread, as ThreadedLoop makes them) share the device copy of the problem and own their device
+  // buffers and HIP stream.  Batches of voxels belong to mrcsd_run (include/mrcsd.h), which has
+  // the throughput; this class is the drop-in for code written against CSD.
+  explicit CSD(const Shared& shared_data);
+  CSD(const CSD& other);
+  CSD& operator=(const CSD&) = delete;
+  ~CSD();
+
+  void set(const std::vector<double>& DW_signals);                // csd.h:236-243
+  template <class V>
+  void set(const V& DW_signals) {
+    std::vector<double> b(size_t(DW_signals.size()));
+    for (size_t i = 0; i < b.size(); ++i) b[i] = double(DW_signals[i]);
+    set(b);
+  }
+  bool iterate();                                                 // csd.h:245-269
+  const std::vector<double>& FOD() const;                         // csd.h:271
+
+  const Shared& shared;
+
+  struct Engine;     // device copy of the problem (shared by copies)
+  struct Voxel;      // device buffers and stream of one object
+
+ private:
+  size_t solve(size_t budget) const;
+
+  std::shared_ptr<Engine> engine_;
+  mutable std::unique_ptr<Voxel> dev_;
+  std::vector<float> b_;                // signals of the current voxel
+  size_t calls_ = 0;                    // iterate() calls since set()
+  mutable std::vector<double> F_;
+  mutable size_t solved_budget_ = 0;    // niter of the cached GPU result (0: none)
+  mutable size_t solved_n_ = 0;         // its loop counter (< solved_budget_: converged there)
+  mutable size_t F_state_ = 0;          // solves that F_ reflects
 };
 
 // Plain-text numeric matrix reader with the rules of core/math/math.h:146-336: '#' starts a

@@ -3,6 +3,7 @@
 // (Solver::operator(), :237-357) runs on the GPU, see kernels/icls_solve.hip.
 #pragma once
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,33 @@ class Problem {
   std::vector<double> t;
   double lambda_min_norm = 0.0, tol = 0.0;
   size_t max_niter = 0, num_eq = 0;
+};
+
+// Solver::operator()(x, b) (:225-359) for one right-hand side, run on the GPU in double
+// precision (one voxel per launch; batches belong to mricls_solve in include/mricls.h).  Returns
+// the reference's niter and leaves the final active set size in active_constraints().  The
+// device copy of the problem is made at construction and shared by copies; each object owns its
+// device buffers and HIP stream.  A right-hand side whose active set would exceed 128
+// constraints throws (the reference has no such limit).
+class Solver {
+ public:
+  explicit Solver(const Problem& problem);
+  Solver(const Solver& other);
+  Solver& operator=(const Solver&) = delete;
+  ~Solver();
+
+  size_t operator()(std::vector<double>& x, const std::vector<double>& b);
+  const Problem& problem() const { return P; }
+  size_t active_constraints() const { return nactive_; }
+
+  struct Engine;
+  struct Voxel;
+
+ private:
+  const Problem& P;
+  std::shared_ptr<Engine> engine_;
+  std::unique_ptr<Voxel> dev_;
+  size_t nactive_ = 0;
 };
 
 }  // namespace ICLS

@@ -44,6 +44,16 @@ class MSMT_CSD {
    private:
     void prepare_responses();                                     // msmt_csd.h:237-255
   };
+
+  // msmt_csd.h:262-275: the per-voxel solve over all volumes of a voxel (ICLS::Solver on the GPU).
+  explicit MSMT_CSD(const Shared& shared_data) : niter(0), shared(shared_data), solver(shared_data.problem) {}
+  void operator()(const std::vector<double>& data, std::vector<double>& output) { niter = solver(output, data); }
+
+  size_t niter;
+  const Shared& shared;
+
+ private:
+  Math::ICLS::Solver solver;
 };
 
 }  // namespace SDeconv

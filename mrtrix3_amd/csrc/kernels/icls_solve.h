@@ -29,6 +29,8 @@ struct IclsLaunch {
   uint16_t* iters;         // [nvox] or null (0xFFFF skipped, 0xFFFE active set beyond the largest capacity)
   uint16_t* nactive;       // [nvox] or null
   unsigned long long* stats;    // [8] or null (see mricls.h)
+  const double* b64;       // when set: double measurements in place of b, same layout
+  double* x64;             // when set: double solutions in place of x
 };
 
 // Scratch of one solve: a work counter per pass and two queues of voxel indices that carry the

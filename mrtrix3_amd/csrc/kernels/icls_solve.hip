@@ -106,7 +106,9 @@ __device__ __forceinline__ double rsqrt_refined(double h) {
   return y;
 }
 
-template <int KCAP, int KQ>
+// IO64: measurements read from L.b64 and solutions written to L.x64 in double (single precision
+// otherwise, the dwi2fod path).
+template <int KCAP, int KQ, bool IO64>
 __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P, IclsLaunch L, IclsPass Q) {
   constexpr int SPL = (KCAP + 63) / 64;           // slots per lane
   __shared__ double s_W[KCAP * (KCAP + 1) / 2];   // inverse factor W = L^-1, packed lower rows
@@ -140,7 +142,10 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
       if (idx >= L.nvox) break;
       vox = (size_t)idx;
       if (L.valid && !L.valid[vox]) {
-        for (int j = lane; j < n; j += 64) L.x[vox * n + j] = 0.0f;
+        for (int j = lane; j < n; j += 64) {
+          if constexpr (IO64) L.x64[vox * n + j] = 0.0;
+          else L.x[vox * n + j] = 0.0f;
+        }
         if (lane == 0) {
           if (L.iters) L.iters[vox] = 0xFFFF;
           if (L.nactive) L.nactive[vox] = 0;
@@ -148,7 +153,10 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
         continue;
       }
     }
-    const float* bsrc = L.b + vox * (size_t)L.ld_b;
+    const auto* bsrc = [&]() {
+      if constexpr (IO64) return L.b64 + vox * (size_t)L.ld_b;
+      else return L.b + vox * (size_t)L.ld_b;
+    }();
 
     // ---- unconstrained solution y_u = b2d^T b (:244), parameter j on lane j % 64 ----
     double yu[2];
@@ -523,7 +531,10 @@ __global__ __launch_bounds__(64) void icls_active_set_kernel(IclsDeviceProblem P
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int j = lane + 64 * h;
-      if (j < n) L.x[vox * n + j] = (float)y[h];
+      if (j < n) {
+        if constexpr (IO64) L.x64[vox * n + j] = y[h];
+        else L.x[vox * n + j] = (float)y[h];
+      }
     }
     if (lane == 0) {
       if (L.iters) L.iters[vox] = overflow ? (uint16_t)0xFFFE : (uint16_t)niter;
@@ -597,7 +608,7 @@ __global__ void reset_counters_kernel(unsigned long long* c, int n) {
 template <int KCAP, int KQ>
 hipError_t launch_pass(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q, size_t max_grid,
                        hipStream_t stream) {
-  auto kern = icls_active_set_kernel<KCAP, KQ>;
+  auto kern = L.b64 ? icls_active_set_kernel<KCAP, KQ, true> : icls_active_set_kernel<KCAP, KQ, false>;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0);
   if (e != hipSuccess) return e;

@@ -29,6 +29,8 @@ def _register():
     _lib.register("mricls_dims", C.c_int, [_P, _I, _I, _I, _I])
     _lib.register("mricls_solve", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P, _P, _P])
     _lib.register("mricls_solve_host", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P])
+    _lib.register("mricls_solve_f64", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P, _P, _P])
+    _lib.register("mricls_solve_host_f64", C.c_int, [_P, _P, C.c_int, _P, C.c_size_t, _P, _P, _P])
     _lib.register("mricls_set_capacity", C.c_int, [_P, C.c_int])
     _lib.register("mricls_get_capacity", C.c_int, [_P])
     _lib.register("mrmsmt_shared_create", C.c_int, [_D, C.c_int, _D, _I, _I, C.c_int, _I, C.c_int, _D,
@@ -76,29 +78,36 @@ class IclsProblem:
         check(_lib.load().mricls_set_capacity(self._h, int(slots)))
 
     def solve(self, b, valid=None, stats=None, x=None):
-        """b: [nvox, >= nmeas] float32 (torch CUDA tensor or numpy).  Returns (x, iters, nactive)."""
+        """b: [nvox, >= nmeas] float32, or float64 for double-precision measurements and solutions
+        (torch CUDA tensor or numpy).  Returns (x, iters, nactive), x in b's precision."""
         lib = _lib.load()
         if _is_torch(b):
             import torch
-            assert b.is_cuda and b.dtype == torch.float32 and b.is_contiguous()
+            assert b.is_cuda and b.dtype in (torch.float32, torch.float64) and b.is_contiguous()
+            f64 = b.dtype == torch.float64
             nvox, ld = b.shape
             if x is None:
-                x = torch.empty((nvox, self.n), dtype=torch.float32, device=b.device)
+                x = torch.empty((nvox, self.n), dtype=b.dtype, device=b.device)
+            assert x.dtype == b.dtype
             it = torch.empty(nvox, dtype=torch.int16, device=b.device)
             na = torch.empty(nvox, dtype=torch.int16, device=b.device)
             stream = torch.cuda.current_stream(b.device).cuda_stream
-            check(lib.mricls_solve(self._h, b.data_ptr(), ld, valid.data_ptr() if valid is not None else None,
-                                   nvox, x.data_ptr(), it.data_ptr(), na.data_ptr(),
-                                   stats.data_ptr() if stats is not None else None, stream))
+            fn = lib.mricls_solve_f64 if f64 else lib.mricls_solve
+            check(fn(self._h, b.data_ptr(), ld, valid.data_ptr() if valid is not None else None, nvox,
+                     x.data_ptr(), it.data_ptr(), na.data_ptr(), stats.data_ptr() if stats is not None else None,
+                     stream))
             return x, it, na
-        b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.float32)
+        f64 = np.asarray(b).dtype == np.float64
+        dt = np.float64 if f64 else np.float32
+        b = np.ascontiguousarray(np.atleast_2d(b), dtype=dt)
         nvox, ld = b.shape
-        out = np.empty((nvox, self.n), dtype=np.float32)
+        out = np.empty((nvox, self.n), dtype=dt)
         it = np.empty(nvox, dtype=np.uint16)
         na = np.empty(nvox, dtype=np.uint16)
         v = None if valid is None else np.ascontiguousarray(valid, dtype=np.uint8)
-        check(lib.mricls_solve_host(self._h, b.ctypes.data, ld, v.ctypes.data if v is not None else None, nvox,
-                                    out.ctypes.data, it.ctypes.data, na.ctypes.data))
+        fn = lib.mricls_solve_host_f64 if f64 else lib.mricls_solve_host
+        check(fn(self._h, b.ctypes.data, ld, v.ctypes.data if v is not None else None, nvox, out.ctypes.data,
+                 it.ctypes.data, na.ctypes.data))
         return out, it, na
 
     def __del__(self):

@@ -16,7 +16,7 @@ template <int KCAP, int KQ>
 void run_pass(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q) {
   const unsigned grid = 2;
   for (unsigned b = 0; b < grid; ++b)
-    emu::run_workgroup([&]() { icls_active_set_kernel<KCAP, KQ>(P, L, Q); }, b, grid);
+    emu::run_workgroup([&]() { icls_active_set_kernel<KCAP, KQ, false>(P, L, Q); }, b, grid);
 }
 template <int KCAP>
 void run_kq(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsPass& Q) {

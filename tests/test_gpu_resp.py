@@ -1,6 +1,8 @@
 """amp2response on the GPU (include/mrresp.h) against the float64 oracle (oracle/resp_oracle.py):
 the accumulated normal equations (cmd/amp2response.cpp:152-191) to 1e-12 and the per-shell
-responses (:336-361) to 1e-5 (the ICLS kernel takes and returns single precision)."""
+responses (:336-361) to 1e-7 (the constrained solve takes b and returns the response in double, as
+Problem<default_type> does; its active constraints -- amplitudes at elevations a degree apart --
+are nearly dependent, so two FP64 active-set solvers that round differently agree to ~2e-8)."""
 import sys
 
 import numpy as np
@@ -81,7 +83,7 @@ def test_solve_matches_oracle(torch_cuda, lmax, constrained):
     rf, it = resp.solve(lmax, M, b, constrained)
     ro, ito = R.solve(lmax, M, b, constrained)
     say("[gpu resp] rf rel %.2e, iterations %d (oracle %d)" % (rel(rf, ro), it, ito))
-    assert rel(rf, ro) <= 1e-5
+    assert rel(rf, ro) <= 1e-7
     assert it == ito
     if constrained and lmax:
         assert it > 0
@@ -116,4 +118,4 @@ def test_brain_single_fibre_voxels(torch_cuda):
     rf, it = resp.solve(10, M, b, True)
     ro, ito = R.solve(10, M, b, True)
     say("[gpu resp] brain response", np.round(rf, 3).tolist(), "rel %.2e" % rel(rf, ro))
-    assert rel(rf, ro) <= 1e-5 and it == ito
+    assert rel(rf, ro) <= 1e-7 and it == ito
