@@ -68,7 +68,10 @@ int mrpeaks_npeaks(const mrpeaks_ctx* ctx);
  * the stream they are given; concurrent calls on one context from different streams race. */
 int mrpeaks_run(mrpeaks_ctx* ctx, const float* sh, int ld_sh, const uint8_t* mask, const float* peaks_in,
                 int ld_peaks_in, size_t nvox, float* out, uint64_t* stats, void* stream);
-/* The same with host pointers (synchronous). */
+/* The same with host pointers (synchronous).  Fails with MRCSD_ERR_SETUP (outputs still written)
+ * when a voxel has more than 64 distinct accepted peaks: the selection then sees the first 64 in
+ * seed order, where the reference chooses among all of them.  The built-in 60 search directions
+ * cannot reach this; -seeds with more than 64 directions can. */
 int mrpeaks_run_host(mrpeaks_ctx* ctx, const float* sh, int ld_sh, const uint8_t* mask, const float* peaks_in,
                      int ld_peaks_in, size_t nvox, float* out);
 

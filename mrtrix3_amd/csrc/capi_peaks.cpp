@@ -182,11 +182,20 @@ int mrpeaks_run_host(mrpeaks_ctx* ctx, const float* sh, int ld_sh, const uint8_t
     d_out.alloc(nvox * size_t(3 * ctx->npeaks));
     if (mask) d_mask.upload(mask, nvox);
     if (peaks_in) d_pk.upload(peaks_in, nvox * size_t(ld_peaks_in));
+    mrcsd::DeviceBuffer<unsigned long long> d_stats;
+    const unsigned long long zero[4] = {0, 0, 0, 0};
+    d_stats.upload(zero, 4);
     int rc = mrpeaks_run(ctx, d_sh.get(), ld_sh, mask ? d_mask.get() : nullptr, peaks_in ? d_pk.get() : nullptr,
-                         ld_peaks_in, nvox, d_out.get(), nullptr, nullptr);
+                         ld_peaks_in, nvox, d_out.get(), reinterpret_cast<uint64_t*>(d_stats.get()), nullptr);
     if (rc) return rc;
     MRCSD_HIP(hipDeviceSynchronize());
     d_out.download(out, nvox * size_t(3 * ctx->npeaks));
+    unsigned long long st[4];
+    d_stats.download(st, 4);
+    if (st[3])
+      return mrcsd::fail(MRCSD_ERR_SETUP, "mrpeaks_run_host: " + std::to_string(st[3]) +
+                                              " voxel(s) have more than 64 distinct peaks; the reported peaks are "
+                                              "chosen among the first 64 (in seed order) only");
     return MRCSD_OK;
   });
 }

@@ -115,12 +115,16 @@ template <int LMAX>
 __device__ __forceinline__ void fast_al(float (&AL)[nforl_mpos(LMAX)], const float* __restrict__ tab, float el) {
   const float inc = (float)(kPi / (kPeaksFastRows - 1));
   float f2 = el / inc, f1;
-  int i = (int)f2;
-  if (i < 0) {
+  int i;
+  // the reference's int(f2) < 0 and >= 511 tests, decided on the float before it is converted (a
+  // NaN elevation, from an SH series with infinite coefficients, takes the first row as x86's
+  // conversion of NaN to INT_MIN makes the reference's host code do)
+  if (!(f2 > -1.0f)) {
     i = 0; f1 = 1.0f; f2 = 0.0f;
-  } else if (i >= kPeaksFastRows - 1) {
+  } else if (f2 >= (float)(kPeaksFastRows - 1)) {
     i = kPeaksFastRows - 1; f1 = 1.0f; f2 = 0.0f;
   } else {
+    i = (int)f2;
     f2 -= (float)i;
     f1 = (float)(1.0 - (double)f2);
   }

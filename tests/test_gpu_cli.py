@@ -157,6 +157,18 @@ def test_cli_sh2peaks_matches_oracle(tmp_path):
     assert bad2 <= 0.001 * n + 1
     assert np.all(np.isnan(g2[mask.reshape(-1) == 0]))
     assert np.all(np.isnan(g[mask.reshape(-1) == 0]))
+    # -fast: the Legendre lookup table (SH.h:402-470)
+    out3 = str(tmp_path / "peaks3.mif")
+    r = subprocess.run([exe, shp, out3, "-num", "2", "-mask", msk, "-fast", "-force"], capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    got3, _ = mif.read(out3)
+    g3 = got3.transpose(2, 1, 0, 3).reshape(n, 6)
+    ref3, _ = orc.sh2peaks(fod, npeaks=2, mask=mask.reshape(-1).astype(np.uint8), nthreads=8, fast=True)
+    bad3 = count_peak_mismatches(g3, ref3, fod)
+    say("[gpu cli] sh2peaks -fast: %d of %d voxels differ" % (bad3, n))
+    assert bad3 <= 0.001 * n + 1
+    assert not np.array_equal(g3, g, equal_nan=True)
 
 
 def test_cli_fslgrad_matches_dw_scheme(tmp_path):

@@ -101,6 +101,38 @@ def test_other_lmax(torch_cuda, lmax):
     assert frac >= 0.99
 
 
+@pytest.mark.parametrize("lmax", [10, 12])
+def test_fast_table_other_lmax_and_infinite_coefficients(torch_cuda, lmax):
+    """-fast at lmax 10 and 12 against the oracle's -fast run, with voxels whose series holds an
+    infinite coefficient: their search directions become NaN, whose elevation takes the table's
+    first row as the reference's conversion does on x86, and the outputs agree with the oracle's."""
+    torch = torch_cuda
+    rng = np.random.default_rng(40 + lmax)
+    nsh = (lmax + 1) * (lmax + 2) // 2
+    dirs = rng.standard_normal((400, 3, 3))
+    dirs /= np.linalg.norm(dirs, axis=2, keepdims=True)
+    ang = np.stack([np.arctan2(dirs[..., 1], dirs[..., 0]), np.arccos(np.clip(dirs[..., 2], -1, 1))], axis=-1)
+    Y = orc.sh_transform(ang.reshape(-1, 2), lmax).reshape(400, 3, nsh)
+    w = rng.uniform(0.3, 1.0, (400, 3, 1))
+    sh = ((w * Y).sum(axis=1) + 0.01 * rng.standard_normal((400, nsh))).astype(np.float32)
+    sh[7, 5] = np.inf
+    sh[8, 0] = -np.inf
+    sh[9, nsh - 1] = np.inf
+    say("[gpu peaks] -fast lmax %d, 400 voxels (3 with an infinite coefficient)" % lmax)
+    out = peaks.SH2Peaks(nsh, fast=True).run(torch.from_numpy(sh).cuda())
+    torch.cuda.synchronize()
+    out = out.cpu().numpy()
+    ref, _ = orc.sh2peaks(sh, lmax=lmax, nthreads=8, fast=True)
+    keep = np.ones(400, dtype=bool)
+    keep[[7, 8, 9]] = False
+    frac, worst = compare(out[keep], ref[keep], 3, sh[keep])
+    say("[gpu peaks] -fast lmax %d agree %.5f worst angle %.2e" % (lmax, frac, worst))
+    assert frac >= 0.99
+    for v in (7, 8, 9):
+        assert np.array_equal(np.isnan(out[v]), np.isnan(ref[v])), v
+        assert np.allclose(out[v], ref[v], equal_nan=True), v
+
+
 def test_options_and_input_rules(torch_cuda, fods):
     torch = torch_cuda
     sub = fods[:512].copy()
