@@ -71,6 +71,12 @@ int mricls_solve_noalloc(mricls_ctx* ctx, const float* b, int ld_b, const uint8_
 int mricls_solve_host(mricls_ctx* ctx, const float* b, int ld_b, const uint8_t* valid, size_t nvox,
                       float* x, uint16_t* iters, uint16_t* nactive);
 
+/* The predicted signal of each voxel's solution, out[v] = H x[v] for the problem matrix H
+ * (MSMT_Processor's -predicted_signal, cmd/dwi2fod.cpp:216-220), FP64 sums rounded to float.
+ * Device pointers x [nvox][ld_x] (the first n values of a row) and out [nvox][ld_out]; asynchronous
+ * on `stream`.  Not available for a problem given in standard form (MRCSD_ERR_ARG). */
+int mricls_predict(mricls_ctx* ctx, const float* x, int ld_x, size_t nvox, float* out, int ld_out, void* stream);
+
 /* mricls_solve and mricls_solve_host with double-precision measurements and solutions, as
  * Solver<double>::operator()(x, b) takes and returns them (:237-357; the solve itself is FP64 in
  * both forms).  For right-hand sides that are not single-precision data, e.g. amp2response's

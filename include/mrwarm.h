@@ -29,6 +29,13 @@ typedef struct mrwarm_net mrwarm_net;
 
 int mrwarm_create(int n_in, int hidden, int nsh, int nlog, uint64_t seed, mrwarm_net** out);
 void mrwarm_destroy(mrwarm_net* net);
+int mrwarm_dims(const mrwarm_net* net, int* n_in, int* hidden, int* nsh, int* nlog);
+/* A network saved by the Python mirror (WarmNet.save: an .npz of `dims` = [n_in, hidden, nsh, nlog]
+ * as integers and `params` = the float32 parameters in mrwarm_get_params order), read without
+ * executing anything from the file. */
+int mrwarm_load(const char* path, mrwarm_net** out);
+/* The layer sizes and parameter count such a file holds (no GPU needed). */
+int mrwarm_file_dims(const char* path, int* n_in, int* hidden, int* nsh, int* nlog, size_t* n_params);
 /* Number of parameters and their order: for each layer, W [fan_in][fan_out] then b [fan_out]. */
 size_t mrwarm_num_params(const mrwarm_net* net);
 int mrwarm_get_params(const mrwarm_net* net, float* host, size_t n);

@@ -42,6 +42,7 @@ LIB_SOURCES = [
     "host/msmt.cpp",
     "host/fmls.cpp",
     "host/voxel.cpp",
+    "host/npz.cpp",
     "capi_common.cpp",
     "capi_csd.cpp",
     "capi_icls.cpp",
@@ -98,7 +99,7 @@ def build(verbose=False):
         objs = list(ex.map(_compile, LIB_SOURCES))
     so = os.path.join(LIB, "libmrcsd.so")
     tmp = so + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))

@@ -63,6 +63,13 @@ int icls_ctx_from_problem(const Problem& pr, mricls_ctx** out) {
   ctx->t.upload(t.data(), t.size());
   ctx->G.upload(G.data(), G.size());
   ctx->L.upload(Lf.data(), Lf.size());
+  ctx->standard_form = pr.standard_form;
+  if (!pr.standard_form) {
+    std::vector<double> Ht(size_t(n) * nm);
+    for (int r = 0; r < nm; ++r)
+      for (int j = 0; j < n; ++j) Ht[size_t(j) * nm + r] = pr.H(r, j);
+    ctx->Ht.upload(Ht.data(), Ht.size());
+  }
   ctx->counters.alloc(2 * mrcsd::kIclsMaxPasses);
   ctx->capacity = mrcsd::icls_default_capacity(n, num_eq);
   mrcsd::IclsDeviceProblem& P = ctx->P;
@@ -217,6 +224,22 @@ int mricls_solve_f64(mricls_ctx* ctx, const double* b, int ld_b, const uint8_t* 
   return mrcsd::guarded([&]() -> int {
     return mrcsd::icls_solve_impl(ctx, nullptr, b, ld_b, valid, nvox, nullptr, x, iters, nactive, stats, nullptr,
                                   stream, "mricls_solve_f64");
+  });
+}
+
+int mricls_predict(mricls_ctx* ctx, const float* x, int ld_x, size_t nvox, float* out, int ld_out, void* stream) {
+  return mrcsd::guarded([&]() -> int {
+    if (!ctx) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_predict: null context");
+    if (nvox == 0) return MRCSD_OK;
+    if (!x || !out) return mrcsd::fail(MRCSD_ERR_ARG, "mricls_predict: null x or out");
+    if (ctx->standard_form)
+      return mrcsd::fail(MRCSD_ERR_ARG, "mricls_predict: the problem was given in standard form (no problem matrix)");
+    if (ld_x < ctx->P.n || ld_out < ctx->P.nmeas)
+      return mrcsd::fail(MRCSD_ERR_ARG, "mricls_predict: ld_x below n or ld_out below nmeas");
+    MRCSD_HIP(hipSetDevice(ctx->device));
+    MRCSD_HIP(mrcsd::icls_predict(ctx->Ht.get(), ctx->P.n, ctx->P.nmeas, x, ld_x, nvox, out, ld_out,
+                                  static_cast<hipStream_t>(stream)));
+    return MRCSD_OK;
   });
 }
 

@@ -12,6 +12,8 @@ struct mricls_ctx {
   int device = 0;
   mrcsd::IclsDeviceProblem P{};
   mrcsd::DeviceBuffer<double> b2d, Bt, t, G, L;
+  mrcsd::DeviceBuffer<double> Ht;   // [n][nmeas] problem matrix, transposed (predicted signals)
+  bool standard_form = false;
   // scratch of a solve: work counters and queue lengths of the passes, and the two queues of
   // voxels re-solved with more active-set slots (grown to the largest voxel count seen)
   mrcsd::DeviceBuffer<unsigned long long> counters;

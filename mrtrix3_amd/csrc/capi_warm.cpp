@@ -11,6 +11,7 @@
 
 #include "../../include/mrwarm.h"
 #include "capi_common.h"
+#include "host/npz.h"
 #include "kernels/mlp.h"
 
 namespace {
@@ -152,6 +153,63 @@ int mrwarm_create(int n_in, int hidden, int nsh, int nlog, uint64_t seed, mrwarm
 }
 
 void mrwarm_destroy(mrwarm_net* net) { delete net; }
+
+int mrwarm_dims(const mrwarm_net* net, int* n_in, int* hidden, int* nsh, int* nlog) {
+  if (!net) return mrcsd::fail(MRCSD_ERR_ARG, "mrwarm_dims: null network");
+  if (n_in) *n_in = net->n_in;
+  if (hidden) *hidden = net->hidden;
+  if (nsh) *nsh = net->nsh;
+  if (nlog) *nlog = net->nlog;
+  return MRCSD_OK;
+}
+
+int mrwarm_file_dims(const char* path, int* n_in, int* hidden, int* nsh, int* nlog, size_t* n_params) {
+  return mrcsd::guarded([&]() -> int {
+    if (!path) return mrcsd::fail(MRCSD_ERR_ARG, "mrwarm_file_dims: null path");
+    const auto arrays = mrcsd::npz::load(path);
+    const auto d = arrays.find("dims"), p = arrays.find("params");
+    if (d == arrays.end() || p == arrays.end())
+      return mrcsd::fail(MRCSD_ERR_ARG, std::string("\"") + path + "\" holds no 'dims' and 'params' arrays");
+    const auto dims = d->second.as_double();
+    if (dims.size() != 4 || p->second.descr != "<f4")
+      return mrcsd::fail(MRCSD_ERR_ARG, std::string("\"") + path +
+                                            "\": expected dims [n_in, hidden, nsh, nlog] and float32 params");
+    if (n_in) *n_in = int(dims[0]);
+    if (hidden) *hidden = int(dims[1]);
+    if (nsh) *nsh = int(dims[2]);
+    if (nlog) *nlog = int(dims[3]);
+    if (n_params) *n_params = p->second.count();
+    return MRCSD_OK;
+  });
+}
+
+int mrwarm_load(const char* path, mrwarm_net** out) {
+  return mrcsd::guarded([&]() -> int {
+    if (!path || !out) return mrcsd::fail(MRCSD_ERR_ARG, "mrwarm_load: null argument");
+    *out = nullptr;
+    const auto arrays = mrcsd::npz::load(path);
+    const auto d = arrays.find("dims"), p = arrays.find("params");
+    if (d == arrays.end() || p == arrays.end())
+      return mrcsd::fail(MRCSD_ERR_ARG, std::string("mrwarm_load: \"") + path + "\" holds no 'dims' and 'params' arrays");
+    const auto dims = d->second.as_double();
+    if (dims.size() != 4 || p->second.descr != "<f4")
+      return mrcsd::fail(MRCSD_ERR_ARG, std::string("mrwarm_load: \"") + path +
+                                            "\": expected dims [n_in, hidden, nsh, nlog] and float32 params");
+    mrwarm_net* net = nullptr;
+    int rc = mrwarm_create(int(dims[0]), int(dims[1]), int(dims[2]), int(dims[3]), 0, &net);
+    if (rc) return rc;
+    std::unique_ptr<mrwarm_net> hold(net);
+    const size_t n = mrwarm_num_params(net);
+    if (p->second.count() != n)
+      return mrcsd::fail(MRCSD_ERR_ARG, std::string("mrwarm_load: \"") + path + "\" holds " +
+                                            std::to_string(p->second.count()) + " parameters, the network has " +
+                                            std::to_string(n));
+    rc = mrwarm_set_params(net, reinterpret_cast<const float*>(p->second.data.data()), n);
+    if (rc) return rc;
+    *out = hold.release();
+    return MRCSD_OK;
+  });
+}
 
 size_t mrwarm_num_params(const mrwarm_net* net) {
   if (!net) return 0;

@@ -9,6 +9,9 @@
 // -info, -nthreads (accepted; the solve runs on the GPU).  Per-voxel semantics follow
 // dwi2fod.cpp:128-231: masked-out voxels are written as zeros, CSD voxels with a non-finite
 // signal give zeros, negative CSD signals are clamped, MSMT uses the raw signal.
+// Added: -warm_net net.npz (csd): start each voxel from the negative set proposed by a network
+// trained for the acquisition protocol (include/mrwarm.h); a voxel is accepted only when that set
+// is a fixed point of the iteration, else it is solved from the reference's start.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -26,6 +30,7 @@
 
 #include "../../include/mrcsd.h"
 #include "../../include/mricls.h"
+#include "../../include/mrwarm.h"
 #include "host/mif.h"
 #include "host/nifti.h"
 
@@ -131,9 +136,31 @@ void check(int rc) {
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device memory of the command's solve (freed on scope exit).
+template <class T>
+struct Dev {
+  T* p = nullptr;
+  explicit Dev(size_t n) {
+    if (n) HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  }
+  Dev(const Dev&) = delete;
+  Dev& operator=(const Dev&) = delete;
+  ~Dev() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() { HIPCHK(hipStreamCreate(&s)); }
+  Stream(const Stream&) = delete;
+  Stream& operator=(const Stream&) = delete;
+  ~Stream() { (void)hipStreamDestroy(s); }
+};
+
 struct Args {
   std::vector<std::string> pos;
-  std::string mask, directions, filter, grad, predicted;
+  std::string mask, directions, filter, grad, predicted, warm_net;
   std::vector<std::string> fslgrad;   // bvecs, bvals
   std::vector<double> shells;         // -shells b-values
   std::vector<int> lmax;
@@ -162,6 +189,7 @@ Args parse_args(int argc, char** argv) {
         a.fslgrad = {bvecs, next()};
       }
       else if (o == "predicted_signal") a.predicted = next();
+      else if (o == "warm_net") a.warm_net = next();
       else if (o == "shells") {
         std::string cur;
         for (char c : next() + ",") {
@@ -405,7 +433,55 @@ int run(int argc, char** argv) {
     console("performing constrained spherical deconvolution on " + std::to_string(nsel) + " voxels (GPU)");
     std::vector<float> fod(nsel * size_t(si.n_sh));
     std::vector<uint16_t> iters(nsel);
-    check(mrcsd_run_host(ctx, rows.data(), nvol, nullptr, nsel, fod.data(), iters.data(), nullptr));
+    bool warm = !a.warm_net.empty();
+    if (warm && !mrcsd_warm_supported(ctx)) {
+      console("[WARNING] -warm_net ignored: the warm start's fixed-point test certifies the reference's answer "
+              "only for -threshold 0 and a positive definite normal matrix (-norm_lambda > 0)");
+      warm = false;
+    }
+    if (!warm) {
+      check(mrcsd_run_host(ctx, rows.data(), nvol, nullptr, nsel, fod.data(), iters.data(), nullptr));
+    } else {
+      mrwarm_net* net = nullptr;
+      check(mrwarm_load(a.warm_net.c_str(), &net));
+      std::unique_ptr<mrwarm_net, void (*)(mrwarm_net*)> hold(net, mrwarm_destroy);
+      int n_in = 0, hidden = 0, nsh = 0, nlog = 0;
+      check(mrwarm_dims(net, &n_in, &hidden, &nsh, &nlog));
+      if (n_in != si.n_dw || nsh != si.n_sh || nlog != si.n_hr)
+        throw std::runtime_error("network \"" + a.warm_net + "\" was trained for " + std::to_string(n_in) +
+                                 " DW volumes, " + std::to_string(nsh) + " coefficients and " + std::to_string(nlog) +
+                                 " constraint directions; this run has " + std::to_string(si.n_dw) + ", " +
+                                 std::to_string(si.n_sh) + " and " + std::to_string(si.n_hr));
+      std::vector<double> dwd(size_t(si.n_dw));
+      check(mrcsd_shared_matrix(shared, 4, dwd.data(), dwd.size()));
+      std::vector<int> dwis(dwd.begin(), dwd.end()), b0;
+      for (int r = 0; r < nvol; ++r)
+        if (gused[size_t(r) * 4 + 3] <= 10.0) b0.push_back(r);   // bzero_threshold() (gradient.cpp:92-95)
+      const size_t words = size_t(si.n_hr + 63) / 64;
+      Stream st;
+      Dev<float> d_rows(nsel * size_t(nvol)), d_fod(nsel * size_t(si.n_sh));
+      Dev<int> d_dwis(dwis.size()), d_b0(b0.size());
+      Dev<uint64_t> d_neg(nsel * words), d_stats(6);
+      Dev<uint16_t> d_it(nsel);
+      HIPCHK(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, st.s));
+      HIPCHK(hipMemcpyAsync(d_dwis.p, dwis.data(), dwis.size() * sizeof(int), hipMemcpyHostToDevice, st.s));
+      if (!b0.empty())
+        HIPCHK(hipMemcpyAsync(d_b0.p, b0.data(), b0.size() * sizeof(int), hipMemcpyHostToDevice, st.s));
+      const uint64_t zero[6] = {0, 0, 0, 0, 0, 0};
+      HIPCHK(hipMemcpyAsync(d_stats.p, zero, sizeof zero, hipMemcpyHostToDevice, st.s));
+      check(mrwarm_infer(net, d_rows.p, nvol, nsel, d_dwis.p, d_b0.p, int(b0.size()), nullptr, d_neg.p, st.s));
+      mrcsd_outputs o{d_it.p, nullptr, nullptr, d_stats.p};
+      mrcsd_warm w{nullptr, d_neg.p, 6};
+      check(mrcsd_run(ctx, d_rows.p, nvol, nullptr, nsel, d_fod.p, &o, &w, st.s));
+      uint64_t stats[6];
+      HIPCHK(hipMemcpyAsync(fod.data(), d_fod.p, fod.size() * sizeof(float), hipMemcpyDeviceToHost, st.s));
+      HIPCHK(hipMemcpyAsync(iters.data(), d_it.p, nsel * sizeof(uint16_t), hipMemcpyDeviceToHost, st.s));
+      HIPCHK(hipMemcpyAsync(stats, d_stats.p, sizeof stats, hipMemcpyDeviceToHost, st.s));
+      HIPCHK(hipStreamSynchronize(st.s));
+      info("warm start: " + std::to_string(stats[5]) + " solves from the proposals, " + std::to_string(stats[4]) +
+           " voxels restarted from the plain initialisation, " + std::to_string(stats[1]) + " solves in all (" +
+           std::to_string(stats[0]) + " voxels)");
+    }
     stages.mark("GPU solve incl. transfers");
     size_t nonconv = 0;
     for (size_t i = 0; i < nsel; ++i)
@@ -463,9 +539,29 @@ int run(int argc, char** argv) {
   stages.mark("setup");
   console("performing MSMT CSD (" + std::to_string(nshell) + " shell" + (nshell > 1 ? "s" : "") + ", " +
           std::to_string(nt) + " tissue" + (nt > 1 ? "s" : "") + ") on " + std::to_string(nsel) + " voxels (GPU)");
-  std::vector<float> x(nsel * size_t(nparams));
+  std::vector<float> x(nsel * size_t(nparams)), pred;
   std::vector<uint16_t> iters(nsel);
-  check(mricls_solve_host(ic, rows.data(), nvol, nullptr, nsel, x.data(), iters.data(), nullptr));
+  {
+    // signals up, the ICLS solve and the predicted signal H x (dwi2fod.cpp:216-220) on the GPU,
+    // results down
+    Stream st;
+    Dev<float> d_rows(nsel * size_t(nvol)), d_x(nsel * size_t(nparams));
+    Dev<uint16_t> d_it(nsel);
+    HIPCHK(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, st.s));
+    check(mricls_solve(ic, d_rows.p, nvol, nullptr, nsel, d_x.p, d_it.p, nullptr, nullptr, st.s));
+    HIPCHK(hipMemcpyAsync(x.data(), d_x.p, x.size() * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipMemcpyAsync(iters.data(), d_it.p, nsel * sizeof(uint16_t), hipMemcpyDeviceToHost, st.s));
+    if (!a.predicted.empty()) {
+      check(mricls_predict(ic, d_x.p, nparams, nsel, d_rows.p, nvol, st.s));   // over the signals
+      pred.resize(rows.size());
+      HIPCHK(hipMemcpyAsync(pred.data(), d_rows.p, pred.size() * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    }
+    HIPCHK(hipStreamSynchronize(st.s));
+  }
+  const size_t over = size_t(std::count(iters.begin(), iters.end(), uint16_t(0xFFFE)));
+  if (over)
+    throw std::runtime_error(std::to_string(over) + " voxel(s) need more than 128 active constraints, beyond the GPU "
+                             "solver's capacity");
   stages.mark("GPU solve incl. transfers");
   std::vector<double> gused = grad;   // stash the table as interpreted (normalised) by the set-up
   {
@@ -483,16 +579,6 @@ int run(int argc, char** argv) {
     off += ncoef;
   }
   if (!a.predicted.empty()) {
-    // predicted signal H x (dwi2fod.cpp:216-220), H = the MSMT problem matrix
-    std::vector<double> Cm(size_t(nvol) * nparams);
-    check(mrmsmt_shared_matrix(ms, 0, Cm.data(), Cm.size()));
-    std::vector<float> pred(nsel * size_t(nvol));
-    for (size_t i = 0; i < nsel; ++i)
-      for (int r = 0; r < nvol; ++r) {
-        double s = 0.0;
-        for (int j = 0; j < nparams; ++j) s += Cm[size_t(r) * nparams + j] * x[i * size_t(nparams) + j];
-        pred[i * size_t(nvol) + r] = float(s);
-      }
     mrcsd::MifHeader h = dwi.h;
     scatter_write(a.predicted, h, sel, pred.data(), nvol, nvol, 0);
   }

@@ -17,7 +17,8 @@ Problem::Problem(const Matrix& problem_matrix, const Matrix& constraint_matrix,
       lambda_min_norm(constraint_min_norm_regularisation),
       tol(tolerance),
       max_niter(max_iterations ? max_iterations : size_t(10 * problem_matrix.cols)),
-      num_eq(num_equalities) {
+      num_eq(num_equalities),
+      standard_form(problem_in_standard_form) {
   if (H.cols != constraint_matrix.cols)
     throw std::runtime_error("FIXME: dimensions of problem and constraint matrices do not match (ICLS)");
   if (solution_min_norm_regularisation < 0.0)

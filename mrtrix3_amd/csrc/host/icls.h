@@ -35,6 +35,7 @@ class Problem {
   std::vector<double> t;
   double lambda_min_norm = 0.0, tol = 0.0;
   size_t max_niter = 0, num_eq = 0;
+  bool standard_form = false;   // H holds H^T H (the problem was given in standard form)
 };
 
 // Solver::operator()(x, b) (:225-359) for one right-hand side, run on the GPU in double

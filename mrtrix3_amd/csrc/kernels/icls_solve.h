@@ -61,4 +61,9 @@ int icls_constraint_words(int ncon);
 hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S,
                        int first_capacity, hipStream_t stream);
 
+// out[v][r] = sum_j H[r][j] x[v][j] (FP64 sums) for the problem matrix H given transposed, Ht [n][nmeas]:
+// the predicted signal of each voxel's solution.
+hipError_t icls_predict(const double* Ht, int n, int nmeas, const float* x, int ld_x, size_t nvox, float* out,
+                        int ld_out, hipStream_t stream);
+
 }  // namespace mrcsd

@@ -648,7 +648,41 @@ hipError_t launch_capacity(int kcap, const IclsDeviceProblem& P, const IclsLaunc
   }
 }
 
+// Four voxels per workgroup: their solutions in LDS, a thread per (voxel, measurement) output, so
+// the reads of Ht and the writes of out are consecutive across threads.
+constexpr int kPredictVox = 4;
+
+__global__ __launch_bounds__(256) void icls_predict_kernel(const double* __restrict__ Ht, int n, int nm,
+                                                           const float* __restrict__ x, int ld_x, size_t nvox,
+                                                           float* __restrict__ out, int ld_out) {
+  __shared__ double xs[kPredictVox][kIclsMaxParams];
+  const size_t v0 = (size_t)blockIdx.x * kPredictVox;
+  for (int i = threadIdx.x; i < kPredictVox * n; i += 256) {
+    const int vl = i / n, j = i - vl * n;
+    const size_t v = v0 + vl;
+    xs[vl][j] = v < nvox ? (double)x[v * (size_t)ld_x + j] : 0.0;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kPredictVox * nm; q += 256) {
+    const int vl = q / nm, r = q - vl * nm;
+    const size_t v = v0 + vl;
+    if (v >= nvox) continue;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc = fma(Ht[(size_t)j * nm + r], xs[vl][j], acc);
+    out[v * (size_t)ld_out + r] = (float)acc;
+  }
+}
+
 }  // namespace
+
+hipError_t icls_predict(const double* Ht, int n, int nmeas, const float* x, int ld_x, size_t nvox, float* out,
+                        int ld_out, hipStream_t stream) {
+  if (n < 1 || n > kIclsMaxParams || nmeas < 1 || ld_x < n || ld_out < nmeas) return hipErrorInvalidValue;
+  if (nvox == 0) return hipSuccess;
+  const size_t blocks = (nvox + kPredictVox - 1) / kPredictVox;
+  icls_predict_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(Ht, n, nmeas, x, ld_x, nvox, out, ld_out);
+  return hipGetLastError();
+}
 
 hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const IclsScratch& S,
                        int first_capacity, hipStream_t stream) {

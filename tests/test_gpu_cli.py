@@ -82,7 +82,8 @@ def test_cli_msmt_matches_oracle(tmp_path):
         args += [rp, op]
         outs.append(op)
     say("[gpu cli] dwi2fod msmt_csd on %d voxels" % ph.nvox)
-    run(args + ["-force"])
+    pred_path = str(tmp_path / "pred.mif")
+    run(args + ["-force", "-predicted_signal", pred_path])
     got = [mif.read(p)[0] for p in outs]
     assert got[0].shape == shape + (45,) and got[1].shape[:3] == shape and got[2].shape[:3] == shape
     s = icls.MSMTShared(ph.grad, ph.response)
@@ -96,6 +97,66 @@ def test_cli_msmt_matches_oracle(tmp_path):
     rel = np.array(rel)
     say("[gpu cli] msmt_csd: max rel %.3e over %d voxels" % (rel.max(), rel.size))
     assert np.mean(rel <= 1e-4) >= 0.999
+    # -predicted_signal (dwi2fod.cpp:216-220): H x per voxel, H the problem matrix (here the oracle's
+    # own), against the command's written solution and against the oracle's solution
+    pred, hp = mif.read(pred_path)
+    assert pred.shape == img.shape
+    xs = np.concatenate([got[0].reshape(-1, 45), got[1].reshape(-1, 1), got[2].reshape(-1, 1)], axis=1)
+    want = xs.astype(np.float64) @ o.C.T
+    p2 = pred.reshape(-1, img.shape[3]).astype(np.float64)
+    scale = np.abs(want).max(axis=1, keepdims=True)
+    err = np.abs(p2 - want) / scale
+    worst_o = 0.0
+    for k, idx in enumerate(np.ndindex(shape)):
+        x, _, _ = o.solve(img[idx].astype(np.float64))
+        worst_o = max(worst_o, float(np.abs(p2[k] - o.C @ x).max() / scale[k, 0]))
+    say("[gpu cli] -predicted_signal: %.2e of the signal scale from H x of the written solution, %.2e from the "
+        "oracle's" % (err.max(), worst_o))
+    assert err.max() <= 1e-6 and worst_o <= 1e-4
+
+
+WARM_NET = os.path.join(ROOT, "tests", "golden", "warm_net_single_shell_brain.npz")
+
+
+def test_cli_warm_net(tmp_path):
+    """dwi2fod csd -warm_net: 16,384 voxels of the whole-brain protocol (8 b=0 + 64 b=3000) from the
+    network trained for it; the same sets and coefficients within 1e-4 as the plain command in every
+    voxel, fewer solves, and the plain path (with a warning) where the certificate does not hold."""
+    ph = phantom.single_shell_brain()
+    shape = (32, 32, 16)
+    n = int(np.prod(shape))
+    img = ph.dw[:n].reshape(shape + (ph.dw.shape[1],))
+    dwi, resp = str(tmp_path / "dwi.mif"), str(tmp_path / "resp.txt")
+    mif.write(dwi, img, vox=[2.0, 2.0, 2.0], keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+    write_response(resp, ph.response)
+    plain, warm = str(tmp_path / "plain.mif"), str(tmp_path / "warm.mif")
+    say("[gpu cli] dwi2fod csd -warm_net on %d voxels" % n)
+    run(["csd", dwi, resp, plain, "-force"])
+    r = run(["csd", dwi, resp, warm, "-force", "-warm_net", WARM_NET, "-info"])
+    info = [l for l in r.stderr.splitlines() if "warm start:" in l]
+    say("[gpu cli] " + info[0])
+    fp, fw = mif.read(plain)[0].reshape(n, 45).astype(np.float64), mif.read(warm)[0].reshape(n, 45).astype(np.float64)
+    rel = np.linalg.norm(fw - fp, axis=1) / np.maximum(np.linalg.norm(fp, axis=1), 1e-30)
+    say("[gpu cli] warm vs plain: max rel %.2e" % rel.max())
+    assert rel.max() <= 1e-4
+    solves, voxels = int(info[0].split(" solves in all")[0].split(", ")[-1]), n
+    assert solves <= 2.7 * voxels
+    # a threshold voids the certificate: plain path, same output as the plain command
+    thr_p, thr_w = str(tmp_path / "thr_p.mif"), str(tmp_path / "thr_w.mif")
+    run(["csd", dwi, resp, thr_p, "-force", "-threshold", "0.1"])
+    r = run(["csd", dwi, resp, thr_w, "-force", "-threshold", "0.1", "-warm_net", WARM_NET])
+    assert "-warm_net ignored" in r.stderr
+    assert np.array_equal(mif.read(thr_p)[0], mif.read(thr_w)[0])
+    # a network trained for another protocol is refused
+    small = phantom.single_shell_small()
+    dwi2 = str(tmp_path / "dwi60.mif")
+    mif.write(dwi2, small.dw[:512].reshape(8, 8, 8, -1), vox=[2.0, 2.0, 2.0],
+              keyval=[("dw_scheme", mif.scheme_text(small.grad))])
+    resp2 = str(tmp_path / "resp60.txt")
+    write_response(resp2, small.response)
+    r = subprocess.run([EXE, "csd", dwi2, resp2, str(tmp_path / "x.mif"), "-warm_net", WARM_NET],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode != 0 and "trained for 64 DW volumes" in r.stderr
 
 
 def count_peak_mismatches(g, ref, fod):
