@@ -50,6 +50,7 @@ LIB_SOURCES = [
     "capi_peaks.cpp",
     "capi_resp.cpp",
     "capi_fmls.cpp",
+    "capi_tune.cpp",
     "icls_noalloc.hip",
 ]
 

@@ -233,7 +233,7 @@ int mrcsd_run(mrcsd_ctx* ctx, const float* dw, int ld_dw, const uint8_t* valid, 
                          "mrcsd_run: a warm start needs threshold 0 and a positive definite normal matrix "
                          "(norm_lambda > 0, or M of full column rank); solve without a proposal");
     L.counter = ctx->counter.get();
-    L.max_groups = 0;
+    L.max_groups = ctx->groups[(L.warm_F0 || L.warm_neg) ? 1 : 0];
     MRCSD_HIP(mrcsd::csd_launch(ctx->P, L, static_cast<hipStream_t>(stream)));
     return MRCSD_OK;
   });

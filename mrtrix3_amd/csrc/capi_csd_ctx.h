@@ -14,4 +14,6 @@ struct mrcsd_ctx {
   // The warm-start certificate (neg(F(S)) == S implies the plain answer) holds when the fixed point
   // is the unique minimiser of the CSD objective: threshold 0 and a positive definite Mt_M.
   bool warm_certifiable = false;
+  // persistent workgroups per launch chosen by mrcsd_autotune / mrcsd_set_groups (0: the full grid)
+  int groups[2] = {0, 0};   // plain, warm-start instantiation
 };

@@ -28,6 +28,9 @@ struct Layer {
 
 struct mrwarm_net {
   int n_in = 0, hidden = 0, nsh = 0, nlog = 0, step = 0;
+  // the transposed copies WT of the fused inference hold the current weights (cleared whenever the
+  // bf16 weights change: creation, set_params, a training step)
+  bool wt_ready = false;
   Layer layers[4];             // n_in->h, h->h, h->h, h->(nsh+nlog)
   size_t cap = 0;              // batch capacity of the workspaces
   mrcsd::DeviceBuffer<float> pre[3], dact, out, dout, loss;
@@ -244,6 +247,7 @@ int mrwarm_set_params(mrwarm_net* net, const float* host, size_t n) {
       o += size_t(L.fout);
       MRCSD_HIP(mrcsd::to_bf16(L.W.get(), L.Wb.get(), nw, nullptr));
     }
+    net->wt_ready = false;
     MRCSD_HIP(hipDeviceSynchronize());
     return MRCSD_OK;
   });
@@ -279,6 +283,7 @@ int mrwarm_train_step(mrwarm_net* net, const void* x_bf16, const float* tgt_coef
     ensure(net, M);
     backward(net, static_cast<const __bf16*>(x_bf16), tgt_coef, tgt_neg, M, wc, wl, s);
     ++net->step;
+    net->wt_ready = false;
     for (auto& L : net->layers) {
       MRCSD_HIP(mrcsd::adam_step(L.W.get(), L.Wb.get(), L.gW.get(), L.mW.get(), L.vW.get(), size_t(L.fin) * L.fout,
                                  lr, 0.9f, 0.999f, 1e-8f, net->step, 0.0f, s));
@@ -329,15 +334,18 @@ int mrwarm_infer(mrwarm_net* net, const float* dw, int ld_dw, size_t M, const in
       Layer& L = net->layers[l];
       const int kpad = (L.fin + 31) / 32 * 32, npad = (L.fout + 15) / 16 * 16;
       if (L.kpad != kpad || L.npad != npad) {
+        MRCSD_HIP(hipStreamSynchronize(s));   // an earlier launch may still read the old copy
         L.WT.alloc(size_t(kpad) * npad);
         L.kpad = kpad;
         L.npad = npad;
+        net->wt_ready = false;
       }
-      MRCSD_HIP(mrcsd::transpose_pad(L.Wb.get(), L.fin, L.fout, L.WT.get(), kpad, npad, s));
+      if (!net->wt_ready) MRCSD_HIP(mrcsd::transpose_pad(L.Wb.get(), L.fin, L.fout, L.WT.get(), kpad, npad, s));
       a.WT[l] = L.WT.get();
       a.ldw[l] = kpad;
       a.bias[l] = L.b.get();
     }
+    net->wt_ready = true;
     a.dw = dw; a.ld = ld_dw; a.M = M;
     a.dwis = dwis_dev; a.n_in = net->n_in; a.k0 = net->layers[0].kpad;
     a.b0 = b0_dev; a.n_b0 = n_b0;

@@ -38,6 +38,9 @@ struct CsdLaunch {
 };
 
 hipError_t csd_launch(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t stream);
+// Workgroups of a full persistent grid (CUs x resident workgroups per CU) for the instantiation a
+// launch with (warm) or without (!warm) a proposal runs.
+hipError_t csd_full_grid(int n_sh, bool warm, int* groups);
 bool csd_supported_nsh(int n_sh);
 int csd_threads_for(int n_sh);
 

@@ -575,6 +575,32 @@ hipError_t csd_launch(const CsdDeviceProblem& P, const CsdLaunch& L, hipStream_t
   }
 }
 
+namespace {
+template <int NSH, int NT>
+hipError_t full_grid_impl(bool warm, int* groups) {
+  auto kern = warm ? csd_persistent_kernel<NSH, NT, true> : csd_persistent_kernel<NSH, NT, false>;
+  int per_cu = 0, dev = 0, ncu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  *groups = ncu * (per_cu < 1 ? 1 : per_cu);
+  return hipSuccess;
+}
+}  // namespace
+
+hipError_t csd_full_grid(int n_sh, bool warm, int* groups) {
+  switch (n_sh) {
+    case 6:  return full_grid_impl<6, 64>(warm, groups);
+    case 15: return full_grid_impl<15, 64>(warm, groups);
+    case 28: return full_grid_impl<28, 64>(warm, groups);
+    case 45: return full_grid_impl<45, 64>(warm, groups);
+    case 66: return full_grid_impl<66, 128>(warm, groups);
+    case 91: return full_grid_impl<91, 128>(warm, groups);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 bool csd_supported_nsh(int n_sh) {
   return n_sh == 6 || n_sh == 15 || n_sh == 28 || n_sh == 45 || n_sh == 66 || n_sh == 91;
 }
