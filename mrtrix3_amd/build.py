@@ -34,6 +34,7 @@ LIB_SOURCES = [
     "kernels/peaks.hip",
     "kernels/resp.hip",
     "kernels/fmls.hip",
+    "kernels/gt.hip",
     "host/sh.cpp",
     "host/dense.cpp",
     "host/gradient.cpp",
@@ -43,6 +44,7 @@ LIB_SOURCES = [
     "host/fmls.cpp",
     "host/voxel.cpp",
     "host/npz.cpp",
+    "host/gt.cpp",
     "capi_common.cpp",
     "capi_csd.cpp",
     "capi_icls.cpp",
@@ -51,6 +53,7 @@ LIB_SOURCES = [
     "capi_resp.cpp",
     "capi_fmls.cpp",
     "capi_tune.cpp",
+    "capi_gt.cpp",
     "icls_noalloc.hip",
 ]
 
