@@ -33,7 +33,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector)
 CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096, "peaks": 65536,
-              "response": 8192}
+              "response": 8192, "gt_energy": 4096}
 
 
 def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
@@ -810,6 +810,80 @@ def bench_response(args, torch, ph):
     return res
 
 
+def gt_flops_per_voxel(nrows, ncols, nf):
+    """Algorithmic FP64 FLOPs of calcEnergy per voxel (externalenergy.cpp:245-252), the NNLS counted
+    as the ICLS solver's unconstrained part (2 nrows (nf+1)) plus one inner solve per constraint."""
+    n = nf + 1
+    return 2.0 * nrows * ncols + 2.0 * nrows * n + n * n * (n + 1) + 2.0 * nrows * nf + 3.0 * nrows
+
+
+def bench_gt(args, torch, ph):
+    """Global tractography's external energy over the whole-brain phantom: the DW signal of 646,128
+    voxels (8 b=0 + 64 b=3000), a TOD per voxel (lmax 8, random coefficients around the phantom's
+    fibres' scale), the WM response of the phantom and one isotropic tissue (ADC 3e-3 mm^2/s); a step
+    is calcEnergy over every voxel and the sum (ExternalEnergyComputer::resetEnergy)."""
+    from mrtrix3_amd import gt, phantom as P
+    from oracle import gt_oracle
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(21)
+    bvals = sorted(set(np.round(ph.grad[:, 3], -2)))
+    wm = np.zeros((len(bvals), np.size(ph.response)))
+    wm[-1] = np.ravel(ph.response)
+    wm[0, 0] = np.mean(ph.dw[:, ph.grad[:, 3] < 10]) * np.sqrt(4 * np.pi)
+    iso = [[P.isotropic_response(b, 3.0e-3)[0] for b in bvals]]
+    E = gt.ExternalEnergy(ph.grad, wm, iso, lmax=8, weight=1.0, ppot=0.01)
+    tod = (rng.standard_normal((ph.nvox, E.ncols)) * 0.01).astype(np.float32)
+    tod[:, 0] = np.abs(tod[:, 0]) + 0.05
+    dw = torch.from_numpy(ph.dw).to(dev)
+    td = torch.from_numpy(tod).to(dev)
+    for _ in range(args.warmup):
+        E.energy(dw, td)
+    torch.cuda.synchronize()
+    timer = Timer(torch, torch.cuda.current_stream(dev), args.steps)
+    holder = {}
+    wall, kern_ms = timer.run(lambda i: holder.__setitem__("r", E.energy(dw, td)), "gt_energy")
+    e, fi, total = holder["r"]
+    fl = ph.nvox * gt_flops_per_voxel(E.nrows, E.ncols, E.nf)
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+    hbm = ph.nvox * 4.0 * (E.nrows + E.ncols + 1 + E.nf)
+    res = {
+        "metric": "global tractography external energy voxels/s on 1 MI355X (calcEnergy over a volume)",
+        "value": ph.nvox * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "external energy on the single_shell_brain phantom's signal (646,128 voxels, 72 "
+                               "volumes), lmax-8 TOD, WM + one isotropic tissue, resident in HBM"},
+        "roofline": {"bound": "hbm", "achieved": hbm / (kern_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": hbm / (kern_ms * 1e-3) / 1e9 / 8000.0, "traffic": None,
+                     "kernel": "gt_residual_kernel + icls_active_set_kernel<16, 4, true> + gt_energy_kernel + gt_sum_kernel",
+                     "kernel_ms": kern_ms, "algorithmic_bytes": hbm, "fp64_flop": fl,
+                     "fp64_achieved_tflops": achieved,
+                     "note": "algorithmic bytes = float32 signal and TOD in, energy and fractions out; the FP64 "
+                             "residual (y) round trip through HBM between the passes is extra traffic"},
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample or CPU_SAMPLE["gt_energy"]
+        print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
+        O = gt_oracle.Energy(E.matrix("K"), E.matrix("Ak"), mu=0.01 * np.sqrt(4 * np.pi))
+        t0 = time.perf_counter()
+        eo = np.array([O.voxel(ph.dw[v].astype(np.float64), tod[v].astype(np.float64))[0] for v in range(sample)])
+        cpu_s = time.perf_counter() - t0
+        eg = e[:sample].cpu().numpy()
+        res["cpu_baseline"] = {"value": sample / cpu_s, "unit": "voxels/s", "cores": 1, "kind": "port",
+                               "sample": "first %d voxels; oracle float64 restatement (oracle/gt_oracle.py: numpy "
+                                         "residual, the oracle's C ICLS for the NNLS), one thread" % sample}
+        res["parity_on_cpu_sample"] = {"max_rel_err_energy": float(np.max(np.abs(eg - eo) / np.abs(eo)))}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -853,7 +927,7 @@ def main():
     elif args.workload == "peaks":
         ph = phantom.single_shell_brain()
         e2e = None if args.no_e2e else end_to_end_peaks(ph)
-    elif args.workload == "response":
+    elif args.workload in ("response", "gt_energy"):
         ph = phantom.single_shell_brain()
         e2e = None
     else:
@@ -872,6 +946,8 @@ def main():
         out = bench_peaks(args, torch, ph)
     elif args.workload == "response":
         out = bench_response(args, torch, ph)
+    elif args.workload == "gt_energy":
+        out = bench_gt(args, torch, ph)
     else:
         out = bench_csd(args, torch, args.workload, ph)
     if e2e is not None:
