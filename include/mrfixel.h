@@ -56,10 +56,7 @@ void mrfixel_options_default(mrfixel_options* opt);
 int mrfixel_create(const mrfixel_options* opt, mrfixel_ctx** out);
 void mrfixel_destroy(mrfixel_ctx* ctx);
 
-/* Segments nvox voxels.  In this build the segmentation kernels are switched off (their first run
- * on an MI355X did not complete and the cause is not known): mrfixel_segment and
- * mrfixel_segment_host fail with MRCSD_ERR_SETUP and say so; the table functions below work.
- * Device pointers:
+/* Segments nvox voxels (one GPU lane per voxel).  Device pointers:
  *   fod       [nvox][ld_fod] float, the first n_sh values of a row are the SH coefficients
  *   mask      [nvox] uint8 or NULL (-mask): 0 -> no fixels
  *   count     [nvox] uint32: fixels of each voxel (the index image's first volume)
@@ -69,12 +66,13 @@ void mrfixel_destroy(mrfixel_ctx* ctx);
  *             (directions.mif, afd, peak_amp; dispersion is integral / peak)
  *   summary   [nvox][5] float or NULL: peak amplitude of fixels 0 and 1 (0 where absent) and the
  *             direction of fixel 0 (NaN where absent)
- *   total     uint64 (device): the number of fixels; when it exceeds capacity, count is complete
- *             but offset and fixels are not: call again with capacity >= total
- * Work buffers are kept in the context: 32768 voxels x n_directions doubles of amplitudes, 5.7 MB
- * of lobe tables for voxels beyond 64 lobes, and a staging copy of the fixels, grown to the
- * largest capacity asked for.  Calls on one context are ordered on the stream they are given;
- * concurrent calls on one context race. */
+ *   total     uint64 (device): the number of fixels; when it exceeds capacity, count and offset
+ *             are complete and only the fixels below capacity are written: call again with
+ *             capacity >= total for the rest
+ * Work arrays are kept in the context, for up to 32768 voxels per pass (fewer when a call has
+ * fewer): about 110 KB per voxel at 1281 directions (amplitudes, labels, visiting order and lobe
+ * tables sized for as many lobes as directions), 3.5 GB for a full pass.  Calls on one context are
+ * ordered on the stream they are given; concurrent calls on one context race. */
 int mrfixel_segment(mrfixel_ctx* ctx, const float* fod, int ld_fod, const uint8_t* mask, size_t nvox, uint32_t* count,
                     uint32_t* offset, float* fixels, size_t capacity, float* summary, uint64_t* total, void* stream);
 

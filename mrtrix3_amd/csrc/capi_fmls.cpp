@@ -1,6 +1,7 @@
 // C ABI of the GPU FOD segmentation (include/mrfixel.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -15,8 +16,13 @@
 
 namespace {
 
-constexpr size_t kChunk = 32768;   // voxels per amplitude pass
-constexpr int kBigBlocks = 64;     // workgroups of the pass over voxels beyond the LDS tables
+// Voxels per pass: one lane each, 512 wavefronts; the work arrays take about 110 KB per voxel at
+// 1281 directions (3.5 GB for a full chunk), sized for the worst case so that no voxel is deferred.
+// (The CPU emulation build of the kernels sets a small chunk to run several passes.)
+#ifndef MRFIXEL_CHUNK
+#define MRFIXEL_CHUNK 32768
+#endif
+constexpr size_t kChunk = MRFIXEL_CHUNK;
 
 const mrcsd::fmls::DirectionSet& direction_set(const double* az_el, int n, mrcsd::fmls::DirectionSet& own) {
   if (!az_el) return mrcsd::fmls::tessellation_1281();
@@ -33,11 +39,49 @@ struct mrfixel_ctx {
   mrcsd::DeviceBuffer<double> unit, weights, sht, al;
   mrcsd::DeviceBuffer<int16_t> adj, cell_dirs;
   mrcsd::DeviceBuffer<int> cell_start;
-  mrcsd::DeviceBuffer<double> amps, big_ws;
-  mrcsd::DeviceBuffer<unsigned long long> counters, total;
-  mrcsd::DeviceBuffer<uint32_t> queue, stage_off, scan_ws;
-  mrcsd::DeviceBuffer<float> stage;
+  mrcsd::DeviceBuffer<unsigned char> work;   // fmls_work_bytes(ndir, chunk) bytes
+  size_t chunk = 0;                           // slots of the work arrays
+  mrcsd::DeviceBuffer<unsigned long long> sums;
 };
+
+namespace mrcsd {
+
+namespace {
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+}  // namespace
+
+size_t fmls_work_bytes(int ndir, size_t chunk) {
+  const size_t n = size_t(ndir) * chunk;
+  // doubles: amps, peak, integ (n each), mean, rev (3 n each); int16: lab, ord, nxt, head, tail, np;
+  // int32: nlobes (one per slot) -- the pieces fmls_work_carve takes, in its order
+  return 3 * align256(8 * n) + 2 * align256(8 * 3 * n) + 6 * align256(2 * n) + align256(4 * chunk);
+}
+
+FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
+  const size_t n = size_t(ndir) * chunk;
+  unsigned char* p = static_cast<unsigned char*>(base);
+  auto take = [&](size_t bytes) {
+    unsigned char* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  FmlsWork W{};
+  W.amps = reinterpret_cast<double*>(take(8 * n));
+  W.peak = reinterpret_cast<double*>(take(8 * n));
+  W.integ = reinterpret_cast<double*>(take(8 * n));
+  W.mean = reinterpret_cast<double*>(take(8 * 3 * n));
+  W.rev = reinterpret_cast<double*>(take(8 * 3 * n));
+  W.lab = reinterpret_cast<int16_t*>(take(2 * n));
+  W.ord = reinterpret_cast<uint16_t*>(take(2 * n));
+  W.nxt = reinterpret_cast<int16_t*>(take(2 * n));
+  W.head = reinterpret_cast<int16_t*>(take(2 * n));
+  W.tail = reinterpret_cast<int16_t*>(take(2 * n));
+  W.np = reinterpret_cast<int16_t*>(take(2 * n));
+  W.nlobes = reinterpret_cast<int32_t*>(take(4 * chunk));
+  return W;
+}
+
+}  // namespace mrcsd
 
 extern "C" {
 
@@ -119,11 +163,6 @@ int mrfixel_create(const mrfixel_options* opt, mrfixel_ctx** out) {
     ctx->adj.upload(adj16.data(), adj16.size());
     ctx->cell_dirs.upload(cells16.data(), cells16.size());
     ctx->cell_start.upload(s.cell_start.data(), s.cell_start.size());
-    ctx->amps.alloc(kChunk * size_t(s.n));
-    ctx->big_ws.alloc(size_t(kBigBlocks) * mrcsd::fmls_big_stride(s.n));
-    ctx->counters.alloc(4);
-    ctx->total.alloc(1);
-    ctx->queue.alloc(kChunk);
     auto& S = ctx->S;
     S.ndir = s.n;
     S.unit = ctx->unit.get();
@@ -163,38 +202,36 @@ int mrfixel_segment(mrfixel_ctx* ctx, const float* fod, int ld_fod, const uint8_
       return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: null fod, count, offset or fixels");
     if (ld_fod < ctx->n_sh) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: ld_fod smaller than n_sh");
     if (nvox > 0xFFFFFFFFull) return mrcsd::fail(MRCSD_ERR_ARG, "mrfixel_segment: at most 2^32 - 1 voxels per call");
-    if (!mrcsd::fmls_gpu_enabled())
-      return mrcsd::fail(MRCSD_ERR_SETUP, "mrfixel_segment: the GPU segmentation kernels are switched off in this build "
-                                          "(their first GPU run did not complete and the cause is not known)");
     MRCSD_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t nb = mrcsd::fmls_scan_blocks(nvox);
-    if (ctx->stage.size() < 5 * capacity || ctx->stage_off.size() < nvox || ctx->scan_ws.size() < nb + 1) {
+    // slots of the work arrays: this call's voxels (whole blocks) up to a full chunk; a larger set
+    // kept from an earlier call is used as it is
+    const size_t want = std::min(kChunk, std::max<size_t>(size_t(mrcsd::kFmlsBlock),
+                                                          (nvox + mrcsd::kFmlsBlock - 1) / mrcsd::kFmlsBlock *
+                                                              mrcsd::kFmlsBlock));
+    if (ctx->chunk < want) {
       MRCSD_HIP(hipStreamSynchronize(st));   // a previous call may still use them
-      if (ctx->stage.size() < 5 * capacity) ctx->stage.alloc(5 * capacity);
-      if (ctx->stage_off.size() < nvox) ctx->stage_off.alloc(nvox);
-      if (ctx->scan_ws.size() < nb + 1) ctx->scan_ws.alloc(nb + 1);
+      ctx->work.reset();
+      ctx->sums.reset();
+      ctx->chunk = 0;
+      ctx->work.alloc(mrcsd::fmls_work_bytes(ctx->ndir, want));
+      ctx->sums.alloc(want / mrcsd::kFmlsBlock + 1);
+      ctx->chunk = want;
     }
     mrcsd::FmlsLaunch L{};
     L.fod = fod;
     L.ld_fod = ld_fod;
     L.valid = mask;
     L.nvox = nvox;
-    L.amps = ctx->amps.get();
-    L.chunk = kChunk;
+    L.chunk = ctx->chunk;
+    L.work = mrcsd::fmls_work_carve(ctx->work.get(), ctx->ndir, ctx->chunk);
     L.count = count;
     L.offset = offset;
     L.fixels = fixels;
-    L.stage = ctx->stage.get();
-    L.stage_off = ctx->stage_off.get();
     L.capacity = capacity;
     L.total = reinterpret_cast<unsigned long long*>(total);
     L.summary = summary;
-    L.counters = ctx->counters.get();
-    L.queue = ctx->queue.get();
-    L.big_ws = ctx->big_ws.get();
-    L.big_blocks = kBigBlocks;
-    L.scan_ws = ctx->scan_ws.get();
+    L.sums = ctx->sums.get();
     MRCSD_HIP(mrcsd::fmls_launch(ctx->S, L, st));
     return MRCSD_OK;
   });

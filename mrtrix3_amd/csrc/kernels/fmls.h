@@ -8,8 +8,8 @@
 namespace mrcsd {
 
 constexpr int kFmlsMaxDirs = 1536;    // directions of the segmentation set (1281 for fod2fixel)
-constexpr int kFmlsMaxLobes = 64;     // positive lobes a voxel's first pass holds in LDS
 constexpr int kFmlsMaxLmax = 12;
+constexpr int kFmlsBlock = 64;        // voxels per workgroup of the segmentation and offset kernels
 
 struct FmlsDeviceSet {
   int ndir;
@@ -30,40 +30,47 @@ struct FmlsDeviceSet {
   int dir_from_peak;         // fixel direction: 1 the maximal peak, 0 the lobe's mean direction
 };
 
+// Per-voxel work arrays of one chunk, lane-interleaved: element i of the voxel in slot s of the
+// chunk lives at [i * chunk + s], so the lanes of a wavefront (consecutive slots) touching the
+// same element index touch one contiguous run of memory.  Every table is sized for the worst case
+// (as many lobes and peaks as directions), so no voxel overflows.
+struct FmlsWork {
+  double* amps;              // [ndir] amplitudes on the directions
+  int16_t* lab;              // [ndir] lobe of each direction; -1 none; -2 - l: waits for lobe l
+  uint16_t* ord;             // [ndir] positive directions in visiting order
+  int16_t* nxt;              // [ndir] peak lists through the seed directions
+  double* peak;              // [ndir] lobe records: maximal peak amplitude
+  double* mean;              // [3 ndir] amplitude-weighted direction sum, then the fixel direction
+  double* integ;             // [ndir]
+  int16_t* head;             // [ndir] first and last seed of the lobe's peak list
+  int16_t* tail;
+  int16_t* np;               // [ndir] peaks of the lobe; -1: lobe dropped by a threshold
+  double* rev;               // [3 ndir] peak directions during the revision
+  int32_t* nlobes;           // [1] lobes of the voxel, kept and dropped
+};
+
 struct FmlsLaunch {
   const float* fod;          // [nvox][ld_fod]: the first nsh values of a row are the SH coefficients
   int ld_fod;
   const uint8_t* valid;      // [nvox] or null
   size_t nvox;
-  double* amps;              // [chunk][ndir] amplitudes of the current chunk
-  size_t chunk;              // voxels per amplitude pass
+  size_t chunk;              // voxels per pass (slots of the work arrays), a multiple of kFmlsBlock
+  FmlsWork work;
   uint32_t* count;           // [nvox] fixels of each voxel
   uint32_t* offset;          // [nvox] first fixel of the voxel in `fixels` (voxel order; 0 when none)
   float* fixels;             // [capacity][5]: direction xyz, integral (AFD), maximal peak amplitude
-  float* stage;              // [capacity][5]: the same in the order voxels finish
-  uint32_t* stage_off;       // [nvox]
   size_t capacity;
-  unsigned long long* total; // fixels found in the batch (above capacity: outputs incomplete)
+  unsigned long long* total; // fixels found in the batch (above capacity: fixels incomplete)
   float* summary;            // [nvox][5] or null: maximal peak amplitude of fixels 0 and 1 (0 when
                              // absent), direction of fixel 0 (NaN when absent)
-  unsigned long long* counters;  // [4] work counters
-  uint32_t* queue;           // [chunk] voxels of the chunk whose lobes do not fit the LDS tables
-  double* big_ws;            // [big_blocks][fmls_big_stride(ndir)] lobe tables of the second pass
-  int big_blocks;
-  uint32_t* scan_ws;         // [fmls_scan_blocks(nvox) + 1]
+  unsigned long long* sums;  // [chunk / kFmlsBlock + 1] fixel counts of the blocks of a chunk
 };
 
-// Doubles of second-pass workspace per workgroup: lobe peak, mean (3), integral, revised peak
-// directions (3), then head, tail and peak count as int16.
-__host__ __device__ inline size_t fmls_big_stride(int ndir) { return 8 * (size_t)ndir + (3 * (size_t)ndir + 3) / 4; }
-// Voxel-count blocks of the offset scan (2048 voxels each).
-inline size_t fmls_scan_blocks(size_t nvox) { return (nvox + 2047) / 2048; }
+// Bytes of the work arrays for `chunk` slots over `ndir` directions (each array 256-byte aligned).
+size_t fmls_work_bytes(int ndir, size_t chunk);
+// The arrays carved out of one allocation of fmls_work_bytes(ndir, chunk) bytes.
+FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk);
 
 hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t stream);
-
-// Whether the segmentation kernels may be launched.  They are switched off: on their first run on
-// an MI355X the launch sequence did not complete, and no cause has been found in the source, the
-// generated code or the CPU emulation of the kernel (which completes and matches the oracle).
-bool fmls_gpu_enabled();
 
 }  // namespace mrcsd

@@ -1,8 +1,8 @@
 // Runs mrfixel on a case written by tests/test_fmls_kernel_emulated.py through the C ABI, with the
-// FMLS segmentation kernel compiled for the CPU against emu_runtime.h (KERNEL_SOURCE: the kernel
-// file without its launchers) and run one workgroup at a time.  The amplitude GEMM (FMA in
-// ascending coefficient order, as the GPU kernel accumulates), the offset scan and the compaction
-// are restated here on the host.
+// FMLS segmentation, offset and emit kernels compiled for the CPU against emu_runtime.h
+// (KERNEL_SOURCE: the kernel file without its launchers) and run one workgroup at a time.  The
+// amplitude GEMM (FMA in ascending coefficient order, as the GPU kernel accumulates, into the same
+// lane-interleaved layout) is restated here on the host.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -14,50 +14,42 @@
 #include KERNEL_SOURCE
 
 namespace mrcsd {
-template <int LMAX>
-void run_segment(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0, size_t nv) {
-  const unsigned grid = 2;
-  for (unsigned b = 0; b < grid; ++b)
-    emu::run_workgroup([&]() { fmls_segment_kernel<LMAX, false>(S, L, v0, nv); }, b, grid);
-  std::fprintf(stderr, "chunk at %zu: %llu voxel(s) queued for the workspace pass\n", v0,
-               (unsigned long long)L.counters[1]);
-  for (unsigned b = 0; b < grid; ++b)
-    emu::run_workgroup([&]() { fmls_segment_kernel<LMAX, true>(S, L, v0, nv); }, b, grid);
+template <class F>
+void run_grid(unsigned grid, F&& body) {
+  for (unsigned b = 0; b < grid; ++b) emu::run_workgroup([&]() { body(); }, b, grid);
 }
 
-// the emulated kernels run whatever the device build's switch says
-bool fmls_gpu_enabled() { return true; }
+template <int LMAX>
+void run_segment(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0, size_t nv, unsigned nb) {
+  run_grid(nb, [&]() { fmls_segment_kernel<LMAX>(S, L, v0, nv); });
+}
 
+// the launch sequence of fmls_launch, each kernel run one workgroup at a time
 hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t) {
+  if (S.ndir < 64 || S.ndir > kFmlsMaxDirs || L.chunk == 0 || L.chunk % kFmlsBlock) return hipErrorInvalidValue;
   *L.total = 0ull;
   for (size_t v0 = 0; v0 < L.nvox; v0 += L.chunk) {
     const size_t nv = L.nvox - v0 < L.chunk ? L.nvox - v0 : L.chunk;
-    for (int i = 0; i < 4; ++i) L.counters[i] = 0ull;
+    const unsigned nb = unsigned((nv + kFmlsBlock - 1) / kFmlsBlock);
     for (size_t v = 0; v < nv; ++v)
       for (int d = 0; d < S.ndir; ++d) {
         double acc = 0.0;
         for (int k = 0; k < S.nsh; ++k)
           acc = std::fma(S.sht[size_t(d) * S.nsh + k], (double)L.fod[(v0 + v) * size_t(L.ld_fod) + k], acc);
-        L.amps[v * size_t(S.ndir) + d] = acc;
+        L.work.amps[size_t(d) * L.chunk + v] = acc;
       }
     switch (S.lmax) {
-      case 2: run_segment<2>(S, L, v0, nv); break;
-      case 4: run_segment<4>(S, L, v0, nv); break;
-      case 6: run_segment<6>(S, L, v0, nv); break;
-      case 8: run_segment<8>(S, L, v0, nv); break;
-      case 10: run_segment<10>(S, L, v0, nv); break;
-      case 12: run_segment<12>(S, L, v0, nv); break;
+      case 2: run_segment<2>(S, L, v0, nv, nb); break;
+      case 4: run_segment<4>(S, L, v0, nv, nb); break;
+      case 6: run_segment<6>(S, L, v0, nv, nb); break;
+      case 8: run_segment<8>(S, L, v0, nv, nb); break;
+      case 10: run_segment<10>(S, L, v0, nv, nb); break;
+      case 12: run_segment<12>(S, L, v0, nv, nb); break;
       default: return hipErrorInvalidValue;
     }
-  }
-  size_t off = 0;
-  for (size_t v = 0; v < L.nvox; ++v) {
-    const uint32_t c = L.count[v];
-    L.offset[v] = c ? uint32_t(off) : 0u;
-    const size_t src = L.stage_off[v];
-    if (c && off + c <= L.capacity && src + c <= L.capacity)
-      for (uint32_t q = 0; q < 5 * c; ++q) L.fixels[off * 5 + q] = L.stage[src * 5 + q];
-    off += c;
+    run_grid(nb, [&]() { fmls_block_sums(L.count, v0, nv, L.sums); });
+    run_grid(1, [&]() { fmls_scan_sums(L.sums, nb, L.total); });
+    run_grid(nb, [&]() { fmls_emit(L, v0, nv); });
   }
   return hipSuccess;
 }

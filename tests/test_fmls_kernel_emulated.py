@@ -1,11 +1,12 @@
-"""The FOD segmentation kernel's own source (mrtrix3_amd/csrc/kernels/fmls.hip) compiled for the
+"""The FOD segmentation kernels' own source (mrtrix3_amd/csrc/kernels/fmls.hip) compiled for the
 CPU against the wavefront emulation of tests/native/emu (each of the 64 lanes a fiber, every
-__syncthreads, ballot, readlane, readfirstlane and shuffle a collective all lanes must reach at the
-same call site) and run through the C ABI (include/mrfixel.h): every voxel's fixel count, offset,
-direction, integral and peak amplitude against the oracle (oracle/fmls_oracle.c), including
-voxels whose lobes overflow the LDS tables and go to the workspace pass, lobe merges, -dirpeak,
--maxnum and the empty cases.  This checks the kernel's control flow, indexing and lane protocol
-without a GPU."""
+__syncthreads a collective all lanes must reach at the same call site) and run through the C ABI
+(include/mrfixel.h) with 64-voxel chunks, so that the offsets run over several passes: every
+voxel's fixel count, offset, direction, integral and peak amplitude against the oracle
+(oracle/fmls_oracle.c), including the constant FOD (over 300 lobes), lobe merges, -dirpeak,
+-maxnum and the empty cases.  The same build under AddressSanitizer shows every access of the
+kernels inside its buffers.  This checks the kernels' control flow, indexing and workgroup
+protocol without a GPU."""
 import os
 import shutil
 import subprocess
@@ -21,11 +22,9 @@ CSRC = os.path.join(ROOT, "mrtrix3_amd", "csrc")
 EMU = os.path.join(HERE, "native", "emu")
 
 
-@pytest.fixture(scope="module")
-def emu_exe(tmp_path_factory):
+def build_emu(tmp_path, asan=False):
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
-    tmp_path = tmp_path_factory.mktemp("emu_fmls")
     src = open(os.path.join(CSRC, "kernels", "fmls.hip")).read()
     cut = src.index("// ---- HIP launches ----")
     prep = tmp_path / "fmls_kernel_emu.hip"
@@ -34,11 +33,18 @@ def emu_exe(tmp_path_factory):
     host = [os.path.join(CSRC, f) for f in ("capi_fmls.cpp", "capi_common.cpp")] + \
            [os.path.join(CSRC, "host", f) for f in ("fmls.cpp", "sh.cpp", "dense.cpp")]
     cmd = ["g++", "-std=c++17", "-O1", "-w", "-I" + os.path.join(EMU, "include"), "-I" + CSRC,
-           "-I" + os.path.join(CSRC, "kernels"), "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(CSRC, "kernels"), "-I" + os.path.join(ROOT, "include"), "-DMRFIXEL_CHUNK=64",
            '-DKERNEL_SOURCE="%s"' % prep, "-x", "c++", os.path.join(EMU, "fmls_harness.cpp"),
            os.path.join(EMU, "emu_runtime.cpp")] + host + ["-o", exe]
+    if asan:
+        cmd[1:1] = ["-fsanitize=address", "-fno-omit-frame-pointer"]
     subprocess.run(cmd, check=True, timeout=600)
     return exe
+
+
+@pytest.fixture(scope="module")
+def emu_exe(tmp_path_factory):
+    return build_emu(tmp_path_factory.mktemp("emu_fmls"))
 
 
 def stick_fod(dirs, weights, lmax=8):
@@ -49,12 +55,12 @@ def stick_fod(dirs, weights, lmax=8):
     return np.asarray(weights, dtype=np.float64) @ orc.sh_transform(np.stack([az, el], axis=1), lmax)
 
 
-def fod_batch(seed=11):
-    """Single, crossing and three-fibre FODs with SH noise, plus the empty cases and the constant
-    FOD (rounding noise splits the sphere into well over 64 lobes)."""
+def fod_batch(seed=11, n=10):
+    """n single, crossing and three-fibre FODs with SH noise, plus the empty cases and the constant
+    FOD (rounding noise splits the sphere into over 300 lobes)."""
     rng = np.random.default_rng(seed)
     fods, mask = [], []
-    for k in range(10):
+    for k in range(n):
         nf = 1 + k % 3
         d = rng.normal(size=(nf, 3))
         w = rng.uniform(0.2, 0.6, nf)
@@ -96,7 +102,18 @@ def expected(oracle, fods, mask, maxnum, dirpeak):
     dict(maxnum=0, dirpeak=0, ithr=0.02, pthr=0.0, ratio=0.6),
 ], ids=["defaults", "merges_dirpeak_maxnum", "integral_threshold"])
 def test_fmls_kernel_under_wavefront_emulation(tmp_path, emu_exe, cfg):
-    fods, mask = fod_batch()
+    run_case(tmp_path, emu_exe, cfg, *fod_batch(n=70))     # 76 voxels: two passes of 64
+
+
+def test_fmls_kernel_under_address_sanitizer(tmp_path):
+    """Defaults on a smaller batch, the kernels built with AddressSanitizer: any access outside a
+    buffer (the lane-interleaved work arrays, the direction tables, the outputs) stops the run."""
+    (tmp_path / "build").mkdir()
+    exe = build_emu(tmp_path / "build", asan=True)
+    run_case(tmp_path, exe, dict(maxnum=0, dirpeak=0, ithr=0.0, pthr=0.1, ratio=1.0), *fod_batch(n=60))
+
+
+def run_case(tmp_path, emu_exe, cfg, fods, mask):
     nvox = fods.shape[0]
     with open(tmp_path / "case.txt", "w") as f:
         f.write("%d 45 %d %d\n%.17g %.17g %.17g\n" % (nvox, cfg["maxnum"], cfg["dirpeak"], cfg["ithr"], cfg["pthr"],
@@ -106,7 +123,7 @@ def test_fmls_kernel_under_wavefront_emulation(tmp_path, emu_exe, cfg):
     r = subprocess.run([emu_exe, str(tmp_path)], capture_output=True, text=True, timeout=1200)
     assert r.returncode == 0, r.stderr[-3000:]
     assert "EMU:" not in r.stderr, r.stderr[-3000:]
-    assert "1 voxel(s) queued for the workspace pass" in r.stderr    # the constant FOD
+    assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
     count = np.fromfile(tmp_path / "count.bin", dtype=np.uint32)
     offset = np.fromfile(tmp_path / "offset.bin", dtype=np.uint32)
     fix = np.fromfile(tmp_path / "fixels.bin", dtype=np.float32).reshape(-1, 5)

@@ -15,11 +15,7 @@ import pytest
 from mrtrix3_amd import fixel, phantom
 from oracle import orc
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="the GPU segmentation kernels are switched off: their first GPU run (the "
-                                      "305-voxel synthetic case of test_synthetic_fods_against_oracle) did not "
-                                      "complete, and no cause has been shown; the kernel source is tested under "
-                                      "CPU emulation in test_fmls_kernel_emulated.py")]
+pytestmark = pytest.mark.gpu
 
 RTOL, ATOL = 2e-6, 2e-7
 
@@ -186,6 +182,9 @@ def test_capacity_shortfall_reports_the_total(torch_cuda):
     fixel._lib.check(lib.mrfixel_segment_host(seg._h, fods.ctypes.data, 45, None, 200, count.ctypes.data,
                                               offset.ctypes.data, fix.ctypes.data, n // 3, None, C.byref(total)))
     assert total.value == n and np.array_equal(count, full.count)
+    # offsets are complete; the fixels up to the capacity are written
+    assert np.array_equal(offset, full.offset)
+    np.testing.assert_array_equal(fix, full.fixels[:n // 3])
     # nothing to segment
     d = torch.zeros((0, 45), dtype=torch.float32, device="cuda")
     r = seg.segment(d)
