@@ -33,7 +33,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector)
 CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096, "peaks": 65536,
-              "response": 8192, "gt_energy": 4096}
+              "response": 8192, "gt_energy": 4096, "fixel": 8192}
 
 
 def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
@@ -724,6 +724,92 @@ def bench_peaks(args, torch, ph):
     return res
 
 
+def bench_fixel(args, torch, ph):
+    """fod2fixel on the whole-brain FODs: the GPU CSD writes the 646,128 FODs into HBM, then a step
+    is one segmentation of every voxel (1281 directions, default thresholds): amplitudes, lobes,
+    peak revision, offsets and fixels in voxel order."""
+    from mrtrix3_amd import csd, fixel
+    from oracle import orc
+    dev = torch.device("cuda:0")
+    shared = csd.CSDShared(ph.grad, ph.response)
+    solver = csd.CSD(shared)
+    dw = torch.from_numpy(ph.dw).to(dev)
+    fod, _, _ = solver.run(dw, iters=False, nneg=False)
+    del dw
+    seg = fixel.Segmenter(shared.n_sh)
+    first = seg.segment(fod)
+    nfix = int(first.fixels.shape[0])
+    lib = fixel._lib.load()
+    count = torch.empty(ph.nvox, dtype=torch.int32, device=dev)
+    offset = torch.empty(ph.nvox, dtype=torch.int32, device=dev)
+    fix = torch.empty((max(nfix, 1), 5), dtype=torch.float32, device=dev)
+    total = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step(i):
+        fixel._lib.check(lib.mrfixel_segment(seg._h, fod.data_ptr(), fod.shape[1], None, ph.nvox, count.data_ptr(),
+                                             offset.data_ptr(), fix.data_ptr(), fix.shape[0], None, total.data_ptr(),
+                                             stream))
+
+    for _ in range(args.warmup):
+        step(0)
+    torch.cuda.synchronize()
+    timer = Timer(torch, torch.cuda.current_stream(dev), args.steps)
+    wall, kern_ms = timer.run(step, "fixel")
+    assert int(total.item()) == nfix and torch.equal(count, first.count) and torch.equal(fix, first.fixels)
+    # algorithmic traffic: the float32 FODs in, count and offset out, five floats per fixel out
+    hbm = ph.nvox * (4.0 * shared.n_sh + 8.0) + 20.0 * nfix
+    res = {
+        "metric": "fod2fixel voxels/s on 1 MI355X (FMLS segmentation of the dwi2fod FODs)",
+        "value": ph.nvox * args.steps / wall,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fod2fixel on the single_shell_brain FODs (646,128 voxels, lmax 8), 1281 "
+                               "directions, default thresholds; FODs from the plain GPU CSD, resident in HBM"},
+        "roofline": {"bound": "hbm", "achieved": hbm / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": hbm / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fmls_amp_kernel + fmls_segment_kernel<%d> + offset and emit kernels" % shared.lmax,
+                     "kernel_ms": kern_ms, "algorithmic_bytes": hbm,
+                     "note": "one lane per voxel, latency-bound: the amplitudes and per-voxel work arrays "
+                             "(lane-interleaved in HBM) are extra traffic; the amplitude GEMM is 2 x 1281 x "
+                             "n_sh FP64 FLOPs per voxel"},
+        "fixels": {"total": nfix, "per_voxel": nfix / ph.nvox},
+    }
+    if not args.no_cpu:
+        sample = args.cpu_sample or CPU_SAMPLE["fixel"]
+        print("[bench] CPU baseline on %d voxels" % sample, file=sys.stderr, flush=True)
+        o = orc.FmlsOracle(shared.lmax)
+        f_host = fod[:sample].cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        lobes = [o.segment(f_host[v], max_out=2048)[0] for v in range(sample)]
+        cpu_s = time.perf_counter() - t0
+        cnt = first.count[:sample].cpu().numpy()
+        off = first.offset[:sample].cpu().numpy()
+        fx = first.fixels.cpu().numpy()
+        same, worst = 0, 0.0
+        for v in range(sample):
+            want = np.concatenate([lobes[v][:, 0:3], lobes[v][:, 6:8]], axis=1).astype(np.float32)
+            got = fx[off[v]:off[v] + cnt[v]]
+            if cnt[v] == len(want):
+                same += 1
+                if len(want):
+                    worst = max(worst, float(np.max(np.abs(got - want) / (np.abs(want) + 1e-6))))
+        res["cpu_baseline"] = {"value": sample / cpu_s, "unit": "voxels/s", "cores": 1, "kind": "port",
+                               "sample": "first %d voxels; oracle FP64 restatement of Segmenter (oracle/fmls_oracle.c), "
+                                         "one thread" % sample}
+        res["parity_on_cpu_sample"] = {"voxels_same_fixel_count": same, "voxels": sample,
+                                       "max_rel_err_fixels": worst}
+    return res
+
+
 def resp_flops_per_sample(lmax):
     """FP64 FLOPs of one (voxel, direction) step of the amp2response accumulation
     (cmd/amp2response.cpp:152-191 as the kernel restates it): the dot product with the fibre
@@ -927,7 +1013,7 @@ def main():
     elif args.workload == "peaks":
         ph = phantom.single_shell_brain()
         e2e = None if args.no_e2e else end_to_end_peaks(ph)
-    elif args.workload in ("response", "gt_energy"):
+    elif args.workload in ("response", "gt_energy", "fixel"):
         ph = phantom.single_shell_brain()
         e2e = None
     else:
@@ -948,6 +1034,8 @@ def main():
         out = bench_response(args, torch, ph)
     elif args.workload == "gt_energy":
         out = bench_gt(args, torch, ph)
+    elif args.workload == "fixel":
+        out = bench_fixel(args, torch, ph)
     else:
         out = bench_csd(args, torch, args.workload, ph)
     if e2e is not None:

@@ -69,9 +69,10 @@ void mrfixel_destroy(mrfixel_ctx* ctx);
  *   total     uint64 (device): the number of fixels; when it exceeds capacity, count and offset
  *             are complete and only the fixels below capacity are written: call again with
  *             capacity >= total for the rest
- * Work arrays are kept in the context, for up to 32768 voxels per pass (fewer when a call has
- * fewer): about 110 KB per voxel at 1281 directions (amplitudes, labels, visiting order and lobe
- * tables sized for as many lobes as directions), 3.5 GB for a full pass.  Calls on one context are
+ * Work arrays are kept in the context, for up to 131072 voxels per pass (fewer when a call has
+ * fewer, or when half the free device memory holds fewer): about 110 KB per voxel at 1281
+ * directions (amplitudes, labels, visiting order and lobe tables sized for as many lobes as
+ * directions), 14 GB for a full pass.  Calls on one context are
  * ordered on the stream they are given; concurrent calls on one context race. */
 int mrfixel_segment(mrfixel_ctx* ctx, const float* fod, int ld_fod, const uint8_t* mask, size_t nvox, uint32_t* count,
                     uint32_t* offset, float* fixels, size_t capacity, float* summary, uint64_t* total, void* stream);

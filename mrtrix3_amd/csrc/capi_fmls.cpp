@@ -16,11 +16,13 @@
 
 namespace {
 
-// Voxels per pass: one lane each, 512 wavefronts; the work arrays take about 110 KB per voxel at
-// 1281 directions (3.5 GB for a full chunk), sized for the worst case so that no voxel is deferred.
-// (The CPU emulation build of the kernels sets a small chunk to run several passes.)
+// Voxels per pass: one lane each, 2048 wavefronts (two per SIMD, the segmentation kernel's
+// occupancy); the work arrays take about 110 KB per voxel at 1281 directions (14 GB for a full
+// pass), sized for the worst case so that no voxel is deferred, and a pass is cut down to what
+// half the free device memory holds.  (The CPU emulation build of the kernels sets a small chunk
+// to run several passes.)
 #ifndef MRFIXEL_CHUNK
-#define MRFIXEL_CHUNK 32768
+#define MRFIXEL_CHUNK 131072
 #endif
 constexpr size_t kChunk = MRFIXEL_CHUNK;
 
@@ -206,14 +208,18 @@ int mrfixel_segment(mrfixel_ctx* ctx, const float* fod, int ld_fod, const uint8_
     const hipStream_t st = static_cast<hipStream_t>(stream);
     // slots of the work arrays: this call's voxels (whole blocks) up to a full chunk; a larger set
     // kept from an earlier call is used as it is
-    const size_t want = std::min(kChunk, std::max<size_t>(size_t(mrcsd::kFmlsBlock),
-                                                          (nvox + mrcsd::kFmlsBlock - 1) / mrcsd::kFmlsBlock *
-                                                              mrcsd::kFmlsBlock));
+    size_t want = std::min(kChunk, std::max<size_t>(size_t(mrcsd::kFmlsBlock),
+                                                    (nvox + mrcsd::kFmlsBlock - 1) / mrcsd::kFmlsBlock *
+                                                        mrcsd::kFmlsBlock));
     if (ctx->chunk < want) {
       MRCSD_HIP(hipStreamSynchronize(st));   // a previous call may still use them
       ctx->work.reset();
       ctx->sums.reset();
       ctx->chunk = 0;
+      size_t free_b = 0, total_b = 0;
+      MRCSD_HIP(hipMemGetInfo(&free_b, &total_b));
+      while (want > size_t(mrcsd::kFmlsBlock) && mrcsd::fmls_work_bytes(ctx->ndir, want) > free_b / 2)
+        want = std::max(size_t(mrcsd::kFmlsBlock), want / 2 / mrcsd::kFmlsBlock * mrcsd::kFmlsBlock);
       ctx->work.alloc(mrcsd::fmls_work_bytes(ctx->ndir, want));
       ctx->sums.alloc(want / mrcsd::kFmlsBlock + 1);
       ctx->chunk = want;

@@ -253,8 +253,13 @@ MifImage read_mif(const std::string& path) {
   return img;
 }
 
-void write_mif(const std::string& path, const MifHeader& hin, const std::vector<float>& data,
-               bool volume_contiguous) {
+namespace {
+
+// T is float (Float32LE) or uint32_t (UInt32LE), both 4 bytes
+template <class T>
+void write_mif_t(const std::string& path, const MifHeader& hin, const std::vector<T>& data, bool volume_contiguous,
+                 const char* dtype) {
+  static_assert(sizeof(T) == 4, "4-byte element types only");
   MifHeader h = hin;
   const size_t nd = h.dim.size();
   if (int64_t(data.size()) != h.nvoxels()) throw std::runtime_error("write_mif: data size does not match dimensions");
@@ -270,7 +275,7 @@ void write_mif(const std::string& path, const MifHeader& hin, const std::vector<
   for (size_t a = 0; a < nd; ++a) hdr << (a ? "," : "") << (a < h.vox.size() ? h.vox[a] : 1.0);
   hdr << "\nlayout: ";
   for (size_t a = 0; a < nd; ++a) hdr << (a ? "," : "") << (h.layout[a] > 0 ? "+" : "-") << std::abs(h.layout[a]) - 1;
-  hdr << "\ndatatype: Float32LE";
+  hdr << "\ndatatype: " << dtype;
   if (h.transform.size() >= 3) {
     for (int r = 0; r < 3; ++r) {
       hdr << "\ntransform: ";
@@ -292,18 +297,28 @@ void write_mif(const std::string& path, const MifHeader& hin, const std::vector<
   for (int64_t i = 0; i < pad; ++i) out.put('\0');
   std::vector<bool> flip;
   const auto st = file_strides(h, flip);
-  std::vector<float> buf(data.size());
+  std::vector<T> buf(data.size());
   const int64_t nx = h.dim[0];
   for_each_row(h.dim, st, flip, [&](int64_t f, int64_t c) {
-    const float* src = &data[size_t(c)];
+    const T* src = &data[size_t(c)];
     if (st[0] == 1) {
-      std::memcpy(&buf[size_t(f)], src, size_t(nx) * sizeof(float));
+      std::memcpy(&buf[size_t(f)], src, size_t(nx) * sizeof(T));
     } else {
       for (int64_t x = 0; x < nx; ++x, f += st[0]) buf[size_t(f)] = src[x];
     }
   });
-  out.write(reinterpret_cast<const char*>(buf.data()), std::streamsize(buf.size() * sizeof(float)));
+  out.write(reinterpret_cast<const char*>(buf.data()), std::streamsize(buf.size() * sizeof(T)));
   if (!out) throw std::runtime_error("failed writing image \"" + path + "\"");
+}
+
+}  // namespace
+
+void write_mif(const std::string& path, const MifHeader& h, const std::vector<float>& data, bool volume_contiguous) {
+  write_mif_t(path, h, data, volume_contiguous, "Float32LE");
+}
+
+void write_mif(const std::string& path, const MifHeader& h, const std::vector<uint32_t>& data) {
+  write_mif_t(path, h, data, false, "UInt32LE");
 }
 
 void transpose_blocked(const float* src, int64_t nr, int64_t nc, float* dst) {

@@ -36,6 +36,8 @@ MifImage read_mif(const std::string& path);
 // otherwise axis 0 fastest.  data is in canonical order.
 void write_mif(const std::string& path, const MifHeader& h, const std::vector<float>& data,
                bool volume_contiguous);
+// Writes UInt32LE (a fixel index image), axis 0 fastest; data in canonical order.
+void write_mif(const std::string& path, const MifHeader& h, const std::vector<uint32_t>& data);
 // dst[j][i] = src[i][j] for src [nr][nc] row-major (between an image's volume-slowest canonical
 // order and voxel rows), in cache-sized tiles.
 void transpose_blocked(const float* src, int64_t nr, int64_t nc, float* dst);

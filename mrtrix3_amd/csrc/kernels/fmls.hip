@@ -23,8 +23,9 @@
 //   * fmls_emit: each voxel's offset in voxel order (the order fod2fixel's index image uses,
 //     cmd/fod2fixel.cpp:239-280; 0 for voxels without fixels) and its fixels.
 // The work arrays hold every table a voxel can need (as many lobes and peaks as directions), so
-// no voxel is deferred, and the segmentation has no cross-lane operation, barrier or atomic: the
-// only cooperation between lanes is the workgroup scans of the offset kernels.
+// no voxel is deferred, and the segmentation has no cross-lane operation, barrier or atomic (its
+// LDS is a column per lane for the Newton search's Legendre values): the only cooperation between
+// lanes is the workgroup scans of the offset kernels.
 // Negative directions only ever form negative lobes, which the reference discards and which take
 // no part in positive lobes' adjacency, order, merges or indices, so they are not visited.
 #include <hip/hip_runtime.h>
@@ -48,56 +49,51 @@ struct Lane {
 __device__ __forceinline__ int fm_mpos(int l, int m) { return l * l / 4 + m; }
 __device__ __forceinline__ int fm_sh(int l, int m) { return l * (l + 1) / 2 + m; }
 
-// The associated Legendre values of one elevation, interpolated between two rows of the
-// precomputed table (PrecomputedAL::set / get, SH.h:436-470).
-struct AlRow {
-  const double* p1;
-  const double* p2;
-  double f1, f2;
-  __device__ __forceinline__ double operator()(int k) const {
-    double v = f1 * p1[k];
-    if (f2 != 0.0) v += f2 * p2[k];
-    return v;
-  }
-};
-
 // Math::SH::derivatives with PrecomputedAL<double> (SH.h:568-647): out = {amplitude, d/del,
-// d/daz, d2/del2, d2/deldaz, d2/daz2}.  sh: the voxel's float32 coefficients.
+// d/daz, d2/del2, d2/deldaz, d2/daz2}.  sh: the voxel's float32 coefficients.  The Legendre values
+// of the elevation, interpolated between two rows of the precomputed table (PrecomputedAL::set /
+// get, SH.h:436-470), go to this lane's own column of an LDS table (entry k at AL[kFmlsBlock k]),
+// read by no other lane.
 template <int LMAX>
-__device__ void fmls_derivatives(const float* sh, double el, double az, const FmlsDeviceSet& S, double* out) {
+__device__ void fmls_derivatives(const float* sh, double el, double az, const FmlsDeviceSet& S, double* AL,
+                                 double* out) {
   constexpr int NAL = (LMAX / 2 + 1) * (LMAX / 2 + 1);
   const double sel = sin(el);
   const bool atpole = sel < 1e-4;
-  AlRow AL;
   {
     const double inc = M_PI / (S.al_ndir - 1);
-    double f2 = el / inc;
+    double f2 = el / inc, f1;
     int i;
     if (!(f2 >= 0.0)) {             // (also a NaN elevation: the first row, never out of bounds)
       i = 0;
-      AL.f1 = 1.0;
+      f1 = 1.0;
       f2 = 0.0;
     } else if (f2 >= (double)(S.al_ndir - 1)) {
       i = S.al_ndir - 1;
-      AL.f1 = 1.0;
+      f1 = 1.0;
       f2 = 0.0;
     } else {
       i = (int)f2;
       f2 -= i;
-      AL.f1 = 1.0 - f2;
+      f1 = 1.0 - f2;
     }
-    AL.f2 = f2;
-    AL.p1 = S.al + (size_t)i * NAL;
-    AL.p2 = AL.p1 + (f2 != 0.0 ? NAL : 0);
+    const double* p1 = S.al + (size_t)i * NAL;
+    const double* p2 = p1 + (f2 != 0.0 ? NAL : 0);
+    for (int k = 0; k < NAL; ++k) {
+      double v = f1 * p1[k];
+      if (f2 != 0.0) v += f2 * p2[k];
+      AL[kFmlsBlock * k] = v;
+    }
   }
-  double amplitude = (double)sh[0] * AL(0);
+#define FM_AL(k) AL[kFmlsBlock * (k)]
+  double amplitude = (double)sh[0] * FM_AL(0);
   double dSH_del = 0.0, dSH_daz = 0.0, d2SH_del2 = 0.0, d2SH_deldaz = 0.0, d2SH_daz2 = 0.0;
   for (int l = 2; l <= LMAX; l += 2) {
     const double v = (double)sh[fm_sh(l, 0)];
-    amplitude += v * AL(fm_mpos(l, 0));
-    dSH_del += v * sqrt((double)(l * (l + 1))) * AL(fm_mpos(l, 1));
-    d2SH_del2 += v * (sqrt((double)(l * (l + 1) * (l - 1) * (l + 2))) * AL(fm_mpos(l, 2)) -
-                      l * (l + 1) * AL(fm_mpos(l, 0))) / 2.0;
+    amplitude += v * FM_AL(fm_mpos(l, 0));
+    dSH_del += v * sqrt((double)(l * (l + 1))) * FM_AL(fm_mpos(l, 1));
+    d2SH_del2 += v * (sqrt((double)(l * (l + 1) * (l - 1) * (l + 2))) * FM_AL(fm_mpos(l, 2)) -
+                      l * (l + 1) * FM_AL(fm_mpos(l, 0))) / 2.0;
   }
 #pragma unroll 1
   for (int m = 1; m <= LMAX; ++m) {
@@ -106,16 +102,16 @@ __device__ void fmls_derivatives(const float* sh, double el, double az, const Fm
     for (int l = (m & 1) ? m + 1 : m; l <= LMAX; l += 2) {
       const double vp = (double)sh[fm_sh(l, m)];
       const double vm = (double)sh[fm_sh(l, -m)];
-      const double alm = AL(fm_mpos(l, m));
+      const double alm = FM_AL(fm_mpos(l, m));
       amplitude += (vp * caz + vm * saz) * alm;
-      double tmp = sqrt((double)((l + m) * (l - m + 1))) * AL(fm_mpos(l, m - 1));
-      if (l > m) tmp -= sqrt((double)((l - m) * (l + m + 1))) * AL(fm_mpos(l, m + 1));
+      double tmp = sqrt((double)((l + m) * (l - m + 1))) * FM_AL(fm_mpos(l, m - 1));
+      if (l > m) tmp -= sqrt((double)((l - m) * (l + m + 1))) * FM_AL(fm_mpos(l, m + 1));
       tmp /= -2.0;
       dSH_del += (vp * caz + vm * saz) * tmp;
       double tmp2 = -((l + m) * (l - m + 1) + (l - m) * (l + m + 1)) * alm;
-      if (m == 1) tmp2 -= sqrt((double)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * AL(fm_mpos(l, 1));
-      else tmp2 += sqrt((double)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * AL(fm_mpos(l, m - 2));
-      if (l > m + 1) tmp2 += sqrt((double)((l - m) * (l + m + 1) * (l - m - 1) * (l + m + 2))) * AL(fm_mpos(l, m + 2));
+      if (m == 1) tmp2 -= sqrt((double)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * FM_AL(fm_mpos(l, 1));
+      else tmp2 += sqrt((double)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * FM_AL(fm_mpos(l, m - 2));
+      if (l > m + 1) tmp2 += sqrt((double)((l - m) * (l + m + 1) * (l - m - 1) * (l + m + 2))) * FM_AL(fm_mpos(l, m + 2));
       tmp2 /= 4.0;
       d2SH_del2 += (vp * caz + vm * saz) * tmp2;
       if (atpole) {
@@ -132,6 +128,7 @@ __device__ void fmls_derivatives(const float* sh, double el, double az, const Fm
     d2SH_deldaz /= sel;
     d2SH_daz2 /= sel * sel;
   }
+#undef FM_AL
   out[0] = amplitude; out[1] = dSH_del; out[2] = dSH_daz;
   out[3] = d2SH_del2; out[4] = d2SH_deldaz; out[5] = d2SH_daz2;
 }
@@ -139,13 +136,13 @@ __device__ void fmls_derivatives(const float* sh, double el, double az, const Fm
 // Math::SH::get_peak (SH.h:514-559): Newton steps along the gradient, at most 0.2 rad each,
 // until a step is below 1e-4 rad; NaN after 50 steps
 template <int LMAX>
-__device__ double fmls_get_peak(const float* sh, double* dir, const FmlsDeviceSet& S) {
+__device__ double fmls_get_peak(const float* sh, double* dir, const FmlsDeviceSet& S, double* AL) {
   const double MAX_DIR_CHANGE = 0.2, ANGLE_TOLERANCE = 1e-4;
   for (int it = 0; it < 50; ++it) {
     const double az = atan2(dir[1], dir[0]);
     const double el = atan2(hypot(dir[0], dir[1]), dir[2]);
     double d[6];
-    fmls_derivatives<LMAX>(sh, el, az, S, d);
+    fmls_derivatives<LMAX>(sh, el, az, S, AL, d);
     double del = sqrt(d[1] * d[1] + d[2] * d[2]);
     double daz = 0.0;
     if (del != 0.0) {
@@ -277,39 +274,45 @@ __device__ void lobe_copy(const Tabs& T, int dst, int src) {
   T.np[dst] = T.np[src];
 }
 
-// direction a is visited after direction b: lower amplitude, or the same amplitude and a higher
-// index
-__device__ __forceinline__ bool visited_after(const Tabs& T, int a, int b) {
-  const double va = T.amp[a], vb = T.amp[b];
-  return va < vb || (va == vb && a > b);
+// direction a (amplitude ka) is visited after direction b (amplitude kb): lower amplitude, or the
+// same amplitude and a higher index
+__device__ __forceinline__ bool visited_after(double ka, int a, double kb, int b) {
+  return ka < kb || (ka == kb && a > b);
 }
 
-// sift-down of a heap on ord[0, end) whose root is the direction visited last; a heap of at most
-// kFmlsMaxDirs entries is 11 levels deep, so 12 steps always reach a leaf
-__device__ void heap_sift(const Tabs& T, int root, int end) {
+// sift-down (hole method) of a heap of (amplitude, direction) entries in key[0, end), ord[0, end)
+// whose root is the direction visited last; a heap of at most kFmlsMaxDirs entries is 11 levels
+// deep, so 12 steps always reach a leaf
+__device__ void heap_sift(const Lane<double>& key, const Lane<uint16_t>& ord, int root, int end) {
+  const int r = ord[root];
+  const double kr = key[root];
   for (int step = 0; step < 12; ++step) {
     int child = 2 * root + 1;
-    if (child >= end) return;
-    int c = T.ord[child];
+    if (child >= end) break;
+    int c = ord[child];
+    double kc = key[child];
     if (child + 1 < end) {
-      const int c2 = T.ord[child + 1];
-      if (visited_after(T, c2, c)) {
+      const int c2 = ord[child + 1];
+      const double k2 = key[child + 1];
+      if (visited_after(k2, c2, kc, c)) {
         ++child;
         c = c2;
+        kc = k2;
       }
     }
-    const int r = T.ord[root];
-    if (!visited_after(T, c, r)) return;
-    T.ord[root] = (uint16_t)c;
-    T.ord[child] = (uint16_t)r;
+    if (!visited_after(kc, c, kr, r)) break;
+    ord[root] = (uint16_t)c;
+    key[root] = kc;
     root = child;
   }
+  ord[root] = (uint16_t)r;
+  key[root] = kr;
 }
 
 // The segmentation of one voxel (fmls.cpp:166-301); returns the number of positive lobes, whose
 // records hold the kept fixels (np < 0: dropped by a threshold).
 template <int LMAX>
-__device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const float* f) {
+__device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const float* f, double* AL) {
   const int ndir = S.ndir;
   // the largest |amplitude| (lowest index among equal ones) must be positive (fmls.cpp:171-184)
   double best = -1.0;
@@ -323,18 +326,27 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const f
   }
   if (!(T.amp[bi] > 0.0)) return 0;
 
-  // the positive directions in visiting order
+  // the positive directions in visiting order, by heapsort on (amplitude, direction) entries; the
+  // amplitudes ride along in the peak-direction array, unused until the revision
   int P = 0;
   for (int d = 0; d < ndir; ++d) {
     T.lab[d] = -1;
-    if (T.amp[d] > 0.0) T.ord[P++] = (uint16_t)d;
+    const double v = T.amp[d];
+    if (v > 0.0) {
+      T.ord[P] = (uint16_t)d;
+      T.rev[P] = v;
+      ++P;
+    }
   }
-  for (int s = P / 2 - 1; s >= 0; --s) heap_sift(T, s, P);
+  for (int s = P / 2 - 1; s >= 0; --s) heap_sift(T.rev, T.ord, s, P);
   for (int e = P - 1; e > 0; --e) {
     const uint16_t t = T.ord[0];
+    const double kt = T.rev[0];
     T.ord[0] = T.ord[e];
+    T.rev[0] = T.rev[e];
     T.ord[e] = t;
-    heap_sift(T, 0, e);
+    T.rev[e] = kt;
+    heap_sift(T.rev, T.ord, 0, e);
   }
 
   // ---- lobe growth (fmls.cpp:188-249) ----
@@ -443,7 +455,7 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const f
     if (keep) {
       for (int p = 0; p < np; ++p) {
         double nd[3] = {T.rev[r0 + 3 * p], T.rev[r0 + 3 * p + 1], T.rev[r0 + 3 * p + 2]};
-        const double nvl = fmls_get_peak<LMAX>(f, nd, S);
+        const double nvl = fmls_get_peak<LMAX>(f, nd, S, AL);
         if (isfinite(nvl) && isfinite(nd[0]) && isfinite(nd[1]) && isfinite(nd[2])) {
           double max_dp = 0.0;
           int nearest = np;
@@ -488,6 +500,8 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const f
 
 template <int LMAX>
 __global__ __launch_bounds__(kFmlsBlock) void fmls_segment_kernel(FmlsDeviceSet S, FmlsLaunch L, size_t v0, size_t nv) {
+  // per-lane Legendre values of the Newton search: lane t's column at s_al[t + kFmlsBlock k]
+  __shared__ double s_al[(LMAX / 2 + 1) * (LMAX / 2 + 1) * kFmlsBlock];
   const size_t slot = (size_t)blockIdx.x * kFmlsBlock + threadIdx.x;
   if (slot >= nv) return;
   const size_t vox = v0 + slot;
@@ -495,7 +509,7 @@ __global__ __launch_bounds__(kFmlsBlock) void fmls_segment_kernel(FmlsDeviceSet 
   const float* f = L.fod + vox * (size_t)L.ld_fod;
   const double c0 = (double)f[0];
   const bool ok = (L.valid == nullptr || L.valid[vox] != 0) && c0 > 0.0 && isfinite(c0);
-  const int nl = ok ? fmls_segment_voxel<LMAX>(S, T, f) : 0;
+  const int nl = ok ? fmls_segment_voxel<LMAX>(S, T, f, s_al + threadIdx.x) : 0;
   // the kept lobes in order, at most maxnum of them (fod2fixel.cpp:139-150)
   int nk = 0;
   double sp0 = 0.0, sp1 = 0.0, sx = NAN, sy = NAN, sz = NAN;

@@ -132,3 +132,4 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t) { return "emu error"; }
 inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 1; return hipSuccess; }
+inline hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b) { *free_b = *total_b = size_t(1) << 36; return hipSuccess; }

@@ -1,6 +1,7 @@
 // Runs mricls on a case written by tests/test_icls_kernel_emulated.py through the C ABI, with the
 // ICLS kernel source compiled for the CPU against emu_runtime.h (KERNEL_SOURCE: the kernel file
 // without its launcher) and run one workgroup at a time.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -46,6 +47,17 @@ hipError_t icls_launch(const IclsDeviceProblem& P, const IclsLaunch& L, const Ic
     std::fprintf(stderr, "pass %d: capacity %d, queued for the next pass %llu\n", i, caps[i],
                  i + 1 < np ? (unsigned long long)S.counters[kIclsMaxPasses + i] : 0ull);
   }
+  return hipSuccess;
+}
+// mricls_predict's launcher, restated on the host (the emulation runs the active-set kernel only)
+hipError_t icls_predict(const double* Ht, int n, int nmeas, const float* x, int ld_x, size_t nvox, float* out,
+                        int ld_out, hipStream_t) {
+  for (size_t v = 0; v < nvox; ++v)
+    for (int r = 0; r < nmeas; ++r) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) acc = std::fma(Ht[size_t(j) * nmeas + r], (double)x[v * size_t(ld_x) + j], acc);
+      out[v * size_t(ld_out) + r] = (float)acc;
+    }
   return hipSuccess;
 }
 }  // namespace mrcsd

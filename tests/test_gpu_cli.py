@@ -232,6 +232,72 @@ def test_cli_sh2peaks_matches_oracle(tmp_path):
     assert not np.array_equal(g3, g, equal_nan=True)
 
 
+def test_cli_fod2fixel_matches_oracle(tmp_path):
+    """fod2fixel FOD fixels -afd -peak -disp -mask -fmls_no_thresholds (the call of dwi2response
+    tournier, -peak standing for -peak_amp): the fixel directory's index, directions and data
+    files against the oracle's segmentation of the same float32 FODs, in voxel order; then -maxnum
+    and -dirpeak."""
+    exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "fod2fixel")
+    assert os.path.exists(exe), "fod2fixel is not built"
+    ph = phantom.single_shell_small()
+    shape = (16, 16, 8)
+    n = int(np.prod(shape))
+    o = orc.CsdOracle(ph.grad, ph.response)
+    fod, _ = o.run_volume(np.ascontiguousarray(ph.dw[:n]), shape, nthreads=8)
+    fod = np.ascontiguousarray(fod, dtype=np.float32)
+    rng = np.random.default_rng(8)
+    mask = (rng.random(shape[::-1]) < 0.85).astype(np.float32)          # [z, y, x]
+    img = fod.reshape(shape[2], shape[1], shape[0], 45).transpose(2, 1, 0, 3)
+    fp, mp = str(tmp_path / "fod.mif"), str(tmp_path / "mask.mif")
+    mif.write(fp, img, vox=[2.0, 2.0, 2.0])
+    mif.write(mp, mask.transpose(2, 1, 0), vox=[2.0, 2.0, 2.0])
+    m = mask.reshape(-1)
+
+    def segment(args, out, oracle, maxnum=0, dirpeak=False):
+        say("[gpu cli] fod2fixel %s on %d voxels" % (" ".join(args), n))
+        r = subprocess.run([exe, fp, out] + args, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stderr
+        idx, ih = mif.read(os.path.join(out, "index.mif"))
+        assert ih["datatype"].lower().startswith("uint32") and idx.shape == shape + (2,)
+        count = idx[..., 0].transpose(2, 1, 0).reshape(-1).astype(np.int64)
+        offset = idx[..., 1].transpose(2, 1, 0).reshape(-1).astype(np.int64)
+        nf = int(dict(ih["keyval"])["nfixels"])
+        dirs, dh = mif.read(os.path.join(out, "directions.mif"))
+        assert dirs.shape == (nf, 3, 1) and nf == count.sum()
+        run = 0
+        for v in range(n):
+            lobes = np.zeros((0, 8))
+            if m[v]:
+                lobes, _ = oracle.segment(fod[v].astype(np.float64), max_out=2048)
+                if maxnum:
+                    lobes = lobes[:maxnum]
+            assert count[v] == len(lobes), "voxel %d" % v
+            assert offset[v] == (run if count[v] else 0)
+            want = (lobes[:, 3:6] if dirpeak else lobes[:, 0:3]).astype(np.float32)
+            np.testing.assert_allclose(dirs[run:run + count[v], :, 0], want, rtol=2e-6, atol=2e-7)
+            run += count[v]
+        return nf
+
+    out = str(tmp_path / "fixels")
+    ref = orc.FmlsOracle(8, integral=0.0, peak_value=0.0)
+    nf = segment(["-afd", "afd.mif", "-peak", "peak.mif", "-disp", "disp.mif", "-mask", mp, "-fmls_no_thresholds"],
+                 out, ref)
+    lob = [ref.segment(fod[v].astype(np.float64), max_out=2048)[0] for v in range(n) if m[v]]
+    lob = np.concatenate([x for x in lob if len(x)]) if lob else np.zeros((0, 8))
+    afd, _ = mif.read(os.path.join(out, "afd.mif"))
+    peak, _ = mif.read(os.path.join(out, "peak.mif"))
+    disp, _ = mif.read(os.path.join(out, "disp.mif"))
+    assert afd.shape == (nf, 1, 1) and peak.shape == (nf, 1, 1) and disp.shape == (nf, 1, 1)
+    np.testing.assert_allclose(afd[:, 0, 0], lob[:, 6].astype(np.float32), rtol=2e-6, atol=2e-7)
+    np.testing.assert_allclose(peak[:, 0, 0], lob[:, 7].astype(np.float32), rtol=2e-6, atol=2e-7)
+    np.testing.assert_allclose(disp[:, 0, 0], lob[:, 6].astype(np.float32) / lob[:, 7].astype(np.float32),
+                               rtol=4e-6, atol=2e-7)
+    say("[gpu cli] fod2fixel: %d fixels in %d masked voxels" % (nf, int(m.sum())))
+    # defaults with -maxnum 2 -dirpeak, no mask
+    m = np.ones(n)
+    segment(["-maxnum", "2", "-dirpeak"], str(tmp_path / "fixels2"), orc.FmlsOracle(8), maxnum=2, dirpeak=True)
+
+
 def test_cli_fslgrad_matches_dw_scheme(tmp_path):
     """-fslgrad bvecs bvals (core/dwi/gradient.cpp:127-200) on an image whose transform is a
     rotation: the bvecs are the scheme's directions in image axes with the first component negated

@@ -139,3 +139,46 @@ def test_nifti_input_checks(tmp_path):
         f.write(b"\0" * 100)
     r = subprocess.run([EXE, "csd", short, "r.txt", str(tmp_path / "o.mif")], capture_output=True, text=True)
     assert r.returncode == 1 and "too small" in r.stderr
+
+
+F2F = os.path.join(ROOT, "mrtrix3_amd", "bin", "fod2fixel")
+
+
+@pytest.mark.skipif(not os.path.exists(F2F), reason="fod2fixel not built")
+def test_fod2fixel_argument_errors(tmp_path):
+    """fod2fixel's checks before any GPU work (cmd/fod2fixel.cpp:285-320, fixel/helpers.h:190-211,
+    Math::SH::check): usage, SH image shape, mask dimensions, the output directory, option names
+    (unambiguous prefixes accepted, as the reference's parser does) and option ranges."""
+    fod = str(tmp_path / "fod.mif")
+    mif.write(fod, np.ones((2, 2, 2, 15), dtype=np.float32))
+    bad = str(tmp_path / "bad.mif")
+    mif.write(bad, np.ones((2, 2, 2, 14), dtype=np.float32))
+    three = str(tmp_path / "three.mif")
+    mif.write(three, np.ones((2, 2, 2), dtype=np.float32))
+    msk = str(tmp_path / "mask.mif")
+    mif.write(msk, np.ones((2, 3, 2), dtype=np.float32))
+    full = tmp_path / "full"
+    full.mkdir()
+    (full / "x.txt").write_text("x")
+    notdir = str(tmp_path / "file")
+    open(notdir, "w").close()
+
+    def fails(args, msg):
+        r = subprocess.run([F2F] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
+
+    out = str(tmp_path / "fixels")
+    fails([fod], "usage")
+    fails([fod, out, "-nii"], "-nii")
+    fails([bad, out], "unexpected number of coefficients")
+    fails([three, out], "not 4D")
+    fails([fod, out, "-mask", msk], "dimensions do not match FOD image")
+    fails([fod, str(full)], "is not empty")
+    fails([fod, notdir], "is not a directory")
+    fails([fod, out, "-bogus"], "unknown option")
+    fails([fod, out, "-fmls", "1"], "several matches possible")
+    fails([fod, out, "-fmls_lobe_merge_ratio", "1.5"], "out of bounds")
+    fails([fod, out, "-maxnum", "0"], "out of bounds")
+    fails([fod, out, "-maxnum", "1.5"], "not an integer")
+    fails([fod, out, "-fmls_integral"], "no value supplied")
+    assert not os.path.exists(out)
