@@ -33,7 +33,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 (vector)
 CPU_SAMPLE = {"single_shell_brain": 131072, "super_resolved": 16384, "multi_shell": 4096, "peaks": 65536,
-              "response": 8192, "gt_energy": 4096, "fixel": 8192}
+              "response": 8192, "gt_energy": 4096, "fixel": 8192, "tournier": 16384}
 
 
 def csd_flops(n_sh, n_init, n_dw, n_hr, stats):
@@ -810,6 +810,113 @@ def bench_fixel(args, torch, ph):
     return res
 
 
+def bench_tournier(args, ph):
+    """dwi2response tournier on the whole-brain phantom, run as a user would: the DWI (72 volumes,
+    646,128 masked voxels of 1,179,648) and the mask written as .mif images, the command run as a
+    child process (-number 300 -iter_voxels 3000 -dilate 1 -max_iters 10, the script's defaults).
+    A step is one whole estimation; the command reports each iteration's GPU CSD and FMLS times
+    (HIP events) and wall time."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from mrtrix3_amd import mif
+    from oracle import tournier_oracle as T
+    exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "dwi2response")
+    if not os.path.exists(exe):
+        raise RuntimeError("dwi2response is not built: run __graft_entry__.build()")
+    work = tempfile.mkdtemp(prefix="mrcsd_tournier_")
+    try:
+        nx, ny, nz = ph.shape
+        vol = np.zeros((nz, ny, nx, ph.dw.shape[1]), dtype=np.float32)
+        vol[ph.mask] = ph.dw
+        dwi, msk = os.path.join(work, "dwi.mif"), os.path.join(work, "mask.mif")
+        mif.write(dwi, vol.transpose(2, 1, 0, 3), vox=[2.0, 2.0, 2.0], keyval=[("dw_scheme", mif.scheme_text(ph.grad))])
+        del vol
+        mif.write(msk, ph.mask.transpose(2, 1, 0).astype(np.float32), vox=[2.0, 2.0, 2.0])
+        runs = []
+        for i in range(args.warmup + args.steps):
+            out = os.path.join(work, "resp%d.txt" % i)
+            print("[bench] dwi2response tournier run %d" % i, file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "tournier", dwi, out, "-mask", msk, "-info"], capture_output=True, text=True,
+                               timeout=900)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError("dwi2response failed: " + r.stderr[-2000:])
+            its = []
+            for m in re.finditer(r"iteration (\d+): (\d+) voxels in the mask; CSD ([\d.]+) ms, FMLS ([\d.]+) ms \(GPU\); "
+                                 r"(\d+) voxels selected; iteration ([\d.]+) s", r.stderr):
+                its.append({"iteration": int(m.group(1)), "voxels": int(m.group(2)), "csd_ms": float(m.group(3)),
+                            "fmls_ms": float(m.group(4)), "selected": int(m.group(5)), "wall_s": float(m.group(6)),
+                            "voxels_per_s": int(m.group(2)) / float(m.group(6))})
+            runs.append({"wall_s": wall, "iterations": its, "response": open(out).read().split(),
+                         "converged": "Convergence of SF voxel selection" in r.stderr})
+        timed = runs[args.warmup:]
+        loop_s = float(np.mean([sum(x["wall_s"] for x in t["iterations"]) for t in timed]))
+        wall = float(np.mean([t["wall_s"] for t in timed]))
+        it0 = timed[-1]["iterations"][0]
+        res = {
+            "metric": "dwi2response tournier: brain-mask voxels per second of the whole estimation on 1 MI355X",
+            "value": ph.nvox / loop_s,
+            "unit": "voxels/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": loop_s * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "dwi2response tournier on the single_shell_brain phantom (646,128 masked voxels, "
+                                   "8 b=0 + 64 b=3000), -number 300 -iter_voxels 3000 -dilate 1 -max_iters 10; "
+                                   "the DWI read once, resident on the GPU across iterations"},
+            "roofline": {"bound": "latency", "achieved": None, "peak": None, "unit": None, "frac": None,
+                         "traffic": None, "kernel": "fmls_segment_kernel<4> (iteration 0, the whole mask)",
+                         "kernel_ms": it0["fmls_ms"],
+                         "note": "iteration 0 segments every masked voxel's lmax-4 FOD and dominates; its roofline is "
+                                 "the fod2fixel workload's (latency- and divergence-bound, one lane per voxel); the "
+                                 "later iterations see the dilated top 3000 voxels only"},
+            "iterations": timed[-1]["iterations"],
+            "converged": timed[-1]["converged"],
+            "response": [float(x) for x in timed[-1]["response"]],
+            "command_wall_s": wall,
+            "note": "value = masked voxels / the iterations' wall time (GPU CSD, GPU FMLS, host selection, GPU "
+                    "amp2response accumulation and solve, host transfers of the per-voxel summary); "
+                    "command_wall_s adds reading the 340 MB .mif and HIP initialisation",
+        }
+        if not args.no_cpu:
+            sample = args.cpu_sample or CPU_SAMPLE["tournier"]
+            # a centred block of the brain with the same protocol, the loop's defaults scaled to it
+            side = int(round(sample ** (1.0 / 3.0)))
+            bx, by, bz = side, side, max(1, sample // (side * side))
+            x0, y0, z0 = (nx - bx) // 2, (ny - by) // 2, (nz - bz) // 2
+            full = np.zeros((nz, ny, nx, ph.dw.shape[1]), dtype=np.float32)
+            full[ph.mask] = ph.dw
+            blk = full[z0:z0 + bz, y0:y0 + by, x0:x0 + bx]
+            del full
+            bm = ph.mask[z0:z0 + bz, y0:y0 + by, x0:x0 + bx].reshape(-1)
+            keep = ph.grad[:, 3] > 10
+            dwb = np.ascontiguousarray(blk.reshape(-1, ph.dw.shape[1])[:, keep])
+            nb = int(bm.sum())
+            number = max(1, 300 * nb // ph.nvox + 1)
+            print("[bench] CPU baseline: oracle loop on a %dx%dx%d block (%d voxels)" % (bx, by, bz, nb),
+                  file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            _, _, n_it, conv, _ = T.run(dwb, ph.grad[keep], (bx, by, bz), bm, number=number, iter_voxels=10 * number,
+                                        nthreads=1)
+            cpu_s = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": nb / cpu_s, "unit": "voxels/s", "cores": 1, "kind": "port",
+                                   "sample": "centred %dx%dx%d block of the phantom (%d voxels), -number %d "
+                                             "-iter_voxels %d, %d iterations; oracle loop (oracle/tournier_oracle.py: "
+                                             "C CSD and FMLS, numpy amp2response), one thread" % (
+                                                 bx, by, bz, nb, number, 10 * number, n_it)}
+        return res
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def resp_flops_per_sample(lmax):
     """FP64 FLOPs of one (voxel, direction) step of the amp2response accumulation
     (cmd/amp2response.cpp:152-191 as the kernel restates it): the dot product with the fibre
@@ -1013,7 +1120,7 @@ def main():
     elif args.workload == "peaks":
         ph = phantom.single_shell_brain()
         e2e = None if args.no_e2e else end_to_end_peaks(ph)
-    elif args.workload in ("response", "gt_energy", "fixel"):
+    elif args.workload in ("response", "gt_energy", "fixel", "tournier"):
         ph = phantom.single_shell_brain()
         e2e = None
     else:
@@ -1025,6 +1132,10 @@ def main():
             # (tests/golden/warm_net_single_shell_brain.npz, written with --save-net)
             e2e_warm = end_to_end_cli(ph, "csd", ["-lmax", str(lmax), "-warm_net", PROTOCOL_NET])
 
+    if args.workload == "tournier":
+        # the command is the whole workload, run as child processes: this process never uses the GPU
+        print(json.dumps(bench_tournier(args, ph)))
+        return
     import torch
     if args.workload == "multi_shell":
         out = bench_msmt(args, torch, ph)

@@ -109,7 +109,8 @@ def build(verbose=False):
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, so)
     for name, extra in (("dwi2fod", []), ("sh2peaks", []), ("fod2fixel", []),
-                        ("amp2response", ["gradient.cpp", "dense.cpp"])):
+                        ("amp2response", ["gradient.cpp", "dense.cpp"]),
+                        ("dwi2response", ["gradient.cpp", "dense.cpp"])):
         cli = os.path.join(CSRC, name + ".cpp")
         exe = os.path.join(BIN, name)
         host = [os.path.join(CSRC, "host", f) for f in ["mif.cpp", "nifti.cpp"] + extra]

@@ -182,3 +182,46 @@ def test_fod2fixel_argument_errors(tmp_path):
     fails([fod, out, "-maxnum", "1.5"], "not an integer")
     fails([fod, out, "-fmls_integral"], "no value supplied")
     assert not os.path.exists(out)
+
+
+D2R = os.path.join(ROOT, "mrtrix3_amd", "bin", "dwi2response")
+
+
+@pytest.mark.skipif(not os.path.exists(D2R), reason="dwi2response not built")
+def test_dwi2response_argument_errors(tmp_path):
+    """dwi2response tournier's checks before any GPU work (bin/dwi2response execute(),
+    tournier.py execute(), dwiextract's shell selection)."""
+    scheme = "\n".join(["0,0,1,0"] * 2 + ["%r,%r,0,3000" % (float(np.cos(a)), float(np.sin(a))) for a in np.linspace(0, 3, 30)])
+    dwi = str(tmp_path / "dwi.mif")
+    mif.write(dwi, np.ones((3, 3, 2, 32), dtype=np.float32), keyval=[("dw_scheme", scheme)])
+    nograd = str(tmp_path / "nograd.mif")
+    mif.write(nograd, np.ones((3, 3, 2, 32), dtype=np.float32))
+    m_ok = str(tmp_path / "m.mif")
+    mif.write(m_ok, np.ones((3, 3, 2), dtype=np.float32))
+    m_bad = str(tmp_path / "mbad.mif")
+    mif.write(m_bad, np.ones((3, 2, 2), dtype=np.float32))
+    m_empty = str(tmp_path / "m0.mif")
+    mif.write(m_empty, np.zeros((3, 3, 2), dtype=np.float32))
+    exists = str(tmp_path / "exists.txt")
+    open(exists, "w").close()
+    out = str(tmp_path / "r.txt")
+
+    def fails(args, msg):
+        r = subprocess.run([D2R] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
+
+    fails(["tournier", dwi], "usage")
+    fails(["dhollander", dwi, out], "not part of this build")
+    fails(["tournier", dwi, exists, "-mask", m_ok], "already exists")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-lmax", "7"], "Parameter lmax must be a number")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-lmax", "4,6"], "single lmax value")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-shells", "1000,3000"], "single b-value shell")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-shells", "x"], "comma-separated list of b-values")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-max_iters", "1"], "at least 2")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-number", "10", "-iter_voxels", "5"], "-iter_voxels")
+    fails(["tournier", nograd, out, "-mask", m_ok], "requires diffusion gradient table")
+    fails(["tournier", dwi, out], "-mask is required")
+    fails(["tournier", dwi, out, "-mask", m_bad], "do not match DWI")
+    fails(["tournier", dwi, out, "-mask", m_empty], "does not contain any voxels")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-shells", "0"], "User selected 0 non b=0 shells")
+    assert not os.path.exists(out)
