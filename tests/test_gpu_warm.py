@@ -157,7 +157,10 @@ def test_warm_solve_matches_plain_on_held_out_voxels(torch_cuda, trained):
     say("[gpu warm] held-out: max_rel=%.3e same_set=%.5f solves/voxel plain %.3f warm %.3f fallback %.4f"
         % (rel.max(), same.mean(), sp[1] / sp[0], sw[1] / sw[0], sw[4] / sw[0]))
     assert np.mean((rel <= 1e-4) & same) >= 0.999
-    assert sw[1] < sp[1]
+    # the ablation: this small protocol's network (about 49k training voxels) saves about 12% of
+    # the solves (measured 4.03 against 4.57 per voxel); the whole-brain protocol's network saves
+    # 44% (test_gpu_warm_c5.py)
+    assert sw[1] <= 0.95 * sp[1]
 
 
 def test_warm_start_refused_where_the_certificate_does_not_hold(torch_cuda, trained):
