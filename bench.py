@@ -821,7 +821,7 @@ def bench_tournier(args, ph):
     import subprocess
     import tempfile
     from mrtrix3_amd import mif
-    from oracle import tournier_oracle as T
+    from oracle import dwi2response_oracle as T
     exe = os.path.join(ROOT, "mrtrix3_amd", "bin", "dwi2response")
     if not os.path.exists(exe):
         raise RuntimeError("dwi2response is not built: run __graft_entry__.build()")
@@ -909,7 +909,7 @@ def bench_tournier(args, ph):
             cpu_s = time.perf_counter() - t0
             res["cpu_baseline"] = {"value": nb / cpu_s, "unit": "voxels/s", "cores": 1, "kind": "port",
                                    "sample": "centred %dx%dx%d block of the phantom (%d voxels), -number %d "
-                                             "-iter_voxels %d, %d iterations; oracle loop (oracle/tournier_oracle.py: "
+                                             "-iter_voxels %d, %d iterations; oracle loop (oracle/dwi2response_oracle.py: "
                                              "C CSD and FMLS, numpy amp2response), one thread" % (
                                                  bx, by, bz, nb, number, 10 * number, n_it)}
         return res

@@ -1,30 +1,39 @@
-// dwi2response tournier on the GPU: the script's arguments, checks and iteration
-// (bin/dwi2response, lib/mrtrix3/dwi2response/tournier.py:26-154) as one process that keeps the
-// DW signal and the FODs on the device between iterations.  Each iteration runs, through
-// libmrcsd.so, what the script runs as separate commands:
+// dwi2response tournier and tax on the GPU: the scripts' arguments, checks and iterations
+// (bin/dwi2response, lib/mrtrix3/dwi2response/tournier.py:26-154, tax.py:26-149) as one process
+// that keeps the DW signal on the device between iterations.  Each iteration runs, through
+// libmrcsd.so, what the scripts run as separate commands:
 //
 //   dwi2fod csd dwi RF FOD -mask M            mrcsd_run over the voxels of M (include/mrcsd.h)
 //   fod2fixel FOD fixel -peak peaks -mask M   mrfixel_segment with no thresholds, its per-voxel
 //     -fmls_no_thresholds                     summary (include/mrfixel.h) standing in for
-//   fixel2voxel ... none -number 2            the first two fixels' peak amplitudes and
+//   fixel2voxel ... none [-number 2]          the first two fixels' peak amplitudes and
 //   fixel2peaks ... -number 1                 the first fixel's direction
+//   amp2response dwi SF dir RF [-lmax]        mrresp_accumulate + mrresp_solve (include/mrresp.h)
+// tournier:
 //   mrcalc p1 -sqrt 1 p2 p1 -div -sub 2 -pow -mult   the cost, in single precision as mrcalc
 //   mrthreshold CF -top N                     the N-th largest non-NaN cost and all at or above
-//   amp2response dwi SF dir RF [-lmax]        mrresp_accumulate + mrresp_solve (include/mrresp.h)
 //   mrcalc SF iter-1_SF -sub; mrstats max     convergence: no voxel newly selected
 //   mrthreshold CF -top 10N | maskfilter dilate -npass D | mrcalc mask - -mult
 //                                             the next iteration's mask (6-neighbour dilation)
+// tax:
+//   mrstats dwi -mask -allvolumes             the initial response from the signal's mean and
+//                                             standard deviation as mrstats prints them
+//   mrcalc p2 p1 -div; ratio R -lt M -mult    the selection, in single precision
+//   the response's coefficients all within -convergence percent of the previous ones: converged;
+//   else the selection is the next mask
 //
 //   dwi2response tournier input output -mask mask [-voxels image] [-shells b] [-lmax l]
 //                [-number n] [-iter_voxels n] [-dilate n] [-max_iters n] [-grad file]
 //                [-scratch dir] [-nocleanup] [-force] [-quiet] [-info] [-nthreads n]
+//   dwi2response tax input output -mask mask [-voxels image] [-shells b] [-lmax l]
+//                [-peak_ratio r] [-max_iters n] [-convergence c] [-grad file] [...]
 //
 // The input is reduced as `dwiextract -singleshell -no_bzero [-shells b]` does.  -mask is
 // required: the brain mask the script computes otherwise (dwi2mask) is not part of this build.
-// With -scratch dir -nocleanup, each iteration's RF text file and cost, selection and next-mask
-// images are written to dir as the script names them (iter<k>_RF.txt, iter<k>_CF.mif,
-// iter<k>_SF.mif, iter<k>_SF_dilated.mif).  Response files are written with 17 significant
-// digits and read back exactly.
+// With -scratch dir -nocleanup, each iteration's RF text file and selection images are written to
+// dir as the scripts name them (iter<k>_RF.txt, iter<k>_SF.mif, and iter<k>_CF.mif,
+// iter<k>_SF_dilated.mif for tournier, iter<k>_peak_ratio.mif for tax).  Response files are
+// written with 17 significant digits and read back exactly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -128,13 +137,26 @@ std::vector<std::string> split(const std::string& s, char sep) {
   out.push_back(cur);
   return out;
 }
-
 struct Args {
   std::vector<std::string> pos;
   std::string mask, voxels, shells, lmax, grad, scratch;
-  int number = 300, iter_voxels = 0, dilate = 1, max_iters = 10;
+  int number = 300, iter_voxels = 0, dilate = 1, max_iters = -1;   // max_iters: 10 (tournier), 20 (tax)
+  double peak_ratio = 0.1, convergence = 0.5;
+  std::vector<std::string> given;   // algorithm-specific options given
   bool force = false, nocleanup = false;
 };
+
+double parse_float(const std::string& opt, const std::string& v) {
+  size_t used = 0;
+  double x = 0.0;
+  try {
+    x = std::stod(v, &used);
+  } catch (const std::exception&) {
+    used = 0;
+  }
+  if (used != v.size()) throw std::runtime_error("argument -" + opt + ": invalid float value: '" + v + "'");
+  return x;
+}
 
 Args parse_args(int argc, char** argv) {
   Args a;
@@ -152,10 +174,12 @@ Args parse_args(int argc, char** argv) {
       else if (o == "lmax") a.lmax = next();
       else if (o == "grad") a.grad = next();
       else if (o == "fslgrad") throw std::runtime_error("-fslgrad: give the gradient table with -grad or in the image header");
-      else if (o == "number") a.number = parse_int(o, next());
-      else if (o == "iter_voxels") a.iter_voxels = parse_int(o, next());
-      else if (o == "dilate") a.dilate = parse_int(o, next());
+      else if (o == "number") { a.number = parse_int(o, next()); a.given.push_back(o); }
+      else if (o == "iter_voxels") { a.iter_voxels = parse_int(o, next()); a.given.push_back(o); }
+      else if (o == "dilate") { a.dilate = parse_int(o, next()); a.given.push_back(o); }
       else if (o == "max_iters") a.max_iters = parse_int(o, next());
+      else if (o == "peak_ratio") { a.peak_ratio = parse_float(o, next()); a.given.push_back(o); }
+      else if (o == "convergence") { a.convergence = parse_float(o, next()); a.given.push_back(o); }
       else if (o == "scratch") a.scratch = next();
       else if (o == "nocleanup") a.nocleanup = true;
       else if (o == "force") a.force = true;
@@ -169,6 +193,7 @@ Args parse_args(int argc, char** argv) {
   }
   return a;
 }
+
 
 template <class T>
 struct Dev {
@@ -243,13 +268,170 @@ void write_mask(const std::string& path, const mrcsd::MifHeader& h3, const std::
   mrcsd::write_mif(path, h3, d, false);
 }
 
+// ostream's default formatting of a double (6 significant digits), read back: how the script sees
+// mrstats' mean and standard deviation (core/stats.h:92-98, str<cdouble>, core/mrtrix.h:353-363)
+double printed6(double x) {
+  char buf[48];
+  std::snprintf(buf, sizeof buf, "%g", x);
+  return std::strtod(buf, nullptr);
+}
+
+// The GPU state of one run: the single-shell rows of the grid on the device, and one iteration's
+// analysis of the voxels of a mask -- dwi2fod csd, fod2fixel -fmls_no_thresholds and the first
+// two fixels' peak amplitudes and the first fixel's direction -- then amp2response over a
+// selection of those voxels.
+class Engine {
+ public:
+  Engine(const std::vector<float>& rows, int nvol, const std::vector<double>& grad_flat, const mrcsd::Matrix& grad,
+         size_t nmask)
+      : nvol_(nvol), grad_(grad_flat), d_rows_(rows.size()), d_in_(nmask * size_t(nvol)), d_summary_(nmask * 5),
+        d_idx_(nmask), d_sf_(nmask), d_count_(nmask), d_offset_(nmask), d_total_(1), summary(nmask * 5) {
+    hip_check(hipStreamCreate(&st_));
+    hip_check(hipMemcpyAsync(d_rows_.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, st_));
+    // amp2response's shell for dwi.mif: the DW directions of the selected volumes (amp2response.cpp
+    // :207-239, spherical2cartesian of gen_direction_matrix)
+    std::vector<size_t> all(static_cast<size_t>(nvol));
+    for (int i = 0; i < nvol; ++i) all[size_t(i)] = size_t(i);
+    const mrcsd::Matrix azel = MR::DWI::gen_direction_matrix(grad, all);
+    cart_.resize(size_t(nvol) * 3);
+    for (int i = 0; i < nvol; ++i) {
+      const double az = azel(i, 0), el = azel(i, 1);
+      cart_[size_t(3 * i)] = std::sin(el) * std::cos(az);
+      cart_[size_t(3 * i + 1)] = std::sin(el) * std::sin(az);
+      cart_[size_t(3 * i + 2)] = std::cos(el);
+    }
+    vols_.resize(static_cast<size_t>(nvol));
+    for (int i = 0; i < nvol; ++i) vols_[size_t(i)] = i;
+  }
+  ~Engine() {
+    for (int i = 0; i < 2; ++i)
+      if (seg_[i]) mrfixel_destroy(seg_[i]);
+    if (st_) (void)hipStreamDestroy(st_);
+  }
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // the voxels of mask_in (packed in idx order) after CSD with response rf and segmentation:
+  // summary [n][5] = first two fixels' peak amplitudes (0 when absent), first fixel's direction
+  void analyse(const std::vector<uint8_t>& mask_in, const std::vector<double>& rf) {
+    idx.clear();
+    for (size_t v = 0; v < mask_in.size(); ++v)
+      if (mask_in[v]) idx.push_back(int(v));
+    const size_t n = idx.size();
+    if (!n) throw std::runtime_error("no voxel left in the mask to test");
+    hip_check(hipMemcpyAsync(d_idx_.p, idx.data(), n * sizeof(int), hipMemcpyHostToDevice, st_));
+    const size_t threads = n * size_t(nvol_);
+    gather_rows<<<dim3(unsigned((threads + 255) / 256)), dim3(256), 0, st_>>>(d_rows_.p, nvol_, d_idx_.p, n, d_in_.p);
+    hip_check(hipGetLastError());
+
+    // dwi2fod csd dwi.mif RF FOD.mif -mask mask_in
+    mrcsd_options copt;
+    mrcsd_options_default(&copt);
+    mrcsd_shared* shared = nullptr;
+    check(mrcsd_shared_create(grad_.data(), nvol_, rf.data(), int(rf.size()), &copt, &shared));
+    mrcsd_shared_info si;
+    const int rc_info = mrcsd_shared_info_get(shared, &si);
+    mrcsd_ctx* ctx = nullptr;
+    const int rc_ctx = rc_info ? rc_info : mrcsd_create_from_shared(shared, &ctx);
+    mrcsd_shared_destroy(shared);
+    check(rc_ctx);
+    Dev<float> d_fod(n * static_cast<size_t>(si.n_sh));
+    hipEvent_t e[3];
+    for (auto& x : e) hip_check(hipEventCreate(&x));
+    hip_check(hipEventRecord(e[0], st_));
+    const int rc_run = mrcsd_run(ctx, d_in_.p, nvol_, nullptr, n, d_fod.p, nullptr, nullptr, st_);
+    mrcsd_destroy(ctx);
+    check(rc_run);
+    hip_check(hipEventRecord(e[1], st_));
+    // fod2fixel FOD fixel -peak peaks -mask mask_in -fmls_no_thresholds; fixel2voxel / fixel2peaks
+    // (one segmenter per FOD size, kept with its work arrays for the later iterations)
+    const int slot = seg_nsh_[0] == si.n_sh ? 0 : seg_nsh_[1] == si.n_sh ? 1 : !seg_[0] ? 0 : 1;
+    if (seg_nsh_[slot] != si.n_sh) {
+      if (seg_[slot]) mrfixel_destroy(seg_[slot]);
+      seg_[slot] = nullptr;
+      seg_nsh_[slot] = -1;
+      mrfixel_options fopt;
+      mrfixel_options_default(&fopt);
+      fopt.n_sh = si.n_sh;
+      fopt.integral_threshold = 0.0;
+      fopt.peak_value_threshold = 0.0;
+      check(mrfixel_create(&fopt, &seg_[slot]));
+      seg_nsh_[slot] = si.n_sh;
+    }
+    check(mrfixel_segment(seg_[slot], d_fod.p, si.n_sh, nullptr, n, d_count_.p, d_offset_.p, nullptr, 0,
+                          d_summary_.p, reinterpret_cast<uint64_t*>(d_total_.p), st_));
+    hip_check(hipEventRecord(e[2], st_));
+    hip_check(hipMemcpyAsync(summary.data(), d_summary_.p, n * 5 * sizeof(float), hipMemcpyDeviceToHost, st_));
+    hip_check(hipStreamSynchronize(st_));
+    hip_check(hipEventElapsedTime(&ms_csd, e[0], e[1]));
+    hip_check(hipEventElapsedTime(&ms_fmls, e[1], e[2]));
+    for (auto& x : e) (void)hipEventDestroy(x);
+  }
+
+  // amp2response dwi.mif SF.mif first_dir.mif RF.txt [-lmax l] over the voxels of the last
+  // analysis that sf selects (sf over the grid, inside the analysed mask)
+  std::vector<double> response(const std::vector<uint8_t>& sf, int lmax) {
+    if (lmax < 0 || lmax > 12) throw std::runtime_error("lmax must be in [0, 12] on the GPU");
+    const size_t n = idx.size();
+    std::vector<uint8_t> sel(n);
+    size_t nsel = 0;
+    for (size_t i = 0; i < n; ++i) nsel += (sel[i] = sf[size_t(idx[i])]);
+    if (!nsel) throw std::runtime_error("input mask does not contain any voxels");
+    hip_check(hipMemcpyAsync(d_sf_.p, sel.data(), n, hipMemcpyHostToDevice, st_));
+    mrresp_ctx* rctx = nullptr;
+    check(mrresp_create(cart_.data(), vols_.data(), nvol_, lmax, &rctx));
+    const int nc = mrresp_ncoef(rctx);
+    std::vector<double> M(static_cast<size_t>(nc * nc)), b(static_cast<size_t>(nc)), rf(static_cast<size_t>(nc));
+    uint64_t count = 0;
+    const int rc = mrresp_accumulate(rctx, d_in_.p, nvol_, d_summary_.p + 2, 5, d_sf_.p, n, M.data(), b.data(),
+                                     &count, st_);
+    mrresp_destroy(rctx);
+    check(rc);
+    int iters = 0;
+    check(mrresp_solve(lmax, M.data(), b.data(), 1, rf.data(), &iters));
+    return rf;
+  }
+
+  std::vector<int> idx;          // grid indices of the last analysis' voxels
+  std::vector<float> summary;    // [idx.size()][5]
+  float ms_csd = 0.0f, ms_fmls = 0.0f;
+
+ private:
+  int nvol_;
+  std::vector<double> grad_, cart_;
+  std::vector<int> vols_;
+  hipStream_t st_ = nullptr;
+  Dev<float> d_rows_, d_in_, d_summary_;
+  Dev<int> d_idx_;
+  Dev<uint8_t> d_sf_;
+  Dev<uint32_t> d_count_, d_offset_;
+  Dev<uint64_t> d_total_;
+  mrfixel_ctx* seg_[2] = {nullptr, nullptr};
+  int seg_nsh_[2] = {-1, -1};
+};
+
+std::string rf_text(const std::vector<double>& rf) {
+  std::ostringstream line;
+  for (size_t i = 0; i < rf.size(); ++i) {
+    char buf[32];
+    std::snprintf(buf, sizeof buf, "%.3f", rf[i]);
+    line << (i ? ", " : "") << buf;
+  }
+  return line.str();
+}
+
 int run(int argc, char** argv) {
   const auto t_start = std::chrono::steady_clock::now();
   Args a = parse_args(argc, argv);
   if (a.pos.empty()) throw std::runtime_error("usage: dwi2response algorithm input output [options]");
-  if (a.pos[0] != "tournier")
-    throw std::runtime_error("algorithm \"" + a.pos[0] + "\" is not part of this build (available: tournier)");
-  if (a.pos.size() != 3) throw std::runtime_error("usage: dwi2response tournier input output [options]");
+  const std::string algo = a.pos[0];
+  if (algo != "tournier" && algo != "tax")
+    throw std::runtime_error("algorithm \"" + algo + "\" is not part of this build (available: tax, tournier)");
+  for (const auto& o : a.given) {
+    const bool tournier_opt = o == "number" || o == "iter_voxels" || o == "dilate";
+    if ((algo == "tax") == tournier_opt) throw std::runtime_error("unrecognized arguments: -" + o);
+  }
+  if (a.pos.size() != 3) throw std::runtime_error("usage: dwi2response " + algo + " input output [options]");
   const std::string& input = a.pos[1];
   const std::string& output = a.pos[2];
   if (exists(output) && !a.force)
@@ -286,12 +468,14 @@ int run(int argc, char** argv) {
     }
     if (shells.size() != 1) throw std::runtime_error("Can only specify a single b-value shell for single-shell algorithms");
   }
-  // tournier.py execute(): iteration options
-  if (a.max_iters < 2) throw std::runtime_error("Number of iterations must be at least 2");
+  const int max_iters = a.max_iters >= 0 ? a.max_iters : (algo == "tax" ? 20 : 10);
   int iter_voxels = a.iter_voxels;
-  if (iter_voxels == 0) iter_voxels = 10 * a.number;
-  else if (iter_voxels < a.number)
-    throw std::runtime_error("Number of selected voxels (-iter_voxels) must be greater than number of voxels desired (-number)");
+  if (algo == "tournier") {   // tournier.py execute()
+    if (max_iters < 2) throw std::runtime_error("Number of iterations must be at least 2");
+    if (iter_voxels == 0) iter_voxels = 10 * a.number;
+    else if (iter_voxels < a.number)
+      throw std::runtime_error("Number of selected voxels (-iter_voxels) must be greater than number of voxels desired (-number)");
+  }
 
   mrcsd::MifImage dwi = mrcsd::read_image(input);
   if (dwi.h.dim.size() != 4) throw std::runtime_error("input DWI must be a 4-D image");
@@ -315,11 +499,9 @@ int run(int argc, char** argv) {
     for (size_t v : sh[s].get_volumes()) vols.push_back(v);
   const int nvol = int(vols.size());
   mrcsd::Matrix grad(nvol, 4);
-  for (int i = 0; i < nvol; ++i)
-    for (int j = 0; j < 4; ++j) grad(i, j) = grad_in(int(vols[size_t(i)]), j);
   std::vector<double> grad_flat(size_t(nvol) * 4);
   for (int i = 0; i < nvol; ++i)
-    for (int j = 0; j < 4; ++j) grad_flat[size_t(4 * i + j)] = grad(i, j);
+    for (int j = 0; j < 4; ++j) grad_flat[size_t(4 * i + j)] = grad(i, j) = grad_in(int(vols[size_t(i)]), j);
 
   const int64_t* dim = dwi.h.dim.data();
   const int64_t nspatial = dim[0] * dim[1] * dim[2];
@@ -330,234 +512,171 @@ int run(int argc, char** argv) {
     throw std::runtime_error("Dimensions of provided mask image do not match DWI");
   if (!(mimg.h.dim.size() == 3 || (mimg.h.dim.size() == 4 && mimg.h.dim[3] == 1)))
     throw std::runtime_error("Provided mask image needs to be a 3D image");
-  std::vector<uint8_t> mask(static_cast<size_t>(nspatial));
+  const size_t ns = static_cast<size_t>(nspatial);
+  std::vector<uint8_t> mask(ns);
   size_t nmask = 0;
-  for (int64_t v = 0; v < nspatial; ++v) nmask += (mask[size_t(v)] = mimg.data[size_t(v)] != 0.0f);
+  for (size_t v = 0; v < ns; ++v) nmask += (mask[v] = mimg.data[v] != 0.0f);
   if (!nmask) throw std::runtime_error("Provided mask image does not contain any voxels");
   mrcsd::MifHeader h3 = dwi.h;
   h3.dim.resize(3);
   if (h3.vox.size() > 3) h3.vox.resize(3);
   for (const char* k : {"dw_scheme", "pe_scheme", "PhaseEncodingDirection", "TotalReadoutTime"}) h3.erase(k);
 
-  // the single-shell signal as voxel rows on the device, for every iteration
-  std::vector<float> rows(size_t(nspatial) * size_t(nvol));
-  for (int64_t v0 = 0; v0 < nspatial; v0 += 256) {
-    const int64_t v1 = std::min(nspatial, v0 + 256);
+  // the single-shell signal as voxel rows, for every iteration
+  std::vector<float> rows(ns * size_t(nvol));
+  for (size_t v0 = 0; v0 < ns; v0 += 256) {
+    const size_t v1 = std::min(ns, v0 + 256);
     for (int i = 0; i < nvol; ++i) {
-      const float* src = &dwi.data[size_t(int64_t(vols[size_t(i)]) * nspatial)];
-      for (int64_t v = v0; v < v1; ++v) rows[size_t(v) * size_t(nvol) + size_t(i)] = src[v];
+      const float* src = &dwi.data[vols[size_t(i)] * ns];
+      for (size_t v = v0; v < v1; ++v) rows[v * size_t(nvol) + size_t(i)] = src[v];
     }
   }
-  hipStream_t st = nullptr;
-  hip_check(hipStreamCreate(&st));
-  // the grid's rows stay on the device; each iteration packs its mask's voxels (a subset of the
-  // initial mask) into the first rows of d_in and runs the CSD, the segmentation and the
-  // response accumulation on those only
-  const size_t ns = static_cast<size_t>(nspatial);
-  Dev<float> d_rows(rows.size()), d_in(nmask * size_t(nvol)), d_summary(nmask * 5);
-  Dev<int> d_idx(nmask);
-  Dev<uint8_t> d_sf(nmask);
-  Dev<uint32_t> d_count(nmask), d_offset(nmask);
-  Dev<uint64_t> d_total(1);
-  hip_check(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice, st));
-
-  // amp2response's shell for dwi.mif: the DW directions of the selected volumes (amp2response.cpp
-  // :207-239, spherical2cartesian of gen_direction_matrix)
-  std::vector<size_t> all(static_cast<size_t>(nvol));
-  for (int i = 0; i < nvol; ++i) all[size_t(i)] = size_t(i);
-  const mrcsd::Matrix azel = MR::DWI::gen_direction_matrix(grad, all);
-  std::vector<double> cart(size_t(nvol) * 3);
-  for (int i = 0; i < nvol; ++i) {
-    const double az = azel(i, 0), el = azel(i, 1);
-    cart[size_t(3 * i)] = std::sin(el) * std::cos(az);
-    cart[size_t(3 * i + 1)] = std::sin(el) * std::sin(az);
-    cart[size_t(3 * i + 2)] = std::cos(el);
-  }
-  std::vector<int> resp_vols(static_cast<size_t>(nvol));
-  for (int i = 0; i < nvol; ++i) resp_vols[size_t(i)] = i;
+  Engine eng(rows, nvol, grad_flat, grad, nmask);
 
   const bool keep = !a.scratch.empty() && a.nocleanup;
   if (keep && !exists(a.scratch) && ::mkdir(a.scratch.c_str(), 0777) != 0)
     throw std::runtime_error("cannot create scratch directory \"" + a.scratch + "\"");
   auto scratch = [&](const std::string& name) { return a.scratch + "/" + name; };
-
-  std::vector<double> rf = {1.0, -1.0, 1.0};   // init_RF.txt "1 -1 1"
-  std::vector<uint8_t> mask_in = mask, sf_prev, sf;
-  std::vector<double> final_rf;
-  std::vector<float> summary(nmask * 5), cf(ns);
-  std::vector<int> idx;
-  std::vector<uint8_t> sf_in;
-  bool converged = false;
-  int iteration = 0;
-  // segmenters of the FOD sizes met (lmax 4 in the first iteration, the CSD default after)
-  mrfixel_ctx* seg[2] = {nullptr, nullptr};
-  int seg_nsh[2] = {-1, -1};
-  struct SegFree {
-    mrfixel_ctx** s;
-    ~SegFree() {
-      for (int i = 0; i < 2; ++i)
-        if (s[i]) mrfixel_destroy(s[i]);
-    }
-  } seg_free{seg};
-  double fod_ms_total = 0.0;
-  for (; iteration < a.max_iters; ++iteration) {
-    const auto t_it = std::chrono::steady_clock::now();
-    const int amp_lmax = iteration == 0 ? 4 : (lmax.empty() ? -1 : lmax[0]);
-    idx.clear();
-    for (int64_t v = 0; v < nspatial; ++v)
-      if (mask_in[size_t(v)]) idx.push_back(int(v));
-    const size_t n_in = idx.size();
-    if (!n_in) throw std::runtime_error("no voxel left in the mask to test");
-    hip_check(hipMemcpyAsync(d_idx.p, idx.data(), n_in * sizeof(int), hipMemcpyHostToDevice, st));
-    {
-      const size_t threads = n_in * size_t(nvol);
-      gather_rows<<<dim3(unsigned((threads + 255) / 256)), dim3(256), 0, st>>>(d_rows.p, nvol, d_idx.p, n_in, d_in.p);
-      hip_check(hipGetLastError());
-    }
-
-    // dwi2fod csd dwi.mif RF FOD.mif -mask mask_in
-    mrcsd_options copt;
-    mrcsd_options_default(&copt);
-    mrcsd_shared* shared = nullptr;
-    check(mrcsd_shared_create(grad_flat.data(), nvol, rf.data(), int(rf.size()), &copt, &shared));
-    mrcsd_shared_info si;
-    check(mrcsd_shared_info_get(shared, &si));
-    mrcsd_ctx* ctx = nullptr;
-    const int rc_ctx = mrcsd_create_from_shared(shared, &ctx);
-    mrcsd_shared_destroy(shared);
-    check(rc_ctx);
-    Dev<float> d_fod(n_in * static_cast<size_t>(si.n_sh));
-    hipEvent_t e0, e1, e2;
-    hip_check(hipEventCreate(&e0));
-    hip_check(hipEventCreate(&e1));
-    hip_check(hipEventCreate(&e2));
-    hip_check(hipEventRecord(e0, st));
-    const int rc_run = mrcsd_run(ctx, d_in.p, nvol, nullptr, n_in, d_fod.p, nullptr, nullptr, st);
-    mrcsd_destroy(ctx);
-    check(rc_run);
-    hip_check(hipEventRecord(e1, st));
-
-    // fod2fixel FOD fixel -peak peaks -mask mask_in -fmls_no_thresholds; fixel2voxel / fixel2peaks
-    // (one segmenter per FOD size, kept with its work arrays for the later iterations)
-    int slot = seg_nsh[0] == si.n_sh ? 0 : seg_nsh[1] == si.n_sh ? 1 : !seg[0] ? 0 : 1;
-    if (seg_nsh[slot] != si.n_sh) {
-      if (seg[slot]) mrfixel_destroy(seg[slot]);
-      seg[slot] = nullptr;
-      seg_nsh[slot] = -1;
-      mrfixel_options fopt;
-      mrfixel_options_default(&fopt);
-      fopt.n_sh = si.n_sh;
-      fopt.integral_threshold = 0.0;
-      fopt.peak_value_threshold = 0.0;
-      check(mrfixel_create(&fopt, &seg[slot]));
-      seg_nsh[slot] = si.n_sh;
-    }
-    mrfixel_ctx* fctx = seg[slot];
-    check(mrfixel_segment(fctx, d_fod.p, si.n_sh, nullptr, n_in, d_count.p, d_offset.p, nullptr, 0, d_summary.p,
-                          reinterpret_cast<uint64_t*>(d_total.p), st));
-    hip_check(hipEventRecord(e2, st));
-    hip_check(hipMemcpyAsync(summary.data(), d_summary.p, n_in * 5 * sizeof(float), hipMemcpyDeviceToHost, st));
-    hip_check(hipStreamSynchronize(st));
-    float ms_csd = 0.0f, ms_fix = 0.0f;
-    hip_check(hipEventElapsedTime(&ms_csd, e0, e1));
-    hip_check(hipEventElapsedTime(&ms_fix, e1, e2));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
-    fod_ms_total += ms_csd + ms_fix;
-
-    // the cost sqrt(|p1|) (1 - |p2|/|p1|)^2 in single precision, as mrcalc evaluates it
-    // (cmd/mrcalc.cpp:103-123, real_type float); NaN where p1 = 0 (no fixel)
-    // (voxels outside this iteration's mask have no fixel: p1 = p2 = 0)
-    std::fill(cf.begin(), cf.end(), std::numeric_limits<float>::quiet_NaN());
-    size_t voxel_count = 0;
-    for (size_t i = 0; i < n_in; ++i) {
-      const float p1 = summary[5 * i], p2 = summary[5 * i + 1];
-      const float r = 1.0f - p2 / p1;
-      const float c = std::sqrt(p1) * std::pow(r, 2.0f);
-      cf[size_t(idx[i])] = c;
-      voxel_count += std::isfinite(c) ? 1 : 0;   // mrstats count: finite values (core/stats.h:55-70)
-    }
-    sf = threshold_top(cf, std::min(size_t(a.number), voxel_count));
-
-    // amp2response dwi.mif SF.mif first_dir.mif RF.txt [-lmax]
-    const int rl = amp_lmax >= 0 ? amp_lmax : 10;   // amp2response's default for a non-b=0 shell
-    if (rl > 12) throw std::runtime_error("lmax must be in [0, 12] on the GPU");
-    size_t nsf = 0;
-    for (uint8_t s : sf) nsf += s;
-    if (!nsf) throw std::runtime_error("input mask does not contain any voxels");
-    sf_in.resize(n_in);   // the selection is inside this iteration's mask (the cost is NaN elsewhere)
-    for (size_t i = 0; i < n_in; ++i) sf_in[i] = sf[size_t(idx[i])];
-    hip_check(hipMemcpyAsync(d_sf.p, sf_in.data(), n_in, hipMemcpyHostToDevice, st));
-    mrresp_ctx* rctx = nullptr;
-    check(mrresp_create(cart.data(), resp_vols.data(), nvol, rl, &rctx));
-    const int n = mrresp_ncoef(rctx);
-    std::vector<double> M(static_cast<size_t>(n * n)), b(static_cast<size_t>(n)), rf_new(static_cast<size_t>(n));
-    uint64_t count = 0;
-    const int rc_acc = mrresp_accumulate(rctx, d_in.p, nvol, d_summary.p + 2, 5, d_sf.p, n_in, M.data(),
-                                         b.data(), &count, st);
-    mrresp_destroy(rctx);
-    check(rc_acc);
-    int solver_iters = 0;
-    check(mrresp_solve(rl, M.data(), b.data(), 1, rf_new.data(), &solver_iters));
-    rf = rf_new;
-
-    {
-      std::ostringstream line;
-      line << "Optimising (" << iteration + 1 << " iterations, RF: [ ";
-      for (size_t i = 0; i < rf.size(); ++i) {
-        char buf[32];
-        std::snprintf(buf, sizeof buf, "%.3f", rf[i]);
-        line << (i ? ", " : "") << buf;
-      }
-      line << "] )";
-      console(line.str());
-    }
-    const std::string prefix = "iter" + std::to_string(iteration) + "_";
-    if (keep) {
-      write_rf(scratch(prefix + "RF.txt"), rf);
-      std::vector<float> cfc = cf;
-      mrcsd::write_mif(scratch(prefix + "CF.mif"), h3, cfc, false);
-      write_mask(scratch(prefix + "SF.mif"), h3, sf);
-    }
-
-    // termination: no voxel newly selected (mrcalc SF prev_SF -sub; mrstats max == 0)
-    bool stop = false;
-    if (iteration > 0) {
-      stop = true;
-      for (size_t v = 0; v < sf.size(); ++v)
-        if (sf[v] && !sf_prev[v]) {
-          stop = false;
-          break;
-        }
-    }
+  auto report = [&](int iteration, const std::chrono::steady_clock::time_point& t_it, size_t nsel) {
     const double it_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
     char buf[260];
     std::snprintf(buf, sizeof buf,
                   "iteration %d: %zu voxels in the mask; CSD %.2f ms, FMLS %.2f ms (GPU); %zu voxels selected; "
                   "iteration %.3f s, %.3g voxels/s",
-                  iteration, n_in, double(ms_csd), double(ms_fix), nsf, it_s, double(n_in) / it_s);
+                  iteration, eng.idx.size(), double(eng.ms_csd), double(eng.ms_fmls), nsel, it_s,
+                  double(eng.idx.size()) / it_s);
     info(buf);
-    if (stop) {
-      converged = true;
-      break;
+  };
+
+  std::vector<double> rf;
+  std::vector<uint8_t> mask_in = mask, sf, sf_prev;
+  std::vector<float> img(ns);
+  bool converged = false;
+  int iteration = 0;
+  double gpu_ms = 0.0;
+  const int amp_lmax = lmax.empty() ? 10 : lmax[0];   // amp2response's default for a non-b=0 shell
+
+  if (algo == "tournier") {
+    rf = {1.0, -1.0, 1.0};   // init_RF.txt "1 -1 1"
+    for (; iteration < max_iters; ++iteration) {
+      const auto t_it = std::chrono::steady_clock::now();
+      eng.analyse(mask_in, rf);
+      gpu_ms += eng.ms_csd + eng.ms_fmls;
+      // the cost sqrt(p1) (1 - p2/p1)^2 in single precision, as mrcalc evaluates it
+      // (cmd/mrcalc.cpp:103-123, real_type float); NaN without a fixel, and outside the mask
+      std::fill(img.begin(), img.end(), std::numeric_limits<float>::quiet_NaN());
+      size_t voxel_count = 0;
+      for (size_t i = 0; i < eng.idx.size(); ++i) {
+        const float p1 = eng.summary[5 * i], p2 = eng.summary[5 * i + 1];
+        const float r = 1.0f - p2 / p1;
+        const float c = std::sqrt(p1) * std::pow(r, 2.0f);
+        img[size_t(eng.idx[i])] = c;
+        voxel_count += std::isfinite(c) ? 1 : 0;   // mrstats count: finite values (core/stats.h:55-70)
+      }
+      sf = threshold_top(img, std::min(size_t(a.number), voxel_count));
+      rf = eng.response(sf, iteration == 0 ? 4 : amp_lmax);
+      console("Optimising (" + std::to_string(iteration + 1) + " iterations, RF: [ " + rf_text(rf) + "] )");
+      const std::string prefix = "iter" + std::to_string(iteration) + "_";
+      if (keep) {
+        write_rf(scratch(prefix + "RF.txt"), rf);
+        mrcsd::write_mif(scratch(prefix + "CF.mif"), h3, img, false);
+        write_mask(scratch(prefix + "SF.mif"), h3, sf);
+      }
+      // termination: no voxel newly selected (mrcalc SF prev_SF -sub; mrstats max == 0)
+      bool stop = iteration > 0;
+      for (size_t v = 0; stop && v < ns; ++v)
+        if (sf[v] && !sf_prev[v]) stop = false;
+      size_t nsf = 0;
+      for (uint8_t s : sf) nsf += s;
+      report(iteration, t_it, nsf);
+      if (stop) {
+        converged = true;
+        break;
+      }
+      // the next iteration's mask: the top iter_voxels, dilated, within the initial mask
+      std::vector<uint8_t> next = dilate(threshold_top(img, std::min(size_t(iter_voxels), voxel_count)), dim, a.dilate);
+      for (size_t v = 0; v < ns; ++v) next[v] = next[v] && mask[v];
+      if (keep) write_mask(scratch(prefix + "SF_dilated.mif"), h3, next);
+      mask_in.swap(next);
+      sf_prev = sf;
     }
-    // the next iteration's mask: the top iter_voxels, dilated, within the initial mask
-    std::vector<uint8_t> next = dilate(threshold_top(cf, std::min(size_t(iter_voxels), voxel_count)), dim, a.dilate);
-    for (size_t v = 0; v < next.size(); ++v) next[v] = next[v] && mask[v];
-    if (keep) write_mask(scratch(prefix + "SF_dilated.mif"), h3, next);
-    mask_in.swap(next);
-    sf_prev = sf;
+    if (converged) console("Convergence of SF voxel selection detected at iteration " + std::to_string(iteration + 1));
+    else console("Exiting after maximum " + std::to_string(max_iters) + " iterations");
+  } else {
+    // tax.py execute(): the initial response from the mean and standard deviation of the DW signal
+    // in the mask (mrstats -allvolumes -mask: finite values, Welford's running mean and variance)
+    double mean = 0.0, m2 = 0.0;
+    size_t count = 0;
+    for (size_t v = 0; v < ns; ++v) {
+      if (!mask[v]) continue;
+      for (int i = 0; i < nvol; ++i) {
+        const double x = rows[v * size_t(nvol) + size_t(i)];
+        if (!std::isfinite(x)) continue;
+        ++count;
+        const double delta = x - mean;
+        mean += delta / double(count);
+        m2 += delta * (x - mean);
+      }
+    }
+    const double std_dev = count > 1 ? std::sqrt(m2 / double(count - 1)) : 0.0;
+    const double s4pi = std::sqrt(4.0 * M_PI);
+    const double m = printed6(mean) * s4pi, s = printed6(std_dev) * s4pi;
+    rf = {m, -0.5 * s, 0.25 * s * s / m};
+    if (keep) write_rf(scratch("init_RF.txt"), rf);
+    const double change = 0.01 * a.convergence;
+    const float thr = float(a.peak_ratio);
+    size_t nsf = 0;
+    for (; iteration < max_iters; ++iteration) {
+      const auto t_it = std::chrono::steady_clock::now();
+      eng.analyse(mask_in, rf);
+      gpu_ms += eng.ms_csd + eng.ms_fmls;
+      // peak_ratio = second / first in single precision; SF = (ratio < peak_ratio) within the mask
+      std::fill(img.begin(), img.end(), std::numeric_limits<float>::quiet_NaN());
+      sf.assign(ns, 0);
+      nsf = 0;
+      for (size_t i = 0; i < eng.idx.size(); ++i) {
+        const float r = eng.summary[5 * i + 1] / eng.summary[5 * i];
+        img[size_t(eng.idx[i])] = r;
+        nsf += (sf[size_t(eng.idx[i])] = r < thr);
+      }
+      if (!nsf) throw std::runtime_error("Aborting: All voxels have been excluded from single-fibre selection");
+      const std::vector<double> rf_new = eng.response(sf, amp_lmax);
+      console("Optimising (" + std::to_string(iteration + 1) + " iterations, " + std::to_string(nsf) +
+              " voxels, RF: [ " + rf_text(rf_new) + "] )");
+      const std::string prefix = "iter" + std::to_string(iteration) + "_";
+      if (keep) {
+        write_rf(scratch(prefix + "RF.txt"), rf_new);
+        mrcsd::write_mif(scratch(prefix + "peak_ratio.mif"), h3, img, false);
+        write_mask(scratch(prefix + "SF.mif"), h3, sf);
+      }
+      // termination: every coefficient's half-difference within `change` of its mean
+      bool reiterate = iteration == 0;
+      for (size_t k = 0; !reiterate && k < std::min(rf.size(), rf_new.size()); ++k) {
+        const double mid = 0.5 * (rf[k] + rf_new[k]);
+        const double diff = std::fabs(0.5 * (rf[k] - rf_new[k]));
+        if (diff / mid > change) reiterate = true;
+      }
+      rf = rf_new;
+      report(iteration, t_it, nsf);
+      if (!reiterate) {
+        converged = true;
+        break;
+      }
+      mask_in = sf;
+    }
+    if (converged)
+      console("Exited at iteration " + std::to_string(iteration + 1) + " with " + std::to_string(nsf) +
+              " SF voxels due to unchanged RF coefficients");
+    else
+      console("Exited after maximum " + std::to_string(max_iters) + " iterations with " + std::to_string(nsf) +
+              " SF voxels");
   }
-  if (converged) console("Convergence of SF voxel selection detected at iteration " + std::to_string(iteration + 1));
-  else console("Exiting after maximum " + std::to_string(a.max_iters) + " iterations");
 
   write_rf(output, rf);
   if (!a.voxels.empty()) write_mask(a.voxels, h3, sf);
-  hip_check(hipStreamDestroy(st));
   char buf[160];
   std::snprintf(buf, sizeof buf, "timing: %d iterations, GPU CSD + FMLS %.1f ms, total %.3f s",
-                converged ? iteration + 1 : a.max_iters, fod_ms_total,
+                converged ? iteration + 1 : max_iters, gpu_ms,
                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count());
   info(buf);
   return 0;

@@ -1,6 +1,6 @@
 """Test infrastructure only (imported by tests/ and bench.py's CPU leg, never by mrtrix3_amd/): the
-dwi2response tournier loop (lib/mrtrix3/dwi2response/tournier.py:77-154) restated on the CPU, the
-checker of the dwi2response command.
+dwi2response tournier and tax loops (lib/mrtrix3/dwi2response/tournier.py:77-154, tax.py:56-149)
+restated on the CPU, the checker of the dwi2response command.
 
 Each step runs, where the script runs a command, the oracle's restatement of it:
   dwi2fod csd -mask                 orc.CsdOracle.run_volume (oracle/csd_oracle.c)
@@ -15,8 +15,13 @@ Each step runs, where the script runs a command, the oracle's restatement of it:
                                     292-299)
   amp2response                      resp_oracle.accumulate / solve
   maskfilter dilate -npass          dilate(): face neighbours, npass passes (filter/dilate.h:60-104)
+  mrstats -allvolumes -mask (tax)   tax_init_rf(): mean and sample standard deviation of the finite
+                                    values, as mrstats prints them: 6 significant digits
+                                    (str<cdouble>, core/mrtrix.h:353-363)
 Images are flat arrays over the voxel grid, x fastest.
 """
+import math
+
 import numpy as np
 
 from . import orc, resp_oracle
@@ -134,4 +139,65 @@ def run(dw, grad, shape, mask, number=300, iter_voxels=0, npass=1, max_iters=10,
         mask_in = next_mask(s["cf"], s["count"], iter_voxels, npass, mask, shape)
         s["next_mask"] = mask_in
         prev = s["sf"]
+    return rf, history[-1]["sf"], max_iters, False, history
+
+
+def printed6(x):
+    """A double as an ostream prints it by default (6 significant digits), read back."""
+    return float("%g" % x)
+
+
+def tax_init_rf(dw, mask):
+    """tax.py:66-73: [m, -s/2, s^2/(4m)] from the mean m and standard deviation s of the DW signal in
+    the mask, each scaled by sqrt(4 pi)."""
+    vals = np.asarray(dw, dtype=np.float64)[np.asarray(mask, dtype=bool)].ravel()
+    vals = vals[np.isfinite(vals)]
+    s4 = math.sqrt(4.0 * math.pi)
+    m = printed6(vals.mean()) * s4
+    sd = printed6(vals.std(ddof=1)) * s4
+    return np.array([m, -0.5 * sd, 0.25 * sd * sd / m])
+
+
+def tax_step(dw, grad, shape, mask_in, rf_in, amp_lmax, peak_ratio, nthreads=8):
+    """One tax iteration: the response, the peak ratio image (float32), the selection, the FODs."""
+    csd = orc.CsdOracle(grad, np.asarray(rf_in, dtype=np.float64))
+    fod, _ = csd.run_volume(np.ascontiguousarray(dw, dtype=np.float32), shape,
+                            mask=np.asarray(mask_in, dtype=np.uint8), nthreads=nthreads)
+    p, d = fixel_summary(fod, mask_in, csd.lmax)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ratio = (p[:, 1] / p[:, 0]).astype(np.float32)
+        sf = (ratio < np.float32(peak_ratio)) & np.asarray(mask_in, dtype=bool)
+    if not sf.any():
+        raise ValueError("Aborting: All voxels have been excluded from single-fibre selection")
+    lm = 10 if amp_lmax is None else int(amp_lmax)
+    M, b, _ = resp_oracle.accumulate(dw, d, sf, shell_directions(grad), np.arange(dw.shape[1]), lm)
+    rf, _ = resp_oracle.solve(lm, M, b, True)
+    return dict(rf=rf, ratio=ratio, sf=sf, fod=fod)
+
+
+def tax_converged(old, new, convergence):
+    """tax.py:124-133: every coefficient pair's half-difference over its mean within convergence
+    percent (a negative mean never asks for another iteration)."""
+    change = 0.01 * convergence
+    for o, n in zip(old, new):
+        if math.fabs(0.5 * (o - n)) / (0.5 * (o + n)) > change:
+            return False
+    return True
+
+
+def tax_run(dw, grad, shape, mask, peak_ratio=0.1, max_iters=20, convergence=0.5, lmax=None, nthreads=8,
+            step_fn=None, init_rf=None):
+    """The tax loop; returns (response, selection, iterations run, converged, history)."""
+    step_fn = step_fn or tax_step
+    mask_in = np.asarray(mask, dtype=bool)
+    rf = tax_init_rf(dw, mask_in) if init_rf is None else np.asarray(init_rf, dtype=np.float64)
+    history = []
+    for it in range(max_iters):
+        s = step_fn(dw, grad, shape, mask_in, rf, lmax, peak_ratio, nthreads=nthreads)
+        s["mask_in"] = mask_in
+        history.append(s)
+        if it > 0 and tax_converged(rf, s["rf"], convergence):
+            return s["rf"], s["sf"], it + 1, True, history
+        rf = s["rf"]
+        mask_in = s["sf"]
     return rf, history[-1]["sf"], max_iters, False, history

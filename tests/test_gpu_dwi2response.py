@@ -1,7 +1,7 @@
-"""dwi2response tournier on the GPU (mrtrix3_amd/bin/dwi2response) against the oracle's restatement
-of the script's loop (oracle/tournier_oracle.py), iteration by iteration: each oracle step starts
-from the command's own mask and response of the previous iteration (its -scratch -nocleanup
-files), so every iteration is checked on its own inputs.
+"""dwi2response tournier and tax on the GPU (mrtrix3_amd/bin/dwi2response) against the oracle's
+restatement of the scripts' loops (oracle/dwi2response_oracle.py), iteration by iteration: each
+oracle step starts from the command's own mask and response of the previous iteration (its
+-scratch -nocleanup files), so every iteration is checked on its own inputs.
 
 Bars: the cost image equal to the oracle's where either is NaN and within 1e-5 relative elsewhere
 (the FODs of the GPU and of the oracle agree to about 1e-7, the cost is float32 arithmetic on
@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from mrtrix3_amd import mif, phantom
-from oracle import tournier_oracle as T
+from oracle import dwi2response_oracle as T
 
 pytestmark = pytest.mark.gpu
 
@@ -111,3 +111,44 @@ def test_tournier_iteration_cap_and_lmax(case):
     assert ("Exiting after maximum 2 iterations" in r.stderr) == (not conv)
     np.testing.assert_allclose(rf, rf_o, rtol=0, atol=1e-5 * abs(rf_o[0]))
     say("[gpu] -max_iters 2 -lmax 6: %s (oracle %s)" % (np.round(rf, 3), np.round(rf_o, 3)))
+
+
+def test_tax_iterations_match_oracle(case):
+    """dwi2response tax: the initial response from the masked signal's statistics as mrstats prints
+    them, then each iteration replayed by the oracle from the command's previous response and
+    selection: peak-ratio image (NaN pattern, 1e-5), selection identical, response within 1e-6,
+    termination by the convergence test."""
+    d = case["dir"]
+    sc = d / "scratch_tax"
+    r = subprocess.run([EXE, "tax", str(d / "dwi.mif"), str(d / "resp_tax.txt"), "-mask", str(d / "mask.mif"),
+                        "-scratch", str(sc), "-nocleanup", "-voxels", str(d / "voxels_tax.mif"), "-info"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    say(r.stderr.strip().replace("dwi2response: ", "  "))
+    mask = case["mask"]
+    init = load_rf(str(sc / "init_RF.txt"))
+    np.testing.assert_allclose(init, T.tax_init_rf(case["dw"], mask), rtol=1e-12)
+    K = 0
+    while (sc / ("iter%d_RF.txt" % K)).exists():
+        K += 1
+    assert K >= 2
+    prev = init
+    for k in range(K):
+        mask_in = mask if k == 0 else flat(mif.read(str(sc / ("iter%d_SF.mif" % (k - 1))))[0]) != 0
+        s = T.tax_step(case["dw"], case["grad"], SHAPE, mask_in, prev, None, 0.1)
+        ratio = flat(mif.read(str(sc / ("iter%d_peak_ratio.mif" % k)))[0])
+        nan_g, nan_o = np.isnan(ratio), np.isnan(s["ratio"])
+        assert np.array_equal(nan_g, nan_o), "iteration %d" % k
+        np.testing.assert_allclose(ratio[~nan_g], s["ratio"][~nan_o], rtol=1e-5, atol=1e-7, err_msg="iteration %d" % k)
+        sf = flat(mif.read(str(sc / ("iter%d_SF.mif" % k)))[0]) != 0
+        assert np.array_equal(sf, s["sf"]), "iteration %d: selections differ in %d voxels" % (k, int((sf ^ s["sf"]).sum()))
+        rf = load_rf(str(sc / ("iter%d_RF.txt" % k)))
+        np.testing.assert_allclose(rf, s["rf"], rtol=0, atol=1e-6 * abs(s["rf"][0]), err_msg="iteration %d" % k)
+        if k > 0:
+            assert T.tax_converged(prev, rf, 0.5) == (k == K - 1), "iteration %d" % k
+        say("[gpu] tax iteration %d: %d voxels tested, %d selected, rf[0] %.4f (oracle %.4f)" % (
+            k, int(mask_in.sum()), int(sf.sum()), rf[0], s["rf"][0]))
+        prev = rf
+    assert "Exited at iteration %d with %d SF voxels due to unchanged RF coefficients" % (K, int(sf.sum())) in r.stderr
+    np.testing.assert_array_equal(load_rf(str(d / "resp_tax.txt")), prev)
+    assert np.array_equal(flat(mif.read(str(d / "voxels_tax.mif"))[0]) != 0, sf)

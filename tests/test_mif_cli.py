@@ -212,6 +212,9 @@ def test_dwi2response_argument_errors(tmp_path):
 
     fails(["tournier", dwi], "usage")
     fails(["dhollander", dwi, out], "not part of this build")
+    fails(["tax", dwi, out, "-mask", m_ok, "-number", "10"], "unrecognized arguments: -number")
+    fails(["tournier", dwi, out, "-mask", m_ok, "-peak_ratio", "0.2"], "unrecognized arguments: -peak_ratio")
+    fails(["tax", dwi, out, "-mask", m_ok, "-peak_ratio", "x"], "invalid float value")
     fails(["tournier", dwi, exists, "-mask", m_ok], "already exists")
     fails(["tournier", dwi, out, "-mask", m_ok, "-lmax", "7"], "Parameter lmax must be a number")
     fails(["tournier", dwi, out, "-mask", m_ok, "-lmax", "4,6"], "single lmax value")
