@@ -1056,11 +1056,14 @@ def bench_gt(args, torch, ph):
                                "volumes), lmax-8 TOD, WM + one isotropic tissue, resident in HBM"},
         "roofline": {"bound": "hbm", "achieved": hbm / (kern_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
                      "frac": hbm / (kern_ms * 1e-3) / 1e9 / 8000.0, "traffic": None,
-                     "kernel": "gt_residual_kernel + icls_active_set_kernel<16, 4, true> + gt_energy_kernel + gt_sum_kernel",
+                     "kernel": ("gt_fused_kernel<%d> + gt_sum_kernel" % (E.nf + 1)) if E.nf + 1 <= 4 else
+                               "gt_residual_kernel + icls_active_set_kernel + gt_energy_kernel + gt_sum_kernel",
                      "kernel_ms": kern_ms, "algorithmic_bytes": hbm, "fp64_flop": fl,
                      "fp64_achieved_tflops": achieved,
-                     "note": "algorithmic bytes = float32 signal and TOD in, energy and fractions out; the FP64 "
-                             "residual (y) round trip through HBM between the passes is extra traffic"},
+                     "note": "algorithmic bytes = float32 signal and TOD in, energy and fractions out; with up to 4 "
+                             "NNLS unknowns one fused kernel (a lane per voxel, LDS-staged TOD and K blocks, the "
+                             "NNLS by active-set enumeration) makes no other traffic; its occupancy is bounded by "
+                             "the LDS tiles, so it is latency-bound below both the HBM and the FP64 rooflines"},
     }
     if not args.no_cpu:
         sample = args.cpu_sample or CPU_SAMPLE["gt_energy"]

@@ -1,6 +1,7 @@
 // C ABI of global tractography's external energy (include/mrgt.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -25,6 +26,9 @@ struct mrgt_ctx {
   Matrix K, Ak;
   mrcsd::DeviceBuffer<double> Kt, Akd;
   mricls_ctx* nnls = nullptr;
+  // the fused kernel's tables (nf + 1 <= 4 unknowns): inverses of G's principal submatrices and G
+  bool fused = false;
+  mrcsd::DeviceBuffer<double> tab;
   // workspace: residuals y [cap][nrows], NNLS solutions [cap][nf + 1], per-workgroup energy sums
   mrcsd::DeviceBuffer<double> y, fk, partial;
   size_t cap = 0;
@@ -46,6 +50,58 @@ void build_kernels(const double* grad, int nvol, const double* resp_wm, int wm_r
   std::vector<std::vector<double>> iso(static_cast<size_t>(nf));
   for (int j = 0; j < nf; ++j) iso[size_t(j)].assign(resp_iso + size_t(j) * nshell, resp_iso + size_t(j + 1) * nshell);
   MR::DWI::Tractography::GT::external_energy_kernels(g, wm, iso, lmax, weight, K, Ak);
+}
+
+// The fused kernel's table for n unknowns: for each subset s < 2^n, (G_SS)^-1 embedded in an n x n
+// zero matrix, then G = Ak^T Ak.  False when a principal submatrix is not safely invertible (the
+// batched ICLS path is taken then).
+bool fused_table(const Matrix& Ak, std::vector<double>& tab) {
+  const int n = Ak.cols;
+  Matrix G(n, n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double a = 0.0;
+      for (int r = 0; r < Ak.rows; ++r) a += Ak(r, i) * Ak(r, j);
+      G(i, j) = a;
+    }
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i) dmax = std::max(dmax, G(i, i));
+  if (!(dmax > 0.0)) return false;
+  tab.assign(size_t((1 << n) * n * n + n * n), 0.0);
+  for (int s = 1; s < (1 << n); ++s) {
+    std::vector<int> m;
+    for (int i = 0; i < n; ++i)
+      if ((s >> i) & 1) m.push_back(i);
+    const int k = int(m.size());
+    // Gauss-Jordan with partial pivoting on [G_SS | I]
+    std::vector<double> a(size_t(k * 2 * k), 0.0);
+    for (int i = 0; i < k; ++i) {
+      for (int j = 0; j < k; ++j) a[size_t(i * 2 * k + j)] = G(m[size_t(i)], m[size_t(j)]);
+      a[size_t(i * 2 * k + k + i)] = 1.0;
+    }
+    for (int c = 0; c < k; ++c) {
+      int p = c;
+      for (int r = c + 1; r < k; ++r)
+        if (std::fabs(a[size_t(r * 2 * k + c)]) > std::fabs(a[size_t(p * 2 * k + c)])) p = r;
+      if (!(std::fabs(a[size_t(p * 2 * k + c)]) > 1e-12 * dmax)) return false;
+      if (p != c)
+        for (int j = 0; j < 2 * k; ++j) std::swap(a[size_t(p * 2 * k + j)], a[size_t(c * 2 * k + j)]);
+      const double inv = 1.0 / a[size_t(c * 2 * k + c)];
+      for (int j = 0; j < 2 * k; ++j) a[size_t(c * 2 * k + j)] *= inv;
+      for (int r = 0; r < k; ++r) {
+        if (r == c) continue;
+        const double f = a[size_t(r * 2 * k + c)];
+        for (int j = 0; j < 2 * k; ++j) a[size_t(r * 2 * k + j)] -= f * a[size_t(c * 2 * k + j)];
+      }
+    }
+    double* Z = tab.data() + size_t(s * n * n);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) Z[m[size_t(i)] * n + m[size_t(j)]] = a[size_t(i * 2 * k + k + j)];
+  }
+  double* Gt = tab.data() + size_t((1 << n) * n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) Gt[i * n + j] = G(i, j);
+  return true;
 }
 
 }  // namespace
@@ -77,6 +133,11 @@ int mrgt_create(const double* grad, int nvol, const double* resp_wm, int wm_rows
     const Matrix Kt = ctx->K.transpose();
     ctx->Kt.upload(Kt.data(), Kt.a.size());
     ctx->Akd.upload(ctx->Ak.data(), ctx->Ak.a.size());
+    std::vector<double> tab;
+    if (mrcsd::gt_fused_fits(nvol, ctx->ncols, nf + 1) && fused_table(ctx->Ak, tab)) {
+      ctx->tab.upload(tab.data(), tab.size());
+      ctx->fused = true;
+    }
     *out = ctx.release();
     return MRCSD_OK;
   });
@@ -125,6 +186,17 @@ int mrgt_energy(mrgt_ctx* ctx, const float* dwi, int ld_dwi, const float* tod, i
     if (nvox > size_t(UINT32_MAX)) return mrcsd::fail(MRCSD_ERR_ARG, "mrgt_energy: more than 2^32 - 1 voxels");
     MRCSD_HIP(hipSetDevice(ctx->device));
     const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (ctx->fused) {
+      const size_t blocks = mrcsd::gt_fused_blocks(nvox);
+      if (ctx->partial.size() < blocks + 1) {
+        MRCSD_HIP(hipStreamSynchronize(s));   // an earlier call may still use the workspace
+        ctx->partial.alloc(blocks + 1);
+      }
+      MRCSD_HIP(mrcsd::gt_fused(ctx->Kt.get(), ctx->nrows, ctx->ncols, ctx->Akd.get(), ctx->nf + 1, ctx->tab.get(),
+                                dwi, ld_dwi, tod, ld_tod, ctx->mu, nvox, eext, fiso, ctx->partial.get(),
+                                ctx->partial.size(), total, s));
+      return MRCSD_OK;
+    }
     if (nvox > ctx->cap) {
       MRCSD_HIP(hipStreamSynchronize(s));   // an earlier call may still use the workspace
       ctx->y.alloc(nvox * size_t(ctx->nrows));
