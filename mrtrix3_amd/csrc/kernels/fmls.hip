@@ -351,9 +351,10 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const f
 
   // ---- lobe growth (fmls.cpp:188-249) ----
   int nl = 0;
+  // (the sort left each direction's amplitude beside it: rev[i] = amp[ord[i]] until the revision)
   for (int i = 0; i < P; ++i) {
     const int d = T.ord[i];
-    const double v = T.amp[d];
+    const double v = T.rev[i];
     // the lobes of d's neighbours, distinct and ascending (INT_MAX pads)
     int al[8];
 #pragma unroll
@@ -434,7 +435,7 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const f
     if (lb <= -2) {
       const int l = -2 - lb;
       T.lab[d] = (int16_t)l;
-      lobe_add(S, T, l, d, T.amp[d]);
+      lobe_add(S, T, l, d, T.rev[i]);
     }
   }
 
