@@ -54,9 +54,11 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 size_t fmls_work_bytes(int ndir, size_t chunk) {
   const size_t n = size_t(ndir) * chunk;
-  // doubles: amps, peak, integ (n each), mean, rev (3 n each); int16: lab, ord, nxt, head, tail, np;
-  // int32: nlobes (one per slot) -- the pieces fmls_work_carve takes, in its order
-  return 3 * align256(8 * n) + 2 * align256(8 * 3 * n) + 6 * align256(2 * n) + align256(4 * chunk);
+  // doubles: amps (fmls_amp_ld(ndir) per slot), peak, integ (n each), mean, rev (3 n each); int16:
+  // lab, ord, nxt, head, tail, np; int32: npos, nlobes (one per slot) -- the pieces
+  // fmls_work_carve takes, in its order
+  return align256(8 * size_t(fmls_amp_ld(ndir)) * chunk) + 2 * align256(8 * n) + 2 * align256(8 * 3 * n) +
+         6 * align256(2 * n) + 2 * align256(4 * chunk);
 }
 
 FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
@@ -68,7 +70,7 @@ FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
     return q;
   };
   FmlsWork W{};
-  W.amps = reinterpret_cast<double*>(take(8 * n));
+  W.amps = reinterpret_cast<double*>(take(8 * size_t(fmls_amp_ld(ndir)) * chunk));
   W.peak = reinterpret_cast<double*>(take(8 * n));
   W.integ = reinterpret_cast<double*>(take(8 * n));
   W.mean = reinterpret_cast<double*>(take(8 * 3 * n));
@@ -79,6 +81,7 @@ FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
   W.head = reinterpret_cast<int16_t*>(take(2 * n));
   W.tail = reinterpret_cast<int16_t*>(take(2 * n));
   W.np = reinterpret_cast<int16_t*>(take(2 * n));
+  W.npos = reinterpret_cast<int32_t*>(take(4 * chunk));
   W.nlobes = reinterpret_cast<int32_t*>(take(4 * chunk));
   return W;
 }

@@ -11,6 +11,9 @@ constexpr int kFmlsMaxDirs = 1536;    // directions of the segmentation set (128
 constexpr int kFmlsMaxLmax = 12;
 constexpr int kFmlsBlock = 64;        // voxels per workgroup of the segmentation and offset kernels
 
+// Row stride (elements) of the voxel-major amplitude array: ndir rounded up to 32 doubles (256 bytes).
+__host__ __device__ inline int fmls_amp_ld(int ndir) { return (ndir + 31) & ~31; }
+
 struct FmlsDeviceSet {
   int ndir;
   const double* unit;        // [ndir][3]
@@ -30,12 +33,14 @@ struct FmlsDeviceSet {
   int dir_from_peak;         // fixel direction: 1 the maximal peak, 0 the lobe's mean direction
 };
 
-// Per-voxel work arrays of one chunk, lane-interleaved: element i of the voxel in slot s of the
-// chunk lives at [i * chunk + s], so the lanes of a wavefront (consecutive slots) touching the
-// same element index touch one contiguous run of memory.  Every table is sized for the worst case
-// (as many lobes and peaks as directions), so no voxel overflows.
+// Per-voxel work arrays of one chunk.  The amplitudes are voxel-major (row s of fmls_amp_ld(ndir)
+// doubles for the voxel in slot s), read by one wavefront per voxel; the other arrays are
+// lane-interleaved: element i of the voxel in slot s lives at [i * chunk + s], so the lanes of a
+// wavefront (consecutive slots) touching the same element index touch one contiguous run of
+// memory.  Every table is sized for the worst case (as many lobes and peaks as directions), so no
+// voxel overflows.
 struct FmlsWork {
-  double* amps;              // [ndir] amplitudes on the directions
+  double* amps;              // [chunk][fmls_amp_ld(ndir)] amplitudes on the directions
   int16_t* lab;              // [ndir] lobe of each direction; -1 none; -2 - l: waits for lobe l
   uint16_t* ord;             // [ndir] positive directions in visiting order
   int16_t* nxt;              // [ndir] peak lists through the seed directions
@@ -45,7 +50,9 @@ struct FmlsWork {
   int16_t* head;             // [ndir] first and last seed of the lobe's peak list
   int16_t* tail;
   int16_t* np;               // [ndir] peaks of the lobe; -1: lobe dropped by a threshold
-  double* rev;               // [3 ndir] peak directions during the revision
+  double* rev;               // [3 ndir] the amplitudes in visiting order, then the peak directions
+                             // during the revision
+  int32_t* npos;             // [1] positive directions to visit (0: the voxel has no positive lobe)
   int32_t* nlobes;           // [1] lobes of the voxel, kept and dropped
 };
 

@@ -1,18 +1,18 @@
-// FOD segmentation on gfx950: fod2fixel's FMLS (src/dwi/fmls.cpp:166-377) for a batch of voxels,
-// one lane per voxel.
+// FOD segmentation on gfx950: fod2fixel's FMLS (src/dwi/fmls.cpp:166-377) for a batch of voxels.
 //
 // Per chunk of voxels:
 //   * fmls_amp_kernel: the FOD amplitudes on the segmentation directions (1281 for fod2fixel) as
-//     one FP64 GEMM, amps[d][v] = sum_k SHT[d][k] F_v[k] accumulated by FMA in ascending k,
-//     LDS-tiled (128 directions x 64 voxels per workgroup, 8 x 4 outputs per thread), written
-//     lane-interleaved for the segmentation;
+//     one FP64 GEMM, amps[v][d] = sum_k SHT[d][k] F_v[k] accumulated by FMA in ascending k,
+//     LDS-tiled (128 directions x 64 voxels per workgroup, 8 x 4 outputs per thread), one row per
+//     voxel;
+//   * fmls_sort_kernel: a wavefront per voxel checks that the largest |amplitude| is positive and
+//     sorts the positive directions by decreasing amplitude, the lowest index first among equal
+//     amplitudes (the reference's multimap order), by a bitonic sort in registers;
 //   * fmls_segment_kernel<LMAX>: each lane segments its own voxel, step for step as the reference:
-//       - the positive directions sorted by decreasing amplitude, the lowest index first among
-//         equal amplitudes (the reference's multimap order), by an in-place heapsort;
-//       - lobe growth in that order: a direction adjacent to no positive lobe seeds one, adjacent
-//         to one joins it, adjacent to several either merges them (bridge amplitude over the peak
-//         of the highest-numbered adjacent lobe above the merge ratio) or waits, outside every
-//         lobe, for its assignment to the lowest-numbered of them after the loop;
+//       - lobe growth in visiting order: a direction adjacent to no positive lobe seeds one,
+//         adjacent to one joins it, adjacent to several either merges them (bridge amplitude over
+//         the peak of the highest-numbered adjacent lobe above the merge ratio) or waits, outside
+//         every lobe, for its assignment to the lowest-numbered of them after the loop;
 //       - per lobe: the integral threshold, the Newton peak search from each of its peaks
 //         (Math::SH::get_peak in double with the precomputed Legendre table, SH.h:398-559), a
 //         revision kept only when it stays nearest to that peak and inside the lobe (nearest
@@ -24,8 +24,8 @@
 //     cmd/fod2fixel.cpp:239-280; 0 for voxels without fixels) and its fixels.
 // The work arrays hold every table a voxel can need (as many lobes and peaks as directions), so
 // no voxel is deferred, and the segmentation has no cross-lane operation, barrier or atomic (its
-// LDS is a column per lane for the Newton search's Legendre values): the only cooperation between
-// lanes is the workgroup scans of the offset kernels.
+// LDS is a column per lane for the Newton search's Legendre values): the cooperation between
+// lanes is the sort's exchanges within a wavefront and the workgroup scans of the offset kernels.
 // Negative directions only ever form negative lobes, which the reference discards and which take
 // no part in positive lobes' adjacency, order, merges or indices, so they are not visited.
 #include <hip/hip_runtime.h>
@@ -198,7 +198,6 @@ __device__ int fmls_select(const FmlsDeviceSet& S, const double* p) {
 // seeds of the lobes merged into it, kept as a linked list through nxt[seed direction] in the
 // reference's order.
 struct Tabs {
-  Lane<double> amp;
   Lane<int16_t> lab;
   Lane<uint16_t> ord;
   Lane<int16_t> nxt;
@@ -209,8 +208,8 @@ struct Tabs {
 
 __device__ Tabs tabs_of(const FmlsWork& W, size_t slot, size_t cs) {
   const uint32_t o = (uint32_t)slot, c = (uint32_t)cs;
-  return Tabs{{W.amps, c, o},  {W.lab, c, o},  {W.ord, c, o},   {W.nxt, c, o},  {W.peak, c, o}, {W.mean, c, o},
-              {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o},  {W.np, c, o},   {W.rev, c, o}};
+  return Tabs{{W.lab, c, o},   {W.ord, c, o},  {W.nxt, c, o},  {W.peak, c, o}, {W.mean, c, o},
+              {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o}, {W.np, c, o},   {W.rev, c, o}};
 }
 
 // a new lobe seeded by direction d (fmls.cpp:199-201, FOD_lobe's constructor fmls.h:64-76)
@@ -274,84 +273,17 @@ __device__ void lobe_copy(const Tabs& T, int dst, int src) {
   T.np[dst] = T.np[src];
 }
 
-// direction a (amplitude ka) is visited after direction b (amplitude kb): lower amplitude, or the
-// same amplitude and a higher index
-__device__ __forceinline__ bool visited_after(double ka, int a, double kb, int b) {
-  return ka < kb || (ka == kb && a > b);
-}
-
-// sift-down (hole method) of a heap of (amplitude, direction) entries in key[0, end), ord[0, end)
-// whose root is the direction visited last; a heap of at most kFmlsMaxDirs entries is 11 levels
-// deep, so 12 steps always reach a leaf
-__device__ void heap_sift(const Lane<double>& key, const Lane<uint16_t>& ord, int root, int end) {
-  const int r = ord[root];
-  const double kr = key[root];
-  for (int step = 0; step < 12; ++step) {
-    int child = 2 * root + 1;
-    if (child >= end) break;
-    int c = ord[child];
-    double kc = key[child];
-    if (child + 1 < end) {
-      const int c2 = ord[child + 1];
-      const double k2 = key[child + 1];
-      if (visited_after(k2, c2, kc, c)) {
-        ++child;
-        c = c2;
-        kc = k2;
-      }
-    }
-    if (!visited_after(kc, c, kr, r)) break;
-    ord[root] = (uint16_t)c;
-    key[root] = kc;
-    root = child;
-  }
-  ord[root] = (uint16_t)r;
-  key[root] = kr;
-}
-
-// The segmentation of one voxel (fmls.cpp:166-301); returns the number of positive lobes, whose
-// records hold the kept fixels (np < 0: dropped by a threshold).
+// The segmentation of one voxel (fmls.cpp:166-301) from its P > 0 positive directions in visiting
+// order (ord, with their amplitudes in rev); returns the number of positive lobes, whose records
+// hold the kept fixels (np < 0: dropped by a threshold).
 template <int LMAX>
-__device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, const float* f, double* AL) {
+__device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, const float* f, double* AL) {
   const int ndir = S.ndir;
-  // the largest |amplitude| (lowest index among equal ones) must be positive (fmls.cpp:171-184)
-  double best = -1.0;
-  int bi = 0;
-  for (int d = 0; d < ndir; ++d) {
-    const double a = fabs(T.amp[d]);
-    if (a > best) {
-      best = a;
-      bi = d;
-    }
-  }
-  if (!(T.amp[bi] > 0.0)) return 0;
-
-  // the positive directions in visiting order, by heapsort on (amplitude, direction) entries; the
-  // amplitudes ride along in the peak-direction array, unused until the revision
-  int P = 0;
-  for (int d = 0; d < ndir; ++d) {
-    T.lab[d] = -1;
-    const double v = T.amp[d];
-    if (v > 0.0) {
-      T.ord[P] = (uint16_t)d;
-      T.rev[P] = v;
-      ++P;
-    }
-  }
-  for (int s = P / 2 - 1; s >= 0; --s) heap_sift(T.rev, T.ord, s, P);
-  for (int e = P - 1; e > 0; --e) {
-    const uint16_t t = T.ord[0];
-    const double kt = T.rev[0];
-    T.ord[0] = T.ord[e];
-    T.rev[0] = T.rev[e];
-    T.ord[e] = t;
-    T.rev[e] = kt;
-    heap_sift(T.rev, T.ord, 0, e);
-  }
+  for (int d = 0; d < ndir; ++d) T.lab[d] = -1;
 
   // ---- lobe growth (fmls.cpp:188-249) ----
   int nl = 0;
-  // (the sort left each direction's amplitude beside it: rev[i] = amp[ord[i]] until the revision)
+  // (the sort left each direction's amplitude beside it: rev[i] is the amplitude of ord[i])
   for (int i = 0; i < P; ++i) {
     const int d = T.ord[i];
     const double v = T.rev[i];
@@ -508,9 +440,8 @@ __global__ __launch_bounds__(kFmlsBlock) void fmls_segment_kernel(FmlsDeviceSet 
   const size_t vox = v0 + slot;
   const Tabs T = tabs_of(L.work, slot, L.chunk);
   const float* f = L.fod + vox * (size_t)L.ld_fod;
-  const double c0 = (double)f[0];
-  const bool ok = (L.valid == nullptr || L.valid[vox] != 0) && c0 > 0.0 && isfinite(c0);
-  const int nl = ok ? fmls_segment_voxel<LMAX>(S, T, f, s_al + threadIdx.x) : 0;
+  const int P = L.work.npos[slot];
+  const int nl = P > 0 ? fmls_segment_voxel<LMAX>(S, T, P, f, s_al + threadIdx.x) : 0;
   // the kept lobes in order, at most maxnum of them (fod2fixel.cpp:139-150)
   int nk = 0;
   double sp0 = 0.0, sp1 = 0.0, sx = NAN, sy = NAN, sz = NAN;
@@ -613,14 +544,128 @@ __global__ __launch_bounds__(kFmlsBlock) void fmls_emit(FmlsLaunch L, size_t v0,
   }
 }
 
+// direction a (amplitude ka) is visited before direction b (amplitude kb): higher amplitude, or
+// the same amplitude and a lower index -- a strict order on the positive directions, so any
+// correct sort yields the reference's visiting order
+__device__ __forceinline__ bool visited_before(double ka, int a, double kb, int b) {
+  return (ka > kb) | ((ka == kb) & (a < b));   // no short circuit: compares and selects, no branches
+}
+
+constexpr int kSortPer = 32;   // entries per lane: 64 x 32 = 2048 >= kFmlsMaxDirs
+
+// Workgroup b of a grid of n, placed so that each of the 8 XCDs (which take workgroups round-robin)
+// runs one contiguous range of them: neighbouring voxels then write the same lines of the
+// lane-interleaved arrays through one L2.
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned n) {
+  const unsigned x = b % 8u, q = n / 8u, r = n % 8u;
+  return x * q + (x < r ? x : r) + b / 8u;
+}
+
+// One wavefront per voxel of the chunk: the check that the largest |amplitude| (the lowest index
+// among equal ones) is positive (fmls.cpp:171-184), then the positive directions in visiting order
+// by a bitonic sort of (amplitude, direction) entries held in registers -- entry e = 32 lane + k,
+// compare-exchanges at distances 1..16 within a lane and at 32..1024 between lanes -- padded to
+// 2048 with entries visited after every positive one.  Writes npos (0: nothing to segment), ord
+// and rev (the amplitudes) for the segmentation kernel.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fmls_sort_kernel(FmlsLaunch L, int ndir, size_t v0, size_t nv) {
+  const size_t slot = xcd_contiguous(blockIdx.x, gridDim.x);
+  if (slot >= nv) return;
+  const int lane = threadIdx.x;
+  const size_t vox = v0 + slot;
+  const double c0 = (double)L.fod[vox * (size_t)L.ld_fod];
+  const bool ok = (L.valid == nullptr || L.valid[vox] != 0) && c0 > 0.0 && isfinite(c0);
+  if (!ok) {
+    if (lane == 0) L.work.npos[slot] = 0;
+    return;
+  }
+  const double* a = L.work.amps + slot * (size_t)fmls_amp_ld(ndir);
+  double key[kSortPer];
+  int idx[kSortPer];
+  double best = -1.0;   // largest |amplitude| (NaN never wins) and its lowest index
+  int bi = 0;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < kSortPer; ++k) {
+    const int d = k * 64 + lane;   // ascending in k: the first of equal maxima is kept
+    const double v = d < ndir ? a[d] : 0.0;
+    if (d < ndir && fabs(v) > best) {
+      best = fabs(v);
+      bi = d;
+    }
+    const bool pos = d < ndir && v > 0.0;
+    key[k] = pos ? v : 0.0;
+    idx[k] = pos ? d : 0xffff;
+    cnt += pos ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o);
+    const int oi = __shfl_xor(bi, o);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+    cnt += __shfl_xor(cnt, o);
+  }
+  if (!(a[bi] > 0.0)) {   // bi is the same on every lane
+    if (lane == 0) L.work.npos[slot] = 0;
+    return;
+  }
+  for (int sz = 2; sz <= 64 * kSortPer; sz <<= 1) {
+    // distances of 32 and more: the partner entry is register k of lane ^ (j / 32)
+    for (int j = sz >> 1; j >= kSortPer; j >>= 1) {
+      const int lj = j / kSortPer;
+      const bool first = ((lane & lj) == 0) == (((lane * kSortPer) & sz) == 0);   // keep the earlier entry
+#pragma unroll
+      for (int k = 0; k < kSortPer; ++k) {
+        const double ok2 = __shfl_xor(key[k], lj);
+        const int oi = __shfl_xor(idx[k], lj);
+        const bool take = visited_before(ok2, oi, key[k], idx[k]) == first;
+        key[k] = take ? ok2 : key[k];
+        idx[k] = take ? oi : idx[k];
+      }
+    }
+    // distances 16..1 within the lane's registers
+#pragma unroll
+    for (int j = kSortPer / 2; j > 0; j >>= 1) {
+      if (j >= sz) continue;
+#pragma unroll
+      for (int k = 0; k < kSortPer; ++k) {
+        if (k & j) continue;
+        const int k2 = k | j;
+        const bool up = (((lane * kSortPer) | k) & sz) == 0;
+        // ascending: k2's entry moves to k when it is visited first; descending: when it is not
+        // (equal entries are padding, where a swap changes nothing)
+        const bool sw = visited_before(key[k2], idx[k2], key[k], idx[k]) == up;
+        const double a0 = key[k], a1 = key[k2];
+        const int i0 = idx[k], i1 = idx[k2];
+        key[k] = sw ? a1 : a0;
+        key[k2] = sw ? a0 : a1;
+        idx[k] = sw ? i1 : i0;
+        idx[k2] = sw ? i0 : i1;
+      }
+    }
+  }
+  const size_t cs = L.chunk;
+#pragma unroll
+  for (int k = 0; k < kSortPer; ++k) {
+    const int e = lane * kSortPer + k;
+    if (e < cnt) {
+      L.work.ord[(size_t)e * cs + slot] = (uint16_t)idx[k];
+      L.work.rev[(size_t)e * cs + slot] = key[k];
+    }
+  }
+  if (lane == 0) L.work.npos[slot] = cnt;
+}
+
 // ---- HIP launches ----
 
 constexpr int kTileD = 128, kTileV = 64, kTileK = 16;
 
-// amps[d * cs + v] for the nv voxels of the chunk starting at v0
+// amps[v * lda + d] for the nv voxels of the chunk starting at v0
 __global__ __launch_bounds__(256) void fmls_amp_kernel(const double* __restrict__ sht, int ndir, int nsh,
                                                        const float* __restrict__ fod, int ld, size_t v0, size_t nv,
-                                                       size_t cs, double* __restrict__ amps) {
+                                                       size_t lda, double* __restrict__ amps) {
   __shared__ double sA[kTileK][kTileD + 1];
   __shared__ double sB[kTileK][kTileV + 1];
   const int tid = threadIdx.x;
@@ -666,7 +711,7 @@ __global__ __launch_bounds__(256) void fmls_amp_kernel(const double* __restrict_
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int d = d0 + ty * 8 + j;
-      if (d < ndir) amps[(size_t)d * cs + v] = acc[i][j];
+      if (d < ndir) amps[v * lda + d] = acc[i][j];
     }
   }
 }
@@ -695,8 +740,10 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t 
     const size_t nv = L.nvox - v0 < L.chunk ? L.nvox - v0 : L.chunk;
     const unsigned nb = (unsigned)((nv + kFmlsBlock - 1) / kFmlsBlock);
     const dim3 grid((unsigned)((S.ndir + kTileD - 1) / kTileD), (unsigned)((nv + kTileV - 1) / kTileV));
-    fmls_amp_kernel<<<grid, dim3(256), 0, stream>>>(S.sht, S.ndir, S.nsh, L.fod, L.ld_fod, v0, nv, L.chunk,
-                                                    L.work.amps);
+    fmls_amp_kernel<<<grid, dim3(256), 0, stream>>>(S.sht, S.ndir, S.nsh, L.fod, L.ld_fod, v0, nv,
+                                                    (size_t)fmls_amp_ld(S.ndir), L.work.amps);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    fmls_sort_kernel<<<dim3((unsigned)nv), dim3(64), 0, stream>>>(L, S.ndir, v0, nv);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     switch (S.lmax) {
       case 2: e = launch_segment<2>(S, L, v0, nv, stream); break;

@@ -1,8 +1,8 @@
 // Runs mrfixel on a case written by tests/test_fmls_kernel_emulated.py through the C ABI, with the
-// FMLS segmentation, offset and emit kernels compiled for the CPU against emu_runtime.h
+// FMLS sort, segmentation, offset and emit kernels compiled for the CPU against emu_runtime.h
 // (KERNEL_SOURCE: the kernel file without its launchers) and run one workgroup at a time.  The
 // amplitude GEMM (FMA in ascending coefficient order, as the GPU kernel accumulates, into the same
-// lane-interleaved layout) is restated here on the host.
+// voxel-major layout) is restated here on the host.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,13 +31,15 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t)
   for (size_t v0 = 0; v0 < L.nvox; v0 += L.chunk) {
     const size_t nv = L.nvox - v0 < L.chunk ? L.nvox - v0 : L.chunk;
     const unsigned nb = unsigned((nv + kFmlsBlock - 1) / kFmlsBlock);
+    const size_t lda = size_t(fmls_amp_ld(S.ndir));
     for (size_t v = 0; v < nv; ++v)
       for (int d = 0; d < S.ndir; ++d) {
         double acc = 0.0;
         for (int k = 0; k < S.nsh; ++k)
           acc = std::fma(S.sht[size_t(d) * S.nsh + k], (double)L.fod[(v0 + v) * size_t(L.ld_fod) + k], acc);
-        L.work.amps[size_t(d) * L.chunk + v] = acc;
+        L.work.amps[v * lda + d] = acc;
       }
+    run_grid(unsigned(nv), [&]() { fmls_sort_kernel(L, S.ndir, v0, nv); });
     switch (S.lmax) {
       case 2: run_segment<2>(S, L, v0, nv, nb); break;
       case 4: run_segment<4>(S, L, v0, nv, nb); break;
