@@ -55,10 +55,10 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 size_t fmls_work_bytes(int ndir, size_t chunk) {
   const size_t n = size_t(ndir) * chunk;
   // doubles: amps (fmls_amp_ld(ndir) per slot), peak, integ (n each), mean, rev (3 n each); int16:
-  // lab, ord, nxt, head, tail, np; int32: npos, nlobes (one per slot) -- the pieces
+  // lab, ord, nxt, head, tail, np; uint8: nbr; int32: npos, nlobes (one per slot) -- the pieces
   // fmls_work_carve takes, in its order
   return align256(8 * size_t(fmls_amp_ld(ndir)) * chunk) + 2 * align256(8 * n) + 2 * align256(8 * 3 * n) +
-         6 * align256(2 * n) + 2 * align256(4 * chunk);
+         6 * align256(2 * n) + align256(n) + 2 * align256(4 * chunk);
 }
 
 FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
@@ -81,6 +81,7 @@ FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
   W.head = reinterpret_cast<int16_t*>(take(2 * n));
   W.tail = reinterpret_cast<int16_t*>(take(2 * n));
   W.np = reinterpret_cast<int16_t*>(take(2 * n));
+  W.nbr = reinterpret_cast<uint8_t*>(take(n));
   W.npos = reinterpret_cast<int32_t*>(take(4 * chunk));
   W.nlobes = reinterpret_cast<int32_t*>(take(4 * chunk));
   return W;

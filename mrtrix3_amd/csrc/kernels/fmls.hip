@@ -200,6 +200,7 @@ __device__ int fmls_select(const FmlsDeviceSet& S, const double* p) {
 struct Tabs {
   Lane<int16_t> lab;
   Lane<uint16_t> ord;
+  Lane<uint8_t> nbr;
   Lane<int16_t> nxt;
   Lane<double> peak, mean, integ;
   Lane<int16_t> head, tail, np;
@@ -208,8 +209,8 @@ struct Tabs {
 
 __device__ Tabs tabs_of(const FmlsWork& W, size_t slot, size_t cs) {
   const uint32_t o = (uint32_t)slot, c = (uint32_t)cs;
-  return Tabs{{W.lab, c, o},   {W.ord, c, o},  {W.nxt, c, o},  {W.peak, c, o}, {W.mean, c, o},
-              {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o}, {W.np, c, o},   {W.rev, c, o}};
+  return Tabs{{W.lab, c, o},  {W.ord, c, o},   {W.nbr, c, o},  {W.nxt, c, o}, {W.peak, c, o}, {W.mean, c, o},
+              {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o}, {W.np, c, o},  {W.rev, c, o}};
 }
 
 // a new lobe seeded by direction d (fmls.cpp:199-201, FOD_lobe's constructor fmls.h:64-76)
@@ -287,14 +288,15 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
   for (int i = 0; i < P; ++i) {
     const int d = T.ord[i];
     const double v = T.rev[i];
-    // the lobes of d's neighbours, distinct and ascending (INT_MAX pads)
+    const unsigned seen_before = T.nbr[i];
+    // the lobes of d's neighbours, distinct and ascending (INT_MAX pads); only neighbours visited
+    // before d can carry one (the others still hold -1), so only their labels are read
     int al[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) al[q] = INT_MAX;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int j = S.adj[d * 8 + r];
-      const int lb = j >= 0 ? (int)T.lab[j] : -1;    // waiting directions (<= -2) belong to no lobe yet
+      const int lb = (seen_before >> r) & 1u ? (int)T.lab[S.adj[d * 8 + r]] : -1;   // waiting (<= -2): no lobe yet
       if (lb >= 0) {
         bool seen = false;
 #pragma unroll
@@ -565,9 +567,13 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned n) {
 // among equal ones) is positive (fmls.cpp:171-184), then the positive directions in visiting order
 // by a bitonic sort of (amplitude, direction) entries held in registers -- entry e = 32 lane + k,
 // compare-exchanges at distances 1..16 within a lane and at 32..1024 between lanes -- padded to
-// 2048 with entries visited after every positive one.  Writes npos (0: nothing to segment), ord
-// and rev (the amplitudes) for the segmentation kernel.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fmls_sort_kernel(FmlsLaunch L, int ndir, size_t v0, size_t nv) {
+// 2048 with entries visited after every positive one.  Writes npos (0: nothing to segment), ord,
+// rev (the amplitudes) and nbr (which neighbours of each direction are visited before it, from
+// the directions' positions in an LDS table) for the segmentation kernel.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fmls_sort_kernel(
+    FmlsDeviceSet S, FmlsLaunch L, size_t v0, size_t nv) {
+  __shared__ uint16_t s_pos[kFmlsMaxDirs];   // visiting position of each direction, 0xffff: not visited
+  const int ndir = S.ndir;
   const size_t slot = xcd_contiguous(blockIdx.x, gridDim.x);
   if (slot >= nv) return;
   const int lane = threadIdx.x;
@@ -646,13 +652,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fm
       }
     }
   }
+  for (int d = lane; d < ndir; d += 64) s_pos[d] = 0xffff;
+  __syncthreads();
   const size_t cs = L.chunk;
 #pragma unroll
   for (int k = 0; k < kSortPer; ++k) {
     const int e = lane * kSortPer + k;
     if (e < cnt) {
+      s_pos[idx[k]] = (uint16_t)e;
       L.work.ord[(size_t)e * cs + slot] = (uint16_t)idx[k];
       L.work.rev[(size_t)e * cs + slot] = key[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSortPer; ++k) {
+    const int e = lane * kSortPer + k;
+    if (e < cnt) {
+      const int d = idx[k];
+      unsigned m = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int j = S.adj[d * 8 + r];
+        m |= (j >= 0 && (int)s_pos[j] < e) ? 1u << r : 0u;
+      }
+      L.work.nbr[(size_t)e * cs + slot] = (uint8_t)m;
     }
   }
   if (lane == 0) L.work.npos[slot] = cnt;
@@ -743,7 +767,7 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t 
     fmls_amp_kernel<<<grid, dim3(256), 0, stream>>>(S.sht, S.ndir, S.nsh, L.fod, L.ld_fod, v0, nv,
                                                     (size_t)fmls_amp_ld(S.ndir), L.work.amps);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    fmls_sort_kernel<<<dim3((unsigned)nv), dim3(64), 0, stream>>>(L, S.ndir, v0, nv);
+    fmls_sort_kernel<<<dim3((unsigned)nv), dim3(64), 0, stream>>>(S, L, v0, nv);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     switch (S.lmax) {
       case 2: e = launch_segment<2>(S, L, v0, nv, stream); break;

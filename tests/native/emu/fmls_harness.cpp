@@ -39,7 +39,7 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t)
           acc = std::fma(S.sht[size_t(d) * S.nsh + k], (double)L.fod[(v0 + v) * size_t(L.ld_fod) + k], acc);
         L.work.amps[v * lda + d] = acc;
       }
-    run_grid(unsigned(nv), [&]() { fmls_sort_kernel(L, S.ndir, v0, nv); });
+    run_grid(unsigned(nv), [&]() { fmls_sort_kernel(S, L, v0, nv); });
     switch (S.lmax) {
       case 2: run_segment<2>(S, L, v0, nv, nb); break;
       case 4: run_segment<4>(S, L, v0, nv, nb); break;
