@@ -213,15 +213,36 @@ __device__ Tabs tabs_of(const FmlsWork& W, size_t slot, size_t cs) {
               {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o}, {W.np, c, o},  {W.rev, c, o}};
 }
 
+// The running direction sum and integral of the lobe a voxel last seeded or grew, kept in
+// registers beside their copies in the work arrays (consecutive directions mostly join the same
+// lobe, so its record need not be read back); l = -1: nothing cached.
+struct LobeCache {
+  int l;
+  double m0, m1, m2, integ;
+};
+
+// a direction's adjacency row (8 int16 entries), loaded as one 16-byte access
+struct alignas(16) AdjRow {
+  uint32_t w[4];
+};
+__device__ __forceinline__ AdjRow adj_row(const FmlsDeviceSet& S, int d) {
+  return *reinterpret_cast<const AdjRow*>(S.adj + d * 8);
+}
+
 // a new lobe seeded by direction d (fmls.cpp:199-201, FOD_lobe's constructor fmls.h:64-76)
-__device__ void lobe_new(const FmlsDeviceSet& S, const Tabs& T, int l, int d, double v) {
+__device__ void lobe_new(const FmlsDeviceSet& S, const Tabs& T, int l, int d, double v, LobeCache& c) {
   const double* u = S.unit + 3 * d;
   const double w = S.weights[d];
+  c.l = l;
+  c.m0 = u[0] * fabs(v) * w;
+  c.m1 = u[1] * fabs(v) * w;
+  c.m2 = u[2] * fabs(v) * w;
+  c.integ = fabs(v * w);
   T.peak[l] = fabs(v);
-  T.mean[3 * l] = u[0] * fabs(v) * w;
-  T.mean[3 * l + 1] = u[1] * fabs(v) * w;
-  T.mean[3 * l + 2] = u[2] * fabs(v) * w;
-  T.integ[l] = fabs(v * w);
+  T.mean[3 * l] = c.m0;
+  T.mean[3 * l + 1] = c.m1;
+  T.mean[3 * l + 2] = c.m2;
+  T.integ[l] = c.integ;
   T.head[l] = (int16_t)d;
   T.tail[l] = (int16_t)d;
   T.np[l] = 1;
@@ -229,17 +250,27 @@ __device__ void lobe_new(const FmlsDeviceSet& S, const Tabs& T, int l, int d, do
   T.lab[d] = (int16_t)l;
 }
 
-// FOD_lobe::add (fmls.h:88-97)
-__device__ void lobe_add(const FmlsDeviceSet& S, const Tabs& T, int l, int d, double v) {
+// FOD_lobe::add (fmls.h:88-97), through the cache
+__device__ void lobe_add(const FmlsDeviceSet& S, const Tabs& T, int l, int d, double v, LobeCache& c) {
+  if (c.l != l) {
+    c.l = l;
+    c.m0 = T.mean[3 * l];
+    c.m1 = T.mean[3 * l + 1];
+    c.m2 = T.mean[3 * l + 2];
+    c.integ = T.integ[l];
+  }
   const double* u = S.unit + 3 * d;
   const double w = S.weights[d];
-  const double m0 = T.mean[3 * l], m1 = T.mean[3 * l + 1], m2 = T.mean[3 * l + 2];
-  const double mult = (m0 * u[0] + m1 * u[1] + m2 * u[2]) > 0.0 ? 1.0 : -1.0;
+  const double mult = (c.m0 * u[0] + c.m1 * u[1] + c.m2 * u[2]) > 0.0 ? 1.0 : -1.0;
   const double av = fabs(v);
-  T.mean[3 * l] = m0 + u[0] * mult * av * w;
-  T.mean[3 * l + 1] = m1 + u[1] * mult * av * w;
-  T.mean[3 * l + 2] = m2 + u[2] * mult * av * w;
-  T.integ[l] = T.integ[l] + fabs(v * w);
+  c.m0 = c.m0 + u[0] * mult * av * w;
+  c.m1 = c.m1 + u[1] * mult * av * w;
+  c.m2 = c.m2 + u[2] * mult * av * w;
+  c.integ = c.integ + fabs(v * w);
+  T.mean[3 * l] = c.m0;
+  T.mean[3 * l + 1] = c.m1;
+  T.mean[3 * l + 2] = c.m2;
+  T.integ[l] = c.integ;
 }
 
 // FOD_lobe::merge (fmls.h:112-127): B into A
@@ -284,11 +315,40 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
 
   // ---- lobe growth (fmls.cpp:188-249) ----
   int nl = 0;
-  // (the sort left each direction's amplitude beside it: rev[i] is the amplitude of ord[i])
+  int nw = 0;   // directions waiting for the retrospective assignment, listed over ord/rev[0, nw)
+  LobeCache lc{-1, 0.0, 0.0, 0.0, 0.0};
+  // The sort left each direction's amplitude beside it (rev[i] is the amplitude of ord[i]).  The
+  // loads of a step that depend on no earlier step -- direction, amplitude, neighbour mask two
+  // steps ahead, the adjacency row one step ahead -- are issued a step early.
+  int d1 = T.ord[0], d2 = 0;
+  double v1 = T.rev[0], v2 = 0.0;
+  unsigned m1 = T.nbr[0], m2 = 0;
+  AdjRow a1 = adj_row(S, d1);
+  if (P > 1) {
+    d2 = T.ord[1];
+    v2 = T.rev[1];
+    m2 = T.nbr[1];
+  }
   for (int i = 0; i < P; ++i) {
-    const int d = T.ord[i];
-    const double v = T.rev[i];
-    const unsigned seen_before = T.nbr[i];
+    const int d = d1;
+    const double v = v1;
+    const unsigned seen_before = m1;
+    const AdjRow arow = a1;
+    if (i + 1 < P) {
+      d1 = d2;
+      v1 = v2;
+      m1 = m2;
+      a1 = adj_row(S, d1);
+    }
+    if (i + 2 < P) {
+      d2 = T.ord[i + 2];
+      v2 = T.rev[i + 2];
+      m2 = T.nbr[i + 2];
+    }
+    // d's adjacency row: 8 int16 entries
+    int adj8[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) adj8[r] = (int16_t)((arow.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
     // the lobes of d's neighbours, distinct and ascending (INT_MAX pads); only neighbours visited
     // before d can carry one (the others still hold -1), so only their labels are read
     int al[8];
@@ -296,7 +356,7 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
     for (int q = 0; q < 8; ++q) al[q] = INT_MAX;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int lb = (seen_before >> r) & 1u ? (int)T.lab[S.adj[d * 8 + r]] : -1;   // waiting (<= -2): no lobe yet
+      const int lb = (seen_before >> r) & 1u ? (int)T.lab[adj8[r]] : -1;   // waiting (<= -2): no lobe yet
       if (lb >= 0) {
         bool seen = false;
 #pragma unroll
@@ -319,19 +379,21 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
         mx = al[q];
       }
     if (na == 0) {
-      lobe_new(S, T, nl, d, v);
+      lobe_new(S, T, nl, d, v, lc);
       ++nl;
     } else if (na == 1) {
       T.lab[d] = (int16_t)al[0];
-      lobe_add(S, T, al[0], d, v);
+      lobe_add(S, T, al[0], d, v, lc);
     } else if (fabs(v) / T.peak[mx] > S.merge_ratio) {
       // every adjacent lobe merged into the lowest-numbered one, in ascending order
       const int A = al[0];
+      lc.l = -1;   // the merges rewrite records and move lobes
 #pragma unroll
       for (int q = 1; q < 8; ++q)
         if (al[q] != INT_MAX) lobe_merge(T, A, al[q]);
       T.lab[d] = (int16_t)A;
-      lobe_add(S, T, A, d, v);
+      lobe_add(S, T, A, d, v, lc);
+      lc.l = -1;
       // merged lobes become A, the lobes above them move down over the erased records; the same
       // for the lobes waiting directions are assigned to (fmls.cpp:220-237)
       for (int dd = 0; dd < ndir; ++dd) {
@@ -359,18 +421,20 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
       }
     } else {
       T.lab[d] = (int16_t)(-2 - al[0]);
+      // listed in visiting order over entries already consumed (nw <= i)
+      T.ord[nw] = (uint16_t)d;
+      T.rev[nw] = v;
+      ++nw;
     }
   }
 
-  // retrospective assignments in visiting order (fmls.cpp:251-252)
-  for (int i = 0; i < P; ++i) {
-    const int d = T.ord[i];
-    const int lb = T.lab[d];
-    if (lb <= -2) {
-      const int l = -2 - lb;
-      T.lab[d] = (int16_t)l;
-      lobe_add(S, T, l, d, T.rev[i]);
-    }
+  // retrospective assignments in visiting order (fmls.cpp:251-252): the waiting directions, whose
+  // labels still name the lobe they wait for (renumbered by later merges, never assigned)
+  for (int w = 0; w < nw; ++w) {
+    const int d = T.ord[w];
+    const int l = -2 - (int)T.lab[d];
+    T.lab[d] = (int16_t)l;
+    lobe_add(S, T, l, d, T.rev[w], lc);
   }
 
   // ---- per lobe: integral threshold, peak revision, peak threshold (fmls.cpp:254-292) ----
