@@ -50,15 +50,17 @@ __device__ __forceinline__ int fm_mpos(int l, int m) { return l * l / 4 + m; }
 __device__ __forceinline__ int fm_sh(int l, int m) { return l * (l + 1) / 2 + m; }
 
 // Math::SH::derivatives with PrecomputedAL<double> (SH.h:568-647): out = {amplitude, d/del,
-// d/daz, d2/del2, d2/deldaz, d2/daz2}.  sh: the voxel's float32 coefficients.  The Legendre values
-// of the elevation, interpolated between two rows of the precomputed table (PrecomputedAL::set /
-// get, SH.h:436-470), go to this lane's own column of an LDS table (entry k at AL[kFmlsBlock k]),
-// read by no other lane.
+// d/daz, d2/del2, d2/deldaz, d2/daz2}.  sh: the voxel's float32 coefficients; el the elevation,
+// (caz, saz) the cosine and sine of the azimuth, sel the sine of the elevation.  The Legendre
+// values of the elevation, interpolated between two rows of the precomputed table
+// (PrecomputedAL::set / get, SH.h:436-470), go to this lane's own column of an LDS table (entry k
+// at AL[kFmlsBlock k]), read by no other lane.  cos(m az), sin(m az) come from the m = 1 values by
+// angle addition; up to lmax 8 the loops over l and m are unrolled, so their square-root
+// constants fold.
 template <int LMAX>
-__device__ void fmls_derivatives(const float* sh, double el, double az, const FmlsDeviceSet& S, double* AL,
-                                 double* out) {
+__device__ void fmls_derivatives(const float* sh, double el, double caz, double saz, double sel,
+                                 const FmlsDeviceSet& S, double* AL, double* out) {
   constexpr int NAL = (LMAX / 2 + 1) * (LMAX / 2 + 1);
-  const double sel = sin(el);
   const bool atpole = sel < 1e-4;
   {
     const double inc = M_PI / (S.al_ndir - 1);
@@ -88,6 +90,7 @@ __device__ void fmls_derivatives(const float* sh, double el, double az, const Fm
 #define FM_AL(k) AL[kFmlsBlock * (k)]
   double amplitude = (double)sh[0] * FM_AL(0);
   double dSH_del = 0.0, dSH_daz = 0.0, d2SH_del2 = 0.0, d2SH_deldaz = 0.0, d2SH_daz2 = 0.0;
+#pragma unroll
   for (int l = 2; l <= LMAX; l += 2) {
     const double v = (double)sh[fm_sh(l, 0)];
     amplitude += v * FM_AL(fm_mpos(l, 0));
@@ -95,33 +98,39 @@ __device__ void fmls_derivatives(const float* sh, double el, double az, const Fm
     d2SH_del2 += v * (sqrt((double)(l * (l + 1) * (l - 1) * (l + 2))) * FM_AL(fm_mpos(l, 2)) -
                       l * (l + 1) * FM_AL(fm_mpos(l, 0))) / 2.0;
   }
-#pragma unroll 1
+  double cm = caz, sm = saz;   // cos(m az), sin(m az)
+  constexpr int kUnrollM = LMAX <= 8 ? LMAX : 1;   // above lmax 8 the unrolled loop spills
+#pragma unroll kUnrollM
   for (int m = 1; m <= LMAX; ++m) {
-    const double caz = M_SQRT2 * cos(m * az);
-    const double saz = M_SQRT2 * sin(m * az);
+    const double caz_m = M_SQRT2 * cm;
+    const double saz_m = M_SQRT2 * sm;
+#pragma unroll
     for (int l = (m & 1) ? m + 1 : m; l <= LMAX; l += 2) {
       const double vp = (double)sh[fm_sh(l, m)];
       const double vm = (double)sh[fm_sh(l, -m)];
       const double alm = FM_AL(fm_mpos(l, m));
-      amplitude += (vp * caz + vm * saz) * alm;
+      amplitude += (vp * caz_m + vm * saz_m) * alm;
       double tmp = sqrt((double)((l + m) * (l - m + 1))) * FM_AL(fm_mpos(l, m - 1));
       if (l > m) tmp -= sqrt((double)((l - m) * (l + m + 1))) * FM_AL(fm_mpos(l, m + 1));
       tmp /= -2.0;
-      dSH_del += (vp * caz + vm * saz) * tmp;
+      dSH_del += (vp * caz_m + vm * saz_m) * tmp;
       double tmp2 = -((l + m) * (l - m + 1) + (l - m) * (l + m + 1)) * alm;
       if (m == 1) tmp2 -= sqrt((double)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * FM_AL(fm_mpos(l, 1));
       else tmp2 += sqrt((double)((l + m) * (l - m + 1) * (l + m - 1) * (l - m + 2))) * FM_AL(fm_mpos(l, m - 2));
       if (l > m + 1) tmp2 += sqrt((double)((l - m) * (l + m + 1) * (l - m - 1) * (l + m + 2))) * FM_AL(fm_mpos(l, m + 2));
       tmp2 /= 4.0;
-      d2SH_del2 += (vp * caz + vm * saz) * tmp2;
+      d2SH_del2 += (vp * caz_m + vm * saz_m) * tmp2;
       if (atpole) {
-        dSH_daz += (vm * caz - vp * saz) * tmp;
+        dSH_daz += (vm * caz_m - vp * saz_m) * tmp;
       } else {
-        d2SH_deldaz += m * (vm * caz - vp * saz) * tmp;
-        dSH_daz += m * (vm * caz - vp * saz) * alm;
-        d2SH_daz2 -= (vp * caz + vm * saz) * m * m * alm;
+        d2SH_deldaz += m * (vm * caz_m - vp * saz_m) * tmp;
+        dSH_daz += m * (vm * caz_m - vp * saz_m) * alm;
+        d2SH_daz2 -= (vp * caz_m + vm * saz_m) * m * m * alm;
       }
     }
+    const double cn = cm * caz - sm * saz;
+    sm = sm * caz + cm * saz;
+    cm = cn;
   }
   if (!atpole) {
     dSH_daz /= sel;
@@ -134,15 +143,29 @@ __device__ void fmls_derivatives(const float* sh, double el, double az, const Fm
 }
 
 // Math::SH::get_peak (SH.h:514-559): Newton steps along the gradient, at most 0.2 rad each,
-// until a step is below 1e-4 rad; NaN after 50 steps
+// until a step is below 1e-4 rad; NaN after 50 steps.  The azimuth's and elevation's cosines and
+// sines come from the unit direction's components (x / rho, y / rho, z, rho with rho = hypot(x, y));
+// only the elevation itself, which indexes the Legendre table, takes an atan2.  At the pole
+// (rho = 0) the azimuth is atan2(y, x) as the reference takes it.
 template <int LMAX>
 __device__ double fmls_get_peak(const float* sh, double* dir, const FmlsDeviceSet& S, double* AL) {
   const double MAX_DIR_CHANGE = 0.2, ANGLE_TOLERANCE = 1e-4;
   for (int it = 0; it < 50; ++it) {
-    const double az = atan2(dir[1], dir[0]);
-    const double el = atan2(hypot(dir[0], dir[1]), dir[2]);
+    const double rho = hypot(dir[0], dir[1]);
+    const double el = atan2(rho, dir[2]);
+    double ca, sa;
+    if (rho > 0.0) {
+      ca = dir[0] / rho;
+      sa = dir[1] / rho;
+    } else {
+      const double az = atan2(dir[1], dir[0]);
+      ca = cos(az);
+      sa = sin(az);
+    }
+    const double r = hypot(rho, dir[2]);
+    const double ce = dir[2] / r, se = rho / r;
     double d[6];
-    fmls_derivatives<LMAX>(sh, el, az, S, AL, d);
+    fmls_derivatives<LMAX>(sh, el, ca, sa, se, S, AL, d);
     double del = sqrt(d[1] * d[1] + d[2] * d[2]);
     double daz = 0.0;
     if (del != 0.0) {
@@ -156,7 +179,6 @@ __device__ double fmls_get_peak(const float* sh, double* dir, const FmlsDeviceSe
     if (dt > MAX_DIR_CHANGE) dt = MAX_DIR_CHANGE;
     del *= dt;
     daz *= dt;
-    const double ca = cos(az), sa = sin(az), ce = cos(el), se = sin(el);
     dir[0] += del * ca * ce - daz * sa;
     dir[1] += del * sa * ce + daz * ca;
     dir[2] -= del * se;
