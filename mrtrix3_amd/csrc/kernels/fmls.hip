@@ -371,42 +371,45 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
     int adj8[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) adj8[r] = (int16_t)((arow.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
-    // the lobes of d's neighbours, distinct and ascending (INT_MAX pads); only neighbours visited
-    // before d can carry one (the others still hold -1), so only their labels are read
-    int al[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) al[q] = INT_MAX;
+    // the lowest- and highest-numbered lobes among d's neighbours; only neighbours visited before
+    // d can carry one (the others still hold -1), so only their labels are read, and waiting
+    // directions (<= -2) belong to no lobe yet
+    int lbs[8];
+    int mn = INT_MAX, mx = -1;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int lb = (seen_before >> r) & 1u ? (int)T.lab[adj8[r]] : -1;   // waiting (<= -2): no lobe yet
-      if (lb >= 0) {
+      lbs[r] = (seen_before >> r) & 1u ? (int)T.lab[adj8[r]] : -1;
+      if (lbs[r] >= 0) {
+        mn = min(mn, lbs[r]);
+        mx = max(mx, lbs[r]);
+      }
+    }
+    if (mx < 0) {
+      lobe_new(S, T, nl, d, v, lc);
+      ++nl;
+    } else if (mn == mx) {
+      T.lab[d] = (int16_t)mn;
+      lobe_add(S, T, mn, d, v, lc);
+    } else if (fabs(v) / T.peak[mx] > S.merge_ratio) {
+      // the adjacent lobes, distinct and ascending (INT_MAX pads)
+      int al[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) al[q] = INT_MAX;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int lb = lbs[r];
+        if (lb < 0) continue;
         bool seen = false;
 #pragma unroll
         for (int q = 0; q < 8; ++q) seen = seen || al[q] == lb;
-        if (!seen) {
+        if (seen) continue;
 #pragma unroll
-          for (int q = 7; q > 0; --q) {
-            if (al[q - 1] > lb) al[q] = al[q - 1];
-            else if (al[q] > lb) al[q] = lb;
-          }
-          if (al[0] > lb) al[0] = lb;
+        for (int q = 7; q > 0; --q) {
+          if (al[q - 1] > lb) al[q] = al[q - 1];
+          else if (al[q] > lb) al[q] = lb;
         }
+        if (al[0] > lb) al[0] = lb;
       }
-    }
-    int na = 0, mx = -1;
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (al[q] != INT_MAX) {
-        ++na;
-        mx = al[q];
-      }
-    if (na == 0) {
-      lobe_new(S, T, nl, d, v, lc);
-      ++nl;
-    } else if (na == 1) {
-      T.lab[d] = (int16_t)al[0];
-      lobe_add(S, T, al[0], d, v, lc);
-    } else if (fabs(v) / T.peak[mx] > S.merge_ratio) {
       // every adjacent lobe merged into the lowest-numbered one, in ascending order
       const int A = al[0];
       lc.l = -1;   // the merges rewrite records and move lobes
@@ -442,7 +445,7 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
         --nl;
       }
     } else {
-      T.lab[d] = (int16_t)(-2 - al[0]);
+      T.lab[d] = (int16_t)(-2 - mn);
       // listed in visiting order over entries already consumed (nw <= i)
       T.ord[nw] = (uint16_t)d;
       T.rev[nw] = v;
