@@ -192,6 +192,13 @@ int mrfixel_create(const mrfixel_options* opt, mrfixel_ctx** out) {
     S.merge_ratio = opt->lobe_merge_ratio;
     S.maxnum = opt->maxnum;
     S.dir_from_peak = opt->dir_from_peak ? 1 : 0;
+    int lds_cu = 0, lds_block = 0;
+    MRCSD_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device));
+    MRCSD_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+    S.lanes8 = mrcsd::fmls_grow_lanes(s.n, 1, lds_cu, lds_block);
+    S.lanes16 = mrcsd::fmls_grow_lanes(s.n, 2, lds_cu, lds_block);
+    if (S.lanes16 < 1)
+      return mrcsd::fail(MRCSD_ERR_NOGPU, "mrfixel_create: the device's LDS cannot hold a voxel's lobe labels");
     *out = ctx.release();
     return MRCSD_OK;
   });

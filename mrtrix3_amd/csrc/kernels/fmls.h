@@ -31,7 +31,17 @@ struct FmlsDeviceSet {
   double integral_threshold, peak_threshold, merge_ratio;
   int maxnum;                // fixels kept per voxel, 0 = all
   int dir_from_peak;         // fixel direction: 1 the maximal peak, 0 the lobe's mean direction
+  int lanes8, lanes16;       // voxels per workgroup of the growth kernels with 8- and 16-bit labels
 };
+
+// Voxels per workgroup of a growth kernel whose LDS labels take `bytes` each (ndir per voxel): as
+// many as let two workgroups share a CU's LDS within one workgroup's limit, at most 64.
+inline int fmls_grow_lanes(int ndir, int bytes, int lds_cu, int lds_block) {
+  const long cu = lds_cu > 0 ? lds_cu : lds_block;
+  const long wg = cu / 2 < lds_block ? cu / 2 : lds_block;
+  const long n = wg / ((long)ndir * bytes);
+  return (int)(n < 64 ? n : 64);
+}
 
 // Per-voxel work arrays of one chunk.  The amplitudes are voxel-major (row s of fmls_amp_ld(ndir)
 // doubles for the voxel in slot s), read by one wavefront per voxel; the other arrays are

@@ -8,24 +8,28 @@
 //   * fmls_sort_kernel: a wavefront per voxel checks that the largest |amplitude| is positive and
 //     sorts the positive directions by decreasing amplitude, the lowest index first among equal
 //     amplitudes (the reference's multimap order), by a bitonic sort in registers;
-//   * fmls_segment_kernel<LMAX>: each lane segments its own voxel, step for step as the reference:
-//       - lobe growth in visiting order: a direction adjacent to no positive lobe seeds one,
-//         adjacent to one joins it, adjacent to several either merges them (bridge amplitude over
-//         the peak of the highest-numbered adjacent lobe above the merge ratio) or waits, outside
-//         every lobe, for its assignment to the lowest-numbered of them after the loop;
-//       - per lobe: the integral threshold, the Newton peak search from each of its peaks
-//         (Math::SH::get_peak in double with the precomputed Legendre table, SH.h:398-559), a
-//         revision kept only when it stays nearest to that peak and inside the lobe (nearest
-//         direction through the FastLookupSet grid), the peak threshold, the fixel direction;
-//       - the voxel's fixel count and summary; the kept lobes stay in the work arrays;
+//   * fmls_grow_kernel<Lab8>: lobe growth, one lane per voxel, step for step as the reference: a
+//     direction adjacent to no positive lobe seeds one, adjacent to one joins it, adjacent to
+//     several either merges them (bridge amplitude over the peak of the highest-numbered adjacent
+//     lobe above the merge ratio) or waits, outside every lobe, for its assignment to the
+//     lowest-numbered of them after the loop.  Each lane's lobe labels are an 8-bit column in LDS
+//     (as many voxels per workgroup as let two share a CU: 63 in 160 KB); the sorted lists and direction tables are
+//     loaded steps ahead; lobe sums are kept in a register cache.  A voxel of more than 127 lobes
+//     is left to fmls_grow_kernel<Lab16> (16-bit labels, 31 voxels per workgroup), which grows it
+//     again from the unchanged sorted lists.  The final labels go to the work arrays;
+//   * fmls_segment_kernel<LMAX>: per lobe, the integral threshold, the Newton peak search from each
+//     of its peaks (Math::SH::get_peak in double with the precomputed Legendre table,
+//     SH.h:398-559), a revision kept only when it stays nearest to that peak and inside the lobe
+//     (nearest direction through the FastLookupSet grid), the peak threshold, the fixel direction;
+//     then the voxel's fixel count and summary; the kept lobes stay in the work arrays;
 //   * fmls_block_sums, fmls_scan_sums: the fixel offset of each block of 64 voxels, continuing
 //     the running total of the earlier chunks;
 //   * fmls_emit: each voxel's offset in voxel order (the order fod2fixel's index image uses,
 //     cmd/fod2fixel.cpp:239-280; 0 for voxels without fixels) and its fixels.
 // The work arrays hold every table a voxel can need (as many lobes and peaks as directions), so
-// no voxel is deferred, and the segmentation has no cross-lane operation, barrier or atomic (its
-// LDS is a column per lane for the Newton search's Legendre values): the cooperation between
-// lanes is the sort's exchanges within a wavefront and the workgroup scans of the offset kernels.
+// no voxel is deferred.  The growth and revision have no cross-lane operation (their LDS is a
+// column per lane): the cooperation between lanes is the sort's exchanges within a wavefront and
+// the workgroup scans of the offset kernels.
 // Negative directions only ever form negative lobes, which the reference discards and which take
 // no part in positive lobes' adjacency, order, merges or indices, so they are not visited.
 #include <hip/hip_runtime.h>
@@ -235,14 +239,6 @@ __device__ Tabs tabs_of(const FmlsWork& W, size_t slot, size_t cs) {
               {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o}, {W.np, c, o},  {W.rev, c, o}};
 }
 
-// The running direction sum and integral of the lobe a voxel last seeded or grew, kept in
-// registers beside their copies in the work arrays (consecutive directions mostly join the same
-// lobe, so its record need not be read back); l = -1: nothing cached.
-struct LobeCache {
-  int l;
-  double m0, m1, m2, integ;
-};
-
 // a direction's adjacency row (8 int16 entries), loaded as one 16-byte access
 struct alignas(16) AdjRow {
   uint32_t w[4];
@@ -251,48 +247,117 @@ __device__ __forceinline__ AdjRow adj_row(const FmlsDeviceSet& S, int d) {
   return *reinterpret_cast<const AdjRow*>(S.adj + d * 8);
 }
 
-// a new lobe seeded by direction d (fmls.cpp:199-201, FOD_lobe's constructor fmls.h:64-76)
-__device__ void lobe_new(const FmlsDeviceSet& S, const Tabs& T, int l, int d, double v, LobeCache& c) {
+// What a growth step needs of its direction d from the direction tables: unit vector,
+// integration weight, adjacency row.
+struct DirInfo {
+  double u0, u1, u2, w;
+  AdjRow adj;
+};
+__device__ __forceinline__ DirInfo dir_info(const FmlsDeviceSet& S, int d) {
   const double* u = S.unit + 3 * d;
-  const double w = S.weights[d];
-  c.l = l;
-  c.m0 = u[0] * fabs(v) * w;
-  c.m1 = u[1] * fabs(v) * w;
-  c.m2 = u[2] * fabs(v) * w;
-  c.integ = fabs(v * w);
+  return DirInfo{u[0], u[1], u[2], S.weights[d], adj_row(S, d)};
+}
+
+// Write-back cache of lobe records in registers: the running direction sum and integral of up to
+// four lobes (entry l & 3 holds lobe l), written to the work arrays when the entry is taken by
+// another lobe or flushed.  tag -1: empty.
+struct LobeCache {
+  int tag[4];
+  double m0[4], m1[4], m2[4], integ[4];
+};
+
+__device__ __forceinline__ void lc_init(LobeCache& c) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) c.tag[q] = -1;
+}
+
+// every entry back to the work arrays, the cache emptied
+__device__ void lc_flush(const Tabs& T, LobeCache& c) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int l = c.tag[q];
+    if (l >= 0) {
+      T.mean[3 * l] = c.m0[q];
+      T.mean[3 * l + 1] = c.m1[q];
+      T.mean[3 * l + 2] = c.m2[q];
+      T.integ[l] = c.integ[q];
+    }
+    c.tag[q] = -1;
+  }
+}
+
+// entry l & 3 given to lobe l: its previous lobe written back, l's record read unless `fresh`
+// (a new lobe whose values the caller sets); the entry's values returned in m0..integ
+__device__ __forceinline__ void lc_take(const Tabs& T, LobeCache& c, int l, bool fresh, double& m0, double& m1,
+                                        double& m2, double& integ) {
+  const int q = l & 3;
+  int tg = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k == q) {
+      tg = c.tag[k];
+      m0 = c.m0[k];
+      m1 = c.m1[k];
+      m2 = c.m2[k];
+      integ = c.integ[k];
+    }
+  if (tg != l) {
+    if (tg >= 0) {
+      T.mean[3 * tg] = m0;
+      T.mean[3 * tg + 1] = m1;
+      T.mean[3 * tg + 2] = m2;
+      T.integ[tg] = integ;
+    }
+    if (!fresh) {
+      m0 = T.mean[3 * l];
+      m1 = T.mean[3 * l + 1];
+      m2 = T.mean[3 * l + 2];
+      integ = T.integ[l];
+    }
+  }
+}
+
+__device__ __forceinline__ void lc_put(LobeCache& c, int l, double m0, double m1, double m2, double integ) {
+  const int q = l & 3;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k == q) {
+      c.tag[k] = l;
+      c.m0[k] = m0;
+      c.m1[k] = m1;
+      c.m2[k] = m2;
+      c.integ[k] = integ;
+    }
+}
+
+// a new lobe seeded by direction d of amplitude v (fmls.cpp:199-201, FOD_lobe's constructor
+// fmls.h:64-76)
+__device__ void lobe_new(const Tabs& T, int l, int d, double v, const DirInfo& di, LobeCache& c) {
+  double m0, m1, m2, integ;
+  lc_take(T, c, l, true, m0, m1, m2, integ);
+  m0 = di.u0 * fabs(v) * di.w;
+  m1 = di.u1 * fabs(v) * di.w;
+  m2 = di.u2 * fabs(v) * di.w;
+  integ = fabs(v * di.w);
+  lc_put(c, l, m0, m1, m2, integ);
   T.peak[l] = fabs(v);
-  T.mean[3 * l] = c.m0;
-  T.mean[3 * l + 1] = c.m1;
-  T.mean[3 * l + 2] = c.m2;
-  T.integ[l] = c.integ;
   T.head[l] = (int16_t)d;
   T.tail[l] = (int16_t)d;
   T.np[l] = 1;
   T.nxt[d] = -1;
-  T.lab[d] = (int16_t)l;
 }
 
 // FOD_lobe::add (fmls.h:88-97), through the cache
-__device__ void lobe_add(const FmlsDeviceSet& S, const Tabs& T, int l, int d, double v, LobeCache& c) {
-  if (c.l != l) {
-    c.l = l;
-    c.m0 = T.mean[3 * l];
-    c.m1 = T.mean[3 * l + 1];
-    c.m2 = T.mean[3 * l + 2];
-    c.integ = T.integ[l];
-  }
-  const double* u = S.unit + 3 * d;
-  const double w = S.weights[d];
-  const double mult = (c.m0 * u[0] + c.m1 * u[1] + c.m2 * u[2]) > 0.0 ? 1.0 : -1.0;
+__device__ void lobe_add(const Tabs& T, int l, double v, const DirInfo& di, LobeCache& c) {
+  double m0, m1, m2, integ;
+  lc_take(T, c, l, false, m0, m1, m2, integ);
+  const double mult = (m0 * di.u0 + m1 * di.u1 + m2 * di.u2) > 0.0 ? 1.0 : -1.0;
   const double av = fabs(v);
-  c.m0 = c.m0 + u[0] * mult * av * w;
-  c.m1 = c.m1 + u[1] * mult * av * w;
-  c.m2 = c.m2 + u[2] * mult * av * w;
-  c.integ = c.integ + fabs(v * w);
-  T.mean[3 * l] = c.m0;
-  T.mean[3 * l + 1] = c.m1;
-  T.mean[3 * l + 2] = c.m2;
-  T.integ[l] = c.integ;
+  m0 = m0 + di.u0 * mult * av * di.w;
+  m1 = m1 + di.u1 * mult * av * di.w;
+  m2 = m2 + di.u2 * mult * av * di.w;
+  integ = integ + fabs(v * di.w);
+  lc_put(c, l, m0, m1, m2, integ);
 }
 
 // FOD_lobe::merge (fmls.h:112-127): B into A
@@ -327,141 +392,190 @@ __device__ void lobe_copy(const Tabs& T, int dst, int src) {
   T.np[dst] = T.np[src];
 }
 
-// The segmentation of one voxel (fmls.cpp:166-301) from its P > 0 positive directions in visiting
-// order (ord, with their amplitudes in rev); returns the number of positive lobes, whose records
-// hold the kept fixels (np < 0: dropped by a threshold).
-template <int LMAX>
-__device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, const float* f, double* AL) {
-  const int ndir = S.ndir;
-  for (int d = 0; d < ndir; ++d) T.lab[d] = -1;
+// A lane's column of lobe labels in LDS: entry d at p[d * s].  -1: no lobe (not visited, or
+// negative); l >= 0: lobe l; -2 - l: waiting for lobe l.  CAP: the lobes the type can number.
+template <class V, int CAP>
+struct LdsLab {
+  using value_type = V;
+  V* p;
+  int s;
+  static constexpr int cap = CAP;
+  __device__ __forceinline__ int get(int d) const { return p[d * s]; }
+  __device__ __forceinline__ void set(int d, int v) const { p[d * s] = (V)v; }
+};
+using Lab8 = LdsLab<int8_t, 127>;        // waiting labels down to -128: lobes 0..126
+using Lab16 = LdsLab<int16_t, 32767>;    // any direction count
 
-  // ---- lobe growth (fmls.cpp:188-249) ----
-  int nl = 0;
-  int nw = 0;   // directions waiting for the retrospective assignment, listed over ord/rev[0, nw)
-  LobeCache lc{-1, 0.0, 0.0, 0.0, 0.0};
-  // The sort left each direction's amplitude beside it (rev[i] is the amplitude of ord[i]).  The
-  // loads of a step that depend on no earlier step -- direction, amplitude, neighbour mask two
-  // steps ahead, the adjacency row one step ahead -- are issued a step early.
-  int d1 = T.ord[0], d2 = 0;
-  double v1 = T.rev[0], v2 = 0.0;
-  unsigned m1 = T.nbr[0], m2 = 0;
-  AdjRow a1 = adj_row(S, d1);
-  if (P > 1) {
-    d2 = T.ord[1];
-    v2 = T.rev[1];
-    m2 = T.nbr[1];
+// Lobe growth and the retrospective assignments of one voxel (fmls.cpp:188-252) from its P > 0
+// positive directions in visiting order (ord, with their amplitudes in rev and the
+// visited-before neighbour masks in nbr), its labels in `lab`; returns the number of positive
+// lobes, or -1 when the voxel needs more lobes than `lab` can number (nothing in ord, rev or nbr
+// is changed, so the voxel can be grown again with wider labels).
+// The ascending distinct lobes among d's neighbour labels (INT_MAX pads), each lobe merged into
+// the lowest-numbered one in ascending order, d added to it, and the labels and lobe records
+// renumbered over the erased records (fmls.cpp:203-237); only lobe merge ratios below 1 can
+// merge.  The caller's cache is flushed; this uses its own.  Returns the new lobe count.
+template <class Lab>
+__device__ __forceinline__ int grow_merge(const FmlsDeviceSet& S, const Tabs& T, Lab lab, const int* lbs, int d, double v,
+                                       double u0, double u1, double u2, double w, int nl) {
+  const DirInfo di{u0, u1, u2, w, AdjRow{}};
+  LobeCache lc;
+  lc_init(lc);
+  int al[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) al[q] = INT_MAX;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int lb = lbs[r];
+    if (lb < 0) continue;
+    bool seen = false;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) seen = seen || al[q] == lb;
+    if (seen) continue;
+#pragma unroll
+    for (int q = 7; q > 0; --q) {
+      if (al[q - 1] > lb) al[q] = al[q - 1];
+      else if (al[q] > lb) al[q] = lb;
+    }
+    if (al[0] > lb) al[0] = lb;
   }
-  for (int i = 0; i < P; ++i) {
-    const int d = d1;
-    const double v = v1;
-    const unsigned seen_before = m1;
-    const AdjRow arow = a1;
-    if (i + 1 < P) {
-      d1 = d2;
-      v1 = v2;
-      m1 = m2;
-      a1 = adj_row(S, d1);
-    }
-    if (i + 2 < P) {
-      d2 = T.ord[i + 2];
-      v2 = T.rev[i + 2];
-      m2 = T.nbr[i + 2];
-    }
-    // d's adjacency row: 8 int16 entries
-    int adj8[8];
+  const int A = al[0];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) adj8[r] = (int16_t)((arow.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
-    // the lowest- and highest-numbered lobes among d's neighbours; only neighbours visited before
-    // d can carry one (the others still hold -1), so only their labels are read, and waiting
-    // directions (<= -2) belong to no lobe yet
-    int lbs[8];
-    int mn = INT_MAX, mx = -1;
+  for (int q = 1; q < 8; ++q)
+    if (al[q] != INT_MAX) lobe_merge(T, A, al[q]);
+  lab.set(d, A);
+  lobe_add(T, A, v, di, lc);
+  lc_flush(T, lc);   // records move below
+  // merged lobes become A, the lobes above them move down over the erased records; the same for
+  // the lobes waiting directions are assigned to
+  for (int dd = 0; dd < S.ndir; ++dd) {
+    const int lb = lab.get(dd);
+    const bool wait = lb <= -2;
+    int l = wait ? -2 - lb : lb;
+    if (l < 0) continue;
+    bool merged = false;
+    int sh = 0;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      lbs[r] = (seen_before >> r) & 1u ? (int)T.lab[adj8[r]] : -1;
-      if (lbs[r] >= 0) {
-        mn = min(mn, lbs[r]);
-        mx = max(mx, lbs[r]);
+    for (int q = 1; q < 8; ++q)
+      if (al[q] != INT_MAX) {
+        merged = merged || l == al[q];
+        sh += al[q] < l ? 1 : 0;
       }
-    }
-    if (mx < 0) {
-      lobe_new(S, T, nl, d, v, lc);
-      ++nl;
-    } else if (mn == mx) {
-      T.lab[d] = (int16_t)mn;
-      lobe_add(S, T, mn, d, v, lc);
-    } else if (fabs(v) / T.peak[mx] > S.merge_ratio) {
-      // the adjacent lobes, distinct and ascending (INT_MAX pads)
-      int al[8];
+    if (!merged && sh == 0) continue;
+    l = merged ? A : l - sh;
+    lab.set(dd, wait ? -2 - l : l);
+  }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) al[q] = INT_MAX;
+  for (int q = 7; q >= 1; --q) {
+    if (al[q] == INT_MAX) continue;
+    for (int l = al[q]; l < nl - 1; ++l) lobe_copy(T, l, l + 1);
+    --nl;
+  }
+  return nl;
+}
+
+// Lobe growth and the retrospective assignments of one voxel (fmls.cpp:188-252) from its P > 0
+// positive directions in visiting order (ord, with their amplitudes in rev and the
+// visited-before neighbour masks in nbr), its labels in `lab`; returns the number of positive
+// lobes, or -1 when the voxel needs more lobes than `lab` can number (nothing in ord, rev or nbr
+// is changed, so the voxel can be grown again with wider labels).
+//
+// Every global load of a step is issued ahead of it: direction, neighbour mask and amplitude
+// eight steps ahead (ring ra/rv), the direction's unit vector, weight and adjacency row four
+// steps ahead (ring rb); the step loop is unrolled by eight so that each ring slot is a fixed
+// register.  Labels live in LDS and lobe sums in the register cache, so a step usually issues no
+// global access of its own.
+template <class Lab>
+__device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab& lab, int P) {
+  constexpr int KA = 8, KB = 4;
+  for (int d = 0; d < S.ndir; ++d) lab.set(d, -1);
+  int nl = 0;
+  LobeCache lc;
+  lc_init(lc);
+  int ra[KA];      // ord | nbr << 16
+  double rv[KA];   // amplitude
+  DirInfo rb[KB];
+  // Ring loads are unconditional, their indices clamped to the last entry (a load past the end is
+  // never used and stays in bounds), and issued after the step that last used the slot: each slot
+  // is then written by its load alone, with no copy that would wait for a load in flight.
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int lb = lbs[r];
-        if (lb < 0) continue;
-        bool seen = false;
+  for (int k = 0; k < KA; ++k) {
+    const int ia = k < P ? k : P - 1;
+    ra[k] = (int)T.ord[ia] | ((int)T.nbr[ia] << 16);
+    rv[k] = T.rev[ia];
+  }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) seen = seen || al[q] == lb;
-        if (seen) continue;
+  for (int k = 0; k < KB; ++k) rb[k] = dir_info(S, ra[k] & 0xffff);
+  const bool can_merge = S.merge_ratio < 1.0;   // (a direction's amplitude never exceeds a visited peak)
+  for (int i0 = 0; i0 < P; i0 += KA) {
 #pragma unroll
-        for (int q = 7; q > 0; --q) {
-          if (al[q - 1] > lb) al[q] = al[q - 1];
-          else if (al[q] > lb) al[q] = lb;
-        }
-        if (al[0] > lb) al[0] = lb;
-      }
-      // every adjacent lobe merged into the lowest-numbered one, in ascending order
-      const int A = al[0];
-      lc.l = -1;   // the merges rewrite records and move lobes
+    for (int j = 0; j < KA; ++j) {
+      const int i = i0 + j;
+      if (i < P) {
+        const int d = ra[j] & 0xffff;
+        const unsigned seen_before = (unsigned)ra[j] >> 16;
+        const double v = rv[j];
+        const DirInfo& di = rb[j % KB];
+        // the lowest- and highest-numbered lobes among d's neighbours; only neighbours visited
+        // before d can carry one (the others still hold -1), so only their labels are read, and
+        // waiting directions (<= -2) belong to no lobe yet
+        int lbs[8];
+        int mn = INT_MAX, mx = -1;
 #pragma unroll
-      for (int q = 1; q < 8; ++q)
-        if (al[q] != INT_MAX) lobe_merge(T, A, al[q]);
-      T.lab[d] = (int16_t)A;
-      lobe_add(S, T, A, d, v, lc);
-      lc.l = -1;
-      // merged lobes become A, the lobes above them move down over the erased records; the same
-      // for the lobes waiting directions are assigned to (fmls.cpp:220-237)
-      for (int dd = 0; dd < ndir; ++dd) {
-        const int lb = T.lab[dd];
-        const bool wait = lb <= -2;
-        int l = wait ? -2 - lb : lb;
-        if (l < 0) continue;
-        bool merged = false;
-        int sh = 0;
-#pragma unroll
-        for (int q = 1; q < 8; ++q)
-          if (al[q] != INT_MAX) {
-            merged = merged || l == al[q];
-            sh += al[q] < l ? 1 : 0;
+        for (int r = 0; r < 8; ++r) {
+          const int jr = (int16_t)((di.adj.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
+          lbs[r] = (seen_before >> r) & 1u ? lab.get(jr) : -1;
+          if (lbs[r] >= 0) {
+            mn = min(mn, lbs[r]);
+            mx = max(mx, lbs[r]);
           }
-        if (!merged && sh == 0) continue;
-        l = merged ? A : l - sh;
-        T.lab[dd] = (int16_t)(wait ? -2 - l : l);
-      }
+        }
+        if (mx < 0) {
+          if (nl == Lab::cap) return -1;
+          lobe_new(T, nl, d, v, di, lc);
+          lab.set(d, nl);
+          ++nl;
+        } else if (mn == mx) {
+          lab.set(d, mn);
+          lobe_add(T, mn, v, di, lc);
+        } else if (can_merge && fabs(v) / T.peak[mx] > S.merge_ratio) {
+          lc_flush(T, lc);   // the merges read and rewrite records
+          int lbc[8];
 #pragma unroll
-      for (int q = 7; q >= 1; --q) {
-        if (al[q] == INT_MAX) continue;
-        for (int l = al[q]; l < nl - 1; ++l) lobe_copy(T, l, l + 1);
-        --nl;
+          for (int r = 0; r < 8; ++r) lbc[r] = lbs[r];
+          nl = grow_merge(S, T, lab, lbc, d, v, di.u0, di.u1, di.u2, di.w, nl);
+        } else {
+          lab.set(d, -2 - mn);
+        }
       }
-    } else {
-      T.lab[d] = (int16_t)(-2 - mn);
-      // listed in visiting order over entries already consumed (nw <= i)
-      T.ord[nw] = (uint16_t)d;
-      T.rev[nw] = v;
-      ++nw;
+      // refills: entry i + KA into slot j, the direction tables of step i + KB into slot j % KB
+      const int ia = i + KA < P ? i + KA : P - 1;
+      ra[j] = (int)T.ord[ia] | ((int)T.nbr[ia] << 16);
+      rv[j] = T.rev[ia];
+      rb[j % KB] = dir_info(S, ra[(j + KB) % KA] & 0xffff);
     }
   }
 
   // retrospective assignments in visiting order (fmls.cpp:251-252): the waiting directions, whose
   // labels still name the lobe they wait for (renumbered by later merges, never assigned)
-  for (int w = 0; w < nw; ++w) {
-    const int d = T.ord[w];
-    const int l = -2 - (int)T.lab[d];
-    T.lab[d] = (int16_t)l;
-    lobe_add(S, T, l, d, T.rev[w], lc);
+  for (int i = 0; i < P; ++i) {
+    const int d = T.ord[i];
+    const int lb = lab.get(d);
+    if (lb <= -2) {
+      const int l = -2 - lb;
+      lab.set(d, l);
+      lobe_add(T, l, T.rev[i], dir_info(S, d), lc);
+    }
   }
+  lc_flush(T, lc);
+  return nl;
+}
 
+// The per-lobe steps of one voxel's segmentation (fmls.cpp:254-292) over its nl grown lobes, the
+// final labels in the work arrays; the lobe records then hold the kept fixels (np < 0: dropped by a
+// threshold).
+template <int LMAX>
+__device__ void fmls_revise_voxel(const FmlsDeviceSet& S, const Tabs& T, int nl, const float* f, double* AL) {
   // ---- per lobe: integral threshold, peak revision, peak threshold (fmls.cpp:254-292) ----
   int run = 0;
   for (int l = 0; l < nl; ++l) {
@@ -519,7 +633,6 @@ __device__ int fmls_segment_voxel(const FmlsDeviceSet& S, const Tabs& T, int P, 
     }
     run += np;
   }
-  return nl;
 }
 
 template <int LMAX>
@@ -531,8 +644,8 @@ __global__ __launch_bounds__(kFmlsBlock) void fmls_segment_kernel(FmlsDeviceSet 
   const size_t vox = v0 + slot;
   const Tabs T = tabs_of(L.work, slot, L.chunk);
   const float* f = L.fod + vox * (size_t)L.ld_fod;
-  const int P = L.work.npos[slot];
-  const int nl = P > 0 ? fmls_segment_voxel<LMAX>(S, T, P, f, s_al + threadIdx.x) : 0;
+  const int nl = L.work.nlobes[slot];
+  if (nl > 0) fmls_revise_voxel<LMAX>(S, T, nl, f, s_al + threadIdx.x);
   // the kept lobes in order, at most maxnum of them (fod2fixel.cpp:139-150)
   int nk = 0;
   double sp0 = 0.0, sp1 = 0.0, sx = NAN, sy = NAN, sz = NAN;
@@ -549,7 +662,6 @@ __global__ __launch_bounds__(kFmlsBlock) void fmls_segment_kernel(FmlsDeviceSet 
     }
     ++nk;
   }
-  L.work.nlobes[slot] = nl;
   L.count[vox] = (uint32_t)nk;
   if (L.summary) {
     float* o = L.summary + vox * 5;
@@ -771,6 +883,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fm
   if (lane == 0) L.work.npos[slot] = cnt;
 }
 
+// LDS of the growth kernels, sized at launch: `lanes` columns of ndir labels
+#ifndef FMLS_GROW_LDS
+#define FMLS_GROW_LDS(name) extern __shared__ unsigned char name[]
+#endif
+
+// Lobe growth for the voxels of the chunk, one lane per voxel, `lanes` voxels per workgroup, each
+// lane's labels in its own LDS column (so the neighbour-label reads of the growth never leave the
+// CU).  The final labels go to the work arrays for the per-lobe kernel; nlobes[slot] = the lobe
+// count, or -1 when the labels could not number the voxel's lobes.  redo: only the voxels left at
+// -1 by an earlier pass are grown.
+template <class Lab>
+__global__ __launch_bounds__(64) void fmls_grow_kernel(FmlsDeviceSet S, FmlsLaunch L, size_t nv, int lanes, int redo) {
+  FMLS_GROW_LDS(s_lab);
+  const int lane = threadIdx.x;
+  const size_t slot = (size_t)blockIdx.x * lanes + lane;
+  if (lane >= lanes || slot >= nv) return;
+  if (redo && L.work.nlobes[slot] != -1) return;
+  const int P = L.work.npos[slot];
+  int nl = 0;
+  if (P > 0) {
+    const Tabs T = tabs_of(L.work, slot, L.chunk);
+    const Lab lab{reinterpret_cast<typename Lab::value_type*>(s_lab) + lane, lanes};
+    nl = fmls_grow_voxel(S, T, lab, P);
+    if (nl >= 0)
+      for (int d = 0; d < S.ndir; ++d) T.lab[d] = (int16_t)lab.get(d);
+  }
+  L.work.nlobes[slot] = nl;
+}
+
 // ---- HIP launches ----
 
 constexpr int kTileD = 128, kTileV = 64, kTileK = 16;
@@ -846,6 +987,7 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t 
   if (S.ndir < 64 || S.ndir > kFmlsMaxDirs || L.chunk == 0 || L.chunk % kFmlsBlock || L.chunk > (1u << 17))
     return hipErrorInvalidValue;
   if (S.nsh != (S.lmax + 1) * (S.lmax + 2) / 2 || L.ld_fod < S.nsh) return hipErrorInvalidValue;
+  if (S.lanes8 < 1 || S.lanes8 > 64 || S.lanes16 < 1 || S.lanes16 > 64) return hipErrorInvalidValue;
   hipError_t e;
   fmls_reset_total<<<1, 64, 0, stream>>>(L.total);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -857,6 +999,13 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t 
                                                     (size_t)fmls_amp_ld(S.ndir), L.work.amps);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     fmls_sort_kernel<<<dim3((unsigned)nv), dim3(64), 0, stream>>>(S, L, v0, nv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // growth with 8-bit labels, then the voxels of more than 127 lobes again with 16-bit ones
+    fmls_grow_kernel<Lab8><<<dim3((unsigned)((nv + S.lanes8 - 1) / S.lanes8)), dim3(64),
+                             (size_t)S.ndir * S.lanes8, stream>>>(S, L, nv, S.lanes8, 0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    fmls_grow_kernel<Lab16><<<dim3((unsigned)((nv + S.lanes16 - 1) / S.lanes16)), dim3(64),
+                              (size_t)S.ndir * S.lanes16 * 2, stream>>>(S, L, nv, S.lanes16, 1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     switch (S.lmax) {
       case 2: e = launch_segment<2>(S, L, v0, nv, stream); break;

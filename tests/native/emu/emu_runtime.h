@@ -16,6 +16,7 @@
 #define __device__
 #define __host__
 #define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
 #define __launch_bounds__(...)
 #define __shared__ static
 
@@ -27,7 +28,11 @@ struct dim3 {
   unsigned x, y, z;
   dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
-enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount };
+enum hipDeviceAttribute_t {
+  hipDeviceAttributeMultiprocessorCount,
+  hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+  hipDeviceAttributeMaxSharedMemoryPerBlock
+};
 
 namespace emu {
 constexpr int W = 64;
@@ -131,5 +136,9 @@ inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline const char* hipGetErrorString(hipError_t) { return "emu error"; }
-inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 1; return hipSuccess; }
+// one CU; LDS as gfx950's 160 KB per CU and per workgroup
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+  *v = a == hipDeviceAttributeMultiprocessorCount ? 1 : 163840;
+  return hipSuccess;
+}
 inline hipError_t hipMemGetInfo(size_t* free_b, size_t* total_b) { *free_b = *total_b = size_t(1) << 36; return hipSuccess; }

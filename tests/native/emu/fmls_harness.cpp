@@ -11,6 +11,10 @@
 #include <vector>
 
 #include "../../../include/mrfixel.h"
+// the growth kernels' dynamic LDS: a buffer of exactly the launch's size (so that AddressSanitizer
+// checks the columns' bounds), shared by the lanes of the running workgroup
+static unsigned char* emu_grow_lds = nullptr;
+#define FMLS_GROW_LDS(name) unsigned char* name = emu_grow_lds
 #include KERNEL_SOURCE
 
 namespace mrcsd {
@@ -40,6 +44,11 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t)
         L.work.amps[v * lda + d] = acc;
       }
     run_grid(unsigned(nv), [&]() { fmls_sort_kernel(S, L, v0, nv); });
+    std::vector<unsigned char> lds8(size_t(S.ndir) * S.lanes8), lds16(size_t(S.ndir) * S.lanes16 * 2);
+    emu_grow_lds = lds8.data();
+    run_grid(unsigned((nv + S.lanes8 - 1) / S.lanes8), [&]() { fmls_grow_kernel<Lab8>(S, L, nv, S.lanes8, 0); });
+    emu_grow_lds = lds16.data();
+    run_grid(unsigned((nv + S.lanes16 - 1) / S.lanes16), [&]() { fmls_grow_kernel<Lab16>(S, L, nv, S.lanes16, 1); });
     switch (S.lmax) {
       case 2: run_segment<2>(S, L, v0, nv, nb); break;
       case 4: run_segment<4>(S, L, v0, nv, nb); break;

@@ -70,7 +70,7 @@ def fod_batch(seed=11, n=10):
         mask.append(1)
     const = np.zeros(45)
     const[0] = 0.7
-    fods.append(const); mask.append(1)                      # many lobes: workspace pass
+    fods.append(const); mask.append(1)                      # over 127 lobes: the 16-bit growth pass
     fods.append(stick_fod([[0, 0, 1]], [0.5])); mask.append(0)   # masked out
     neg = -stick_fod([[1, 0, 0]], [0.5]); neg[0] = 0.1
     fods.append(neg); mask.append(1)                        # largest |amplitude| negative

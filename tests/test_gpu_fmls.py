@@ -54,7 +54,7 @@ def synthetic(n, seed, lmax=8, noise=0.02):
 
 
 def edge_cases():
-    const = np.zeros(45); const[0] = 0.7                      # over 64 lobes: the workspace pass
+    const = np.zeros(45); const[0] = 0.7                      # over 127 lobes: the 16-bit growth pass
     neg = -stick_fod([[1, 0, 0]], [0.5]); neg[0] = 0.1        # largest |amplitude| negative
     z = stick_fod([[0, 1, 0]], [0.5]); z[0] = 0.0             # c0 = 0
     nan = stick_fod([[0, 1, 0]], [0.5]); nan[0] = np.nan      # c0 NaN
