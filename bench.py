@@ -776,11 +776,13 @@ def bench_fixel(args, torch, ph):
                                "directions, default thresholds; FODs from the plain GPU CSD, resident in HBM"},
         "roofline": {"bound": "hbm", "achieved": hbm / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hbm / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fmls_amp_kernel + fmls_segment_kernel<%d> + offset and emit kernels" % shared.lmax,
+                     "kernel": "fmls_amp_kernel + fmls_sort_kernel + fmls_grow_kernel + fmls_segment_kernel<%d> "
+                               "+ offset and emit kernels" % shared.lmax,
                      "kernel_ms": kern_ms, "algorithmic_bytes": hbm,
-                     "note": "one lane per voxel, latency-bound: the amplitudes and per-voxel work arrays "
-                             "(lane-interleaved in HBM) are extra traffic; the amplitude GEMM is 2 x 1281 x "
-                             "n_sh FP64 FLOPs per voxel"},
+                     "note": "sequential per voxel (lobe growth in visiting order): a wavefront per voxel "
+                             "sorts, a lane per voxel grows lobes (labels in LDS) and revises peaks; the "
+                             "amplitudes and per-voxel work arrays in HBM are extra traffic; the amplitude "
+                             "GEMM is 2 x 1281 x n_sh FP64 FLOPs per voxel"},
         "fixels": {"total": nfix, "per_voxel": nfix / ph.nvox},
     }
     if not args.no_cpu:
