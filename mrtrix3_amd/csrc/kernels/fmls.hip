@@ -13,8 +13,9 @@
 //     several either merges them (bridge amplitude over the peak of the highest-numbered adjacent
 //     lobe above the merge ratio) or waits, outside every lobe, for its assignment to the
 //     lowest-numbered of them after the loop.  Each lane's lobe labels are an 8-bit column in LDS
-//     (as many voxels per workgroup as let two share a CU: 63 in 160 KB); the sorted lists and direction tables are
-//     loaded steps ahead; lobe sums are kept in a register cache.  A voxel of more than 127 lobes
+//     (as many voxels per workgroup as let two share a CU: 63 in 160 KB); the sorted lists and
+//     direction tables are loaded a block of steps at a time; lobe sums are kept in a register
+//     cache.  A voxel of more than 127 lobes
 //     is left to fmls_grow_kernel<Lab16> (16-bit labels, 31 voxels per workgroup), which grows it
 //     again from the unchanged sorted lists.  The final labels go to the work arrays;
 //   * fmls_segment_kernel<LMAX>: per lobe, the integral threshold, the Newton peak search from each
@@ -480,34 +481,35 @@ __device__ __forceinline__ int grow_merge(const FmlsDeviceSet& S, const Tabs& T,
 // lobes, or -1 when the voxel needs more lobes than `lab` can number (nothing in ord, rev or nbr
 // is changed, so the voxel can be grown again with wider labels).
 //
-// Every global load of a step is issued ahead of it: direction, neighbour mask and amplitude
-// eight steps ahead (ring ra/rv), the direction's unit vector, weight and adjacency row four
-// steps ahead (ring rb); the step loop is unrolled by eight so that each ring slot is a fixed
-// register.  Labels live in LDS and lobe sums in the register cache, so a step usually issues no
-// global access of its own.
+// The global loads of a block of eight steps (visiting order, neighbour masks, amplitudes, then
+// the directions' unit vectors, weights and adjacency rows) are issued before its first step.
+// Labels live in LDS and lobe sums in the register cache, so a step usually issues no global
+// access of its own.
 template <class Lab>
 __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab& lab, int P) {
-  constexpr int KA = 8, KB = 4;
+  constexpr int KA = 8;
   for (int d = 0; d < S.ndir; ++d) lab.set(d, -1);
   int nl = 0;
   LobeCache lc;
   lc_init(lc);
-  int ra[KA];      // ord | nbr << 16
-  double rv[KA];   // amplitude
-  DirInfo rb[KB];
-  // Ring loads are unconditional, their indices clamped to the last entry (a load past the end is
-  // never used and stays in bounds), and issued after the step that last used the slot: each slot
-  // is then written by its load alone, with no copy that would wait for a load in flight.
-#pragma unroll
-  for (int k = 0; k < KA; ++k) {
-    const int ia = k < P ? k : P - 1;
-    ra[k] = (int)T.ord[ia] | ((int)T.nbr[ia] << 16);
-    rv[k] = T.rev[ia];
-  }
-#pragma unroll
-  for (int k = 0; k < KB; ++k) rb[k] = dir_info(S, ra[k] & 0xffff);
   const bool can_merge = S.merge_ratio < 1.0;   // (a direction's amplitude never exceeds a visited peak)
+  // Blocks of KA steps: the block's visiting order, masks and amplitudes are loaded first, then
+  // its directions' tables, all before its first step, and consumed in straight-line code -- no
+  // load is in flight across the loop's back edge (the memory counter is in order, so a load
+  // carried over would make the compiler wait for everything at the loop head).  Indices past the
+  // end are clamped to the last entry: in bounds, never used.
   for (int i0 = 0; i0 < P; i0 += KA) {
+    int ra[KA];      // ord | nbr << 16
+    double rv[KA];   // amplitude
+    DirInfo rb[KA];
+#pragma unroll
+    for (int j = 0; j < KA; ++j) {
+      const int ia = i0 + j < P ? i0 + j : P - 1;
+      ra[j] = (int)T.ord[ia] | ((int)T.nbr[ia] << 16);
+      rv[j] = T.rev[ia];
+    }
+#pragma unroll
+    for (int j = 0; j < KA; ++j) rb[j] = dir_info(S, ra[j] & 0xffff);
 #pragma unroll
     for (int j = 0; j < KA; ++j) {
       const int i = i0 + j;
@@ -515,7 +517,7 @@ __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab&
         const int d = ra[j] & 0xffff;
         const unsigned seen_before = (unsigned)ra[j] >> 16;
         const double v = rv[j];
-        const DirInfo& di = rb[j % KB];
+        const DirInfo& di = rb[j];
         // the lowest- and highest-numbered lobes among d's neighbours; only neighbours visited
         // before d can carry one (the others still hold -1), so only their labels are read, and
         // waiting directions (<= -2) belong to no lobe yet
@@ -548,11 +550,6 @@ __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab&
           lab.set(d, -2 - mn);
         }
       }
-      // refills: entry i + KA into slot j, the direction tables of step i + KB into slot j % KB
-      const int ia = i + KA < P ? i + KA : P - 1;
-      ra[j] = (int)T.ord[ia] | ((int)T.nbr[ia] << 16);
-      rv[j] = T.rev[ia];
-      rb[j % KB] = dir_info(S, ra[(j + KB) % KA] & 0xffff);
     }
   }
 
