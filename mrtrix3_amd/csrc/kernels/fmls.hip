@@ -751,7 +751,83 @@ __device__ __forceinline__ bool visited_before(double ka, int a, double kb, int 
   return (ka > kb) | ((ka == kb) & (a < b));   // no short circuit: compares and selects, no branches
 }
 
-constexpr int kSortPer = 32;   // entries per lane: 64 x 32 = 2048 >= kFmlsMaxDirs
+// Bitonic sort into visiting order of 64 x PER (amplitude, direction) entries, entry e = PER lane
+// + k in key[k], idx[k] of lane `lane`: compare-exchanges at distances below PER within the lane's
+// registers, at PER and above between lanes (partner lane ^ (j / PER)).
+template <int PER>
+__device__ __forceinline__ void fmls_bitonic(double (&key)[PER], int (&idx)[PER], int lane) {
+  for (int sz = 2; sz <= 64 * PER; sz <<= 1) {
+    for (int j = sz >> 1; j >= PER; j >>= 1) {
+      const int lj = j / PER;
+      const bool first = ((lane & lj) == 0) == (((lane * PER) & sz) == 0);   // keep the earlier entry
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const double ok2 = __shfl_xor(key[k], lj);
+        const int oi = __shfl_xor(idx[k], lj);
+        const bool take = visited_before(ok2, oi, key[k], idx[k]) == first;
+        key[k] = take ? ok2 : key[k];
+        idx[k] = take ? oi : idx[k];
+      }
+    }
+#pragma unroll
+    for (int j = PER / 2; j > 0; j >>= 1) {
+      if (j >= sz) continue;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        if (k & j) continue;
+        const int k2 = k | j;
+        const bool up = (((lane * PER) | k) & sz) == 0;
+        // ascending: k2's entry moves to k when it is visited first; descending: when it is not
+        // (equal entries are padding, where a swap changes nothing)
+        const bool sw = visited_before(key[k2], idx[k2], key[k], idx[k]) == up;
+        const double a0 = key[k], a1 = key[k2];
+        const int i0 = idx[k], i1 = idx[k2];
+        key[k] = sw ? a1 : a0;
+        key[k2] = sw ? a0 : a1;
+        idx[k] = sw ? i1 : i0;
+        idx[k2] = sw ? i0 : i1;
+      }
+    }
+  }
+}
+
+// The sorted entries [0, cnt) to ord and rev, their positions to s_pos (LDS, ndir entries), then
+// each one's mask of neighbours visited before it to nbr.
+template <int PER>
+__device__ __forceinline__ void fmls_sorted_out(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t slot, int lane,
+                                                int cnt, const double (&key)[PER], const int (&idx)[PER],
+                                                uint16_t* s_pos) {
+  for (int d = lane; d < S.ndir; d += 64) s_pos[d] = 0xffff;
+  __syncthreads();
+  const size_t cs = L.chunk;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = lane * PER + k;
+    if (e < cnt) {
+      s_pos[idx[k]] = (uint16_t)e;
+      L.work.ord[(size_t)e * cs + slot] = (uint16_t)idx[k];
+      L.work.rev[(size_t)e * cs + slot] = key[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = lane * PER + k;
+    if (e < cnt) {
+      const int d = idx[k];
+      unsigned m = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int j = S.adj[d * 8 + r];
+        m |= (j >= 0 && (int)s_pos[j] < e) ? 1u << r : 0u;
+      }
+      L.work.nbr[(size_t)e * cs + slot] = (uint8_t)m;
+    }
+  }
+}
+
+constexpr int kSortPer = 32;        // entries per lane of the full sort: 64 x 32 = 2048 >= kFmlsMaxDirs
+constexpr int kSortSmallPer = 16;   // of the sort of at most 1024 positive directions (nearly every voxel)
 
 // Workgroup b of a grid of n, placed so that each of the 8 XCDs (which take workgroups round-robin)
 // runs one contiguous range of them: neighbouring voxels then write the same lines of the
@@ -763,14 +839,16 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned n) {
 
 // One wavefront per voxel of the chunk: the check that the largest |amplitude| (the lowest index
 // among equal ones) is positive (fmls.cpp:171-184), then the positive directions in visiting order
-// by a bitonic sort of (amplitude, direction) entries held in registers -- entry e = 32 lane + k,
-// compare-exchanges at distances 1..16 within a lane and at 32..1024 between lanes -- padded to
-// 2048 with entries visited after every positive one.  Writes npos (0: nothing to segment), ord,
+// by a bitonic sort of (amplitude, direction) entries held in registers, padded with entries
+// visited after every positive one: with at most 1024 positive directions (nearly every voxel)
+// they are first packed through LDS and sorted as 64 x 16 entries, otherwise as 64 x 32.  Writes npos (0: nothing to segment), ord,
 // rev (the amplitudes) and nbr (which neighbours of each direction are visited before it, from
 // the directions' positions in an LDS table) for the segmentation kernel.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fmls_sort_kernel(
     FmlsDeviceSet S, FmlsLaunch L, size_t v0, size_t nv) {
-  __shared__ uint16_t s_pos[kFmlsMaxDirs];   // visiting position of each direction, 0xffff: not visited
+  // the small sort's staging area (keys, then directions), then the visiting position of each
+  // direction (0xffff: not visited)
+  __shared__ double s_buf[64 * kSortSmallPer * 10 / 8];
   const int ndir = S.ndir;
   const size_t slot = xcd_contiguous(blockIdx.x, gridDim.x);
   if (slot >= nv) return;
@@ -815,67 +893,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fm
     if (lane == 0) L.work.npos[slot] = 0;
     return;
   }
-  for (int sz = 2; sz <= 64 * kSortPer; sz <<= 1) {
-    // distances of 32 and more: the partner entry is register k of lane ^ (j / 32)
-    for (int j = sz >> 1; j >= kSortPer; j >>= 1) {
-      const int lj = j / kSortPer;
-      const bool first = ((lane & lj) == 0) == (((lane * kSortPer) & sz) == 0);   // keep the earlier entry
+  uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_buf);
+  if (cnt <= 64 * kSortSmallPer) {
+    // the positive entries packed into staging slots [0, cnt) (in direction order), then sorted
+    // as 64 x 16 entries
+    double* s_key = s_buf;
+    uint16_t* s_idx = reinterpret_cast<uint16_t*>(s_buf + 64 * kSortSmallPer);
+    int base = 0;
 #pragma unroll
-      for (int k = 0; k < kSortPer; ++k) {
-        const double ok2 = __shfl_xor(key[k], lj);
-        const int oi = __shfl_xor(idx[k], lj);
-        const bool take = visited_before(ok2, oi, key[k], idx[k]) == first;
-        key[k] = take ? ok2 : key[k];
-        idx[k] = take ? oi : idx[k];
+    for (int k = 0; k < kSortPer; ++k) {
+      const bool pos = idx[k] != 0xffff;
+      const unsigned long long m = __ballot(pos);
+      if (pos) {
+        const int o = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+        s_key[o] = key[k];
+        s_idx[o] = (uint16_t)idx[k];
       }
+      base += (int)__popcll(m);
     }
-    // distances 16..1 within the lane's registers
+    __syncthreads();
+    double ks[kSortSmallPer];
+    int is[kSortSmallPer];
 #pragma unroll
-    for (int j = kSortPer / 2; j > 0; j >>= 1) {
-      if (j >= sz) continue;
-#pragma unroll
-      for (int k = 0; k < kSortPer; ++k) {
-        if (k & j) continue;
-        const int k2 = k | j;
-        const bool up = (((lane * kSortPer) | k) & sz) == 0;
-        // ascending: k2's entry moves to k when it is visited first; descending: when it is not
-        // (equal entries are padding, where a swap changes nothing)
-        const bool sw = visited_before(key[k2], idx[k2], key[k], idx[k]) == up;
-        const double a0 = key[k], a1 = key[k2];
-        const int i0 = idx[k], i1 = idx[k2];
-        key[k] = sw ? a1 : a0;
-        key[k2] = sw ? a0 : a1;
-        idx[k] = sw ? i1 : i0;
-        idx[k2] = sw ? i0 : i1;
-      }
+    for (int k = 0; k < kSortSmallPer; ++k) {
+      const int e = lane * kSortSmallPer + k;
+      ks[k] = e < cnt ? s_key[e] : 0.0;
+      is[k] = e < cnt ? (int)s_idx[e] : 0xffff;
     }
-  }
-  for (int d = lane; d < ndir; d += 64) s_pos[d] = 0xffff;
-  __syncthreads();
-  const size_t cs = L.chunk;
-#pragma unroll
-  for (int k = 0; k < kSortPer; ++k) {
-    const int e = lane * kSortPer + k;
-    if (e < cnt) {
-      s_pos[idx[k]] = (uint16_t)e;
-      L.work.ord[(size_t)e * cs + slot] = (uint16_t)idx[k];
-      L.work.rev[(size_t)e * cs + slot] = key[k];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kSortPer; ++k) {
-    const int e = lane * kSortPer + k;
-    if (e < cnt) {
-      const int d = idx[k];
-      unsigned m = 0;
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int j = S.adj[d * 8 + r];
-        m |= (j >= 0 && (int)s_pos[j] < e) ? 1u << r : 0u;
-      }
-      L.work.nbr[(size_t)e * cs + slot] = (uint8_t)m;
-    }
+    __syncthreads();   // the staging area becomes s_pos
+    fmls_bitonic<kSortSmallPer>(ks, is, lane);
+    fmls_sorted_out<kSortSmallPer>(S, L, slot, lane, cnt, ks, is, s_pos);
+  } else {
+    fmls_bitonic<kSortPer>(key, idx, lane);
+    fmls_sorted_out<kSortPer>(S, L, slot, lane, cnt, key, idx, s_pos);
   }
   if (lane == 0) L.work.npos[slot] = cnt;
 }
