@@ -38,6 +38,12 @@
 
 #include "fmls.h"
 
+// LDS of the growth kernels, sized at launch: `lanes` columns of ndir labels
+#ifndef FMLS_GROW_LDS_ARRAY
+extern __shared__ unsigned char fmls_grow_lds[];
+#define FMLS_GROW_LDS_ARRAY fmls_grow_lds
+#endif
+
 namespace mrcsd {
 namespace {
 
@@ -260,8 +266,9 @@ __device__ __forceinline__ DirInfo dir_info(const FmlsDeviceSet& S, int d) {
 }
 
 // Write-back cache of lobe records in registers: the running direction sum and integral of up to
-// four lobes (entry l & 3 holds lobe l), written to the work arrays when the entry is taken by
-// another lobe or flushed.  tag -1: empty.
+// four lobes, entry 0 the lobe last touched (a voxel mostly grows the lobe it grew last, which then
+// costs no select or memory access), entries 1..3 the ones touched before it, most recent first.
+// Entries leave through entry 3, written back to the work arrays.  tag -1: empty.
 struct LobeCache {
   int tag[4];
   double m0[4], m1[4], m2[4], integ[4];
@@ -272,75 +279,75 @@ __device__ __forceinline__ void lc_init(LobeCache& c) {
   for (int q = 0; q < 4; ++q) c.tag[q] = -1;
 }
 
+__device__ __forceinline__ void lc_store(const Tabs& T, int l, double m0, double m1, double m2, double integ) {
+  T.mean[3 * l] = m0;
+  T.mean[3 * l + 1] = m1;
+  T.mean[3 * l + 2] = m2;
+  T.integ[l] = integ;
+}
+
 // every entry back to the work arrays, the cache emptied
 __device__ void lc_flush(const Tabs& T, LobeCache& c) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int l = c.tag[q];
-    if (l >= 0) {
-      T.mean[3 * l] = c.m0[q];
-      T.mean[3 * l + 1] = c.m1[q];
-      T.mean[3 * l + 2] = c.m2[q];
-      T.integ[l] = c.integ[q];
-    }
+    if (c.tag[q] >= 0) lc_store(T, c.tag[q], c.m0[q], c.m1[q], c.m2[q], c.integ[q]);
     c.tag[q] = -1;
   }
 }
 
-// entry l & 3 given to lobe l: its previous lobe written back, l's record read unless `fresh`
-// (a new lobe whose values the caller sets); the entry's values returned in m0..integ
-__device__ __forceinline__ void lc_take(const Tabs& T, LobeCache& c, int l, bool fresh, double& m0, double& m1,
-                                        double& m2, double& integ) {
-  const int q = l & 3;
-  int tg = -1;
+// lobe l to entry 0: found among entries 1..3, or read from the work arrays (unless `fresh`, a
+// new lobe whose values the caller sets) after entry 3 is written back; the entries before it
+// move down one
+__device__ __forceinline__ void lc_take(const Tabs& T, LobeCache& c, int l, bool fresh) {
+  if (c.tag[0] == l) return;
+  int q = 3;   // the entry l comes from (3: none, evicted)
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 3; k >= 1; --k)
+    if (c.tag[k] == l) q = k;
+  double n0 = 0.0, n1 = 0.0, n2 = 0.0, ni = 0.0;
+  int tq = -1;
+#pragma unroll
+  for (int k = 1; k < 4; ++k)
     if (k == q) {
-      tg = c.tag[k];
-      m0 = c.m0[k];
-      m1 = c.m1[k];
-      m2 = c.m2[k];
-      integ = c.integ[k];
+      tq = c.tag[k];
+      n0 = c.m0[k];
+      n1 = c.m1[k];
+      n2 = c.m2[k];
+      ni = c.integ[k];
     }
-  if (tg != l) {
-    if (tg >= 0) {
-      T.mean[3 * tg] = m0;
-      T.mean[3 * tg + 1] = m1;
-      T.mean[3 * tg + 2] = m2;
-      T.integ[tg] = integ;
-    }
+  if (tq != l) {
+    if (c.tag[3] >= 0) lc_store(T, c.tag[3], c.m0[3], c.m1[3], c.m2[3], c.integ[3]);
     if (!fresh) {
-      m0 = T.mean[3 * l];
-      m1 = T.mean[3 * l + 1];
-      m2 = T.mean[3 * l + 2];
-      integ = T.integ[l];
+      n0 = T.mean[3 * l];
+      n1 = T.mean[3 * l + 1];
+      n2 = T.mean[3 * l + 2];
+      ni = T.integ[l];
     }
   }
-}
-
-__device__ __forceinline__ void lc_put(LobeCache& c, int l, double m0, double m1, double m2, double integ) {
-  const int q = l & 3;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (k == q) {
-      c.tag[k] = l;
-      c.m0[k] = m0;
-      c.m1[k] = m1;
-      c.m2[k] = m2;
-      c.integ[k] = integ;
+  for (int k = 3; k >= 1; --k)
+    if (k <= q) {
+      c.tag[k] = c.tag[k - 1];
+      c.m0[k] = c.m0[k - 1];
+      c.m1[k] = c.m1[k - 1];
+      c.m2[k] = c.m2[k - 1];
+      c.integ[k] = c.integ[k - 1];
     }
+  c.tag[0] = l;
+  c.m0[0] = n0;
+  c.m1[0] = n1;
+  c.m2[0] = n2;
+  c.integ[0] = ni;
 }
 
 // a new lobe seeded by direction d of amplitude v (fmls.cpp:199-201, FOD_lobe's constructor
 // fmls.h:64-76)
 __device__ void lobe_new(const Tabs& T, int l, int d, double v, const DirInfo& di, LobeCache& c) {
-  double m0, m1, m2, integ;
-  lc_take(T, c, l, true, m0, m1, m2, integ);
-  m0 = di.u0 * fabs(v) * di.w;
-  m1 = di.u1 * fabs(v) * di.w;
-  m2 = di.u2 * fabs(v) * di.w;
-  integ = fabs(v * di.w);
-  lc_put(c, l, m0, m1, m2, integ);
+  lc_take(T, c, l, true);
+  c.m0[0] = di.u0 * fabs(v) * di.w;
+  c.m1[0] = di.u1 * fabs(v) * di.w;
+  c.m2[0] = di.u2 * fabs(v) * di.w;
+  c.integ[0] = fabs(v * di.w);
   T.peak[l] = fabs(v);
   T.head[l] = (int16_t)d;
   T.tail[l] = (int16_t)d;
@@ -350,15 +357,14 @@ __device__ void lobe_new(const Tabs& T, int l, int d, double v, const DirInfo& d
 
 // FOD_lobe::add (fmls.h:88-97), through the cache
 __device__ void lobe_add(const Tabs& T, int l, double v, const DirInfo& di, LobeCache& c) {
-  double m0, m1, m2, integ;
-  lc_take(T, c, l, false, m0, m1, m2, integ);
+  lc_take(T, c, l, false);
+  const double m0 = c.m0[0], m1 = c.m1[0], m2 = c.m2[0];
   const double mult = (m0 * di.u0 + m1 * di.u1 + m2 * di.u2) > 0.0 ? 1.0 : -1.0;
   const double av = fabs(v);
-  m0 = m0 + di.u0 * mult * av * di.w;
-  m1 = m1 + di.u1 * mult * av * di.w;
-  m2 = m2 + di.u2 * mult * av * di.w;
-  integ = integ + fabs(v * di.w);
-  lc_put(c, l, m0, m1, m2, integ);
+  c.m0[0] = m0 + di.u0 * mult * av * di.w;
+  c.m1[0] = m1 + di.u1 * mult * av * di.w;
+  c.m2[0] = m2 + di.u2 * mult * av * di.w;
+  c.integ[0] = c.integ[0] + fabs(v * di.w);
 }
 
 // FOD_lobe::merge (fmls.h:112-127): B into A
@@ -393,16 +399,19 @@ __device__ void lobe_copy(const Tabs& T, int dst, int src) {
   T.np[dst] = T.np[src];
 }
 
-// A lane's column of lobe labels in LDS: entry d at p[d * s].  -1: no lobe (not visited, or
-// negative); l >= 0: lobe l; -2 - l: waiting for lobe l.  CAP: the lobes the type can number.
+// A lane's column of lobe labels in the growth kernels' LDS: entry d at element o + d * s (32-bit
+// indices, so addresses stay LDS offsets).  -1: no lobe (not visited, or negative); l >= 0: lobe
+// l; -2 - l: waiting for lobe l.  CAP: the lobes the type can number.
 template <class V, int CAP>
 struct LdsLab {
-  using value_type = V;
-  V* p;
-  int s;
+  uint32_t o, s;
   static constexpr int cap = CAP;
-  __device__ __forceinline__ int get(int d) const { return p[d * s]; }
-  __device__ __forceinline__ void set(int d, int v) const { p[d * s] = (V)v; }
+  __device__ __forceinline__ int get(int d) const {
+    return reinterpret_cast<V*>(FMLS_GROW_LDS_ARRAY)[o + (uint32_t)d * s];
+  }
+  __device__ __forceinline__ void set(int d, int v) const {
+    reinterpret_cast<V*>(FMLS_GROW_LDS_ARRAY)[o + (uint32_t)d * s] = (V)v;
+  }
 };
 using Lab8 = LdsLab<int8_t, 127>;        // waiting labels down to -128: lobes 0..126
 using Lab16 = LdsLab<int16_t, 32767>;    // any direction count
@@ -521,17 +530,21 @@ __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab&
         // the lowest- and highest-numbered lobes among d's neighbours; only neighbours visited
         // before d can carry one (the others still hold -1), so only their labels are read, and
         // waiting directions (<= -2) belong to no lobe yet
+        // (every label is read -- padding entries read entry 0 -- and the mask picks; the
+        // highest label by signed max, where -1 and waiting labels stay below every lobe; the
+        // lowest by unsigned min, where they stay above)
         int lbs[8];
-        int mn = INT_MAX, mx = -1;
+        int mx = -1;
+        unsigned umn = 0xffffffffu;
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
           const int jr = (int16_t)((di.adj.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
-          lbs[r] = (seen_before >> r) & 1u ? lab.get(jr) : -1;
-          if (lbs[r] >= 0) {
-            mn = min(mn, lbs[r]);
-            mx = max(mx, lbs[r]);
-          }
+          const int raw = lab.get(jr < 0 ? 0 : jr);
+          lbs[r] = (seen_before >> r) & 1u ? raw : -1;
+          mx = max(mx, lbs[r]);
+          umn = min(umn, (unsigned)lbs[r]);
         }
+        const int mn = (int)umn;
         if (mx < 0) {
           if (nl == Lab::cap) return -1;
           lobe_new(T, nl, d, v, di, lc);
@@ -930,11 +943,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fm
   if (lane == 0) L.work.npos[slot] = cnt;
 }
 
-// LDS of the growth kernels, sized at launch: `lanes` columns of ndir labels
-#ifndef FMLS_GROW_LDS
-#define FMLS_GROW_LDS(name) extern __shared__ unsigned char name[]
-#endif
-
 // Lobe growth for the voxels of the chunk, one lane per voxel, `lanes` voxels per workgroup, each
 // lane's labels in its own LDS column (so the neighbour-label reads of the growth never leave the
 // CU).  The final labels go to the work arrays for the per-lobe kernel; nlobes[slot] = the lobe
@@ -942,7 +950,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fm
 // -1 by an earlier pass are grown.
 template <class Lab>
 __global__ __launch_bounds__(64) void fmls_grow_kernel(FmlsDeviceSet S, FmlsLaunch L, size_t nv, int lanes, int redo) {
-  FMLS_GROW_LDS(s_lab);
   const int lane = threadIdx.x;
   const size_t slot = (size_t)blockIdx.x * lanes + lane;
   if (lane >= lanes || slot >= nv) return;
@@ -951,7 +958,7 @@ __global__ __launch_bounds__(64) void fmls_grow_kernel(FmlsDeviceSet S, FmlsLaun
   int nl = 0;
   if (P > 0) {
     const Tabs T = tabs_of(L.work, slot, L.chunk);
-    const Lab lab{reinterpret_cast<typename Lab::value_type*>(s_lab) + lane, lanes};
+    const Lab lab{(uint32_t)lane, (uint32_t)lanes};
     nl = fmls_grow_voxel(S, T, lab, P);
     if (nl >= 0)
       for (int d = 0; d < S.ndir; ++d) T.lab[d] = (int16_t)lab.get(d);

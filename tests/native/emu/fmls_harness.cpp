@@ -14,7 +14,7 @@
 // the growth kernels' dynamic LDS: a buffer of exactly the launch's size (so that AddressSanitizer
 // checks the columns' bounds), shared by the lanes of the running workgroup
 static unsigned char* emu_grow_lds = nullptr;
-#define FMLS_GROW_LDS(name) unsigned char* name = emu_grow_lds
+#define FMLS_GROW_LDS_ARRAY emu_grow_lds
 #include KERNEL_SOURCE
 
 namespace mrcsd {
