@@ -55,10 +55,10 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 size_t fmls_work_bytes(int ndir, size_t chunk) {
   const size_t n = size_t(ndir) * chunk;
   // doubles: amps (fmls_amp_ld(ndir) per slot), peak, integ (n each), mean, rev (3 n each); int16:
-  // lab, ord, nxt, head, tail, np; uint8: nbr; int32: npos, nlobes (one per slot) -- the pieces
-  // fmls_work_carve takes, in its order
+  // lab, ord, nxt, head, tail, np; uint8: nbr; int32: npos, nlobes (one per slot); uint32: items
+  // (n); the item count -- the pieces fmls_work_carve takes, in its order
   return align256(8 * size_t(fmls_amp_ld(ndir)) * chunk) + 2 * align256(8 * n) + 2 * align256(8 * 3 * n) +
-         6 * align256(2 * n) + align256(n) + 2 * align256(4 * chunk);
+         6 * align256(2 * n) + align256(n) + 2 * align256(4 * chunk) + align256(4 * n) + align256(8);
 }
 
 FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
@@ -84,6 +84,8 @@ FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
   W.nbr = reinterpret_cast<uint8_t*>(take(n));
   W.npos = reinterpret_cast<int32_t*>(take(4 * chunk));
   W.nlobes = reinterpret_cast<int32_t*>(take(4 * chunk));
+  W.items = reinterpret_cast<uint32_t*>(take(4 * n));
+  W.nitems = reinterpret_cast<unsigned long long*>(take(8));
   return W;
 }
 

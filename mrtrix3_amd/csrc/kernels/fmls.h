@@ -65,6 +65,8 @@ struct FmlsWork {
                              // during the revision
   int32_t* npos;             // [1] positive directions to visit (0: the voxel has no positive lobe)
   int32_t* nlobes;           // [1] lobes of the voxel, kept and dropped
+  uint32_t* items;           // [ndir x chunk] the chunk's lobes as slot << 11 | lobe, voxel by voxel
+  unsigned long long* nitems;   // [1] items of the chunk
 };
 
 struct FmlsLaunch {

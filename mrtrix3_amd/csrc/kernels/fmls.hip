@@ -18,11 +18,14 @@
 //     cache.  A voxel of more than 127 lobes
 //     is left to fmls_grow_kernel<Lab16> (16-bit labels, 31 voxels per workgroup), which grows it
 //     again from the unchanged sorted lists.  The final labels go to the work arrays;
-//   * fmls_segment_kernel<LMAX>: per lobe, the integral threshold, the Newton peak search from each
-//     of its peaks (Math::SH::get_peak in double with the precomputed Legendre table,
-//     SH.h:398-559), a revision kept only when it stays nearest to that peak and inside the lobe
-//     (nearest direction through the FastLookupSet grid), the peak threshold, the fixel direction;
-//     then the voxel's fixel count and summary; the kept lobes stay in the work arrays;
+//   * fmls_block_sums, fmls_scan_sums, fmls_lobe_items: the chunk's lobes listed as work items;
+//   * fmls_revise_kernel<LMAX>: one lane per lobe, the lanes striding over the items: the
+//     integral threshold, the Newton peak search from each of its peaks (Math::SH::get_peak in
+//     double with the precomputed Legendre table, SH.h:398-559), a revision kept only when it stays
+//     nearest to that peak and inside the lobe (nearest direction through the FastLookupSet grid),
+//     the peak threshold, the fixel direction;
+//   * fmls_finalize_kernel: each voxel's fixel count and summary; the kept lobes stay in the work
+//     arrays;
 //   * fmls_block_sums, fmls_scan_sums: the fixel offset of each block of 64 voxels, continuing
 //     the running total of the earlier chunks;
 //   * fmls_emit: each voxel's offset in voxel order (the order fod2fixel's index image uses,
@@ -581,82 +584,110 @@ __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab&
   return nl;
 }
 
-// The per-lobe steps of one voxel's segmentation (fmls.cpp:254-292) over its nl grown lobes, the
-// final labels in the work arrays; the lobe records then hold the kept fixels (np < 0: dropped by a
-// threshold).
+// The per-lobe steps of the segmentation (fmls.cpp:254-292) for lobe l of a voxel, the final labels
+// in the work arrays: the integral threshold, the Newton search from each of its peaks in the
+// reference's order, the nearest-peak and inside-the-lobe checks, the peak threshold, the fixel
+// direction; np = -1 when a threshold drops it.  Peak p, the p-th seed s_p on the lobe's list,
+// keeps its direction at rev[3 s_p]: seeds are distinct directions, so the lobes of a voxel use
+// disjoint entries and can be revised independently.
 template <int LMAX>
-__device__ void fmls_revise_voxel(const FmlsDeviceSet& S, const Tabs& T, int nl, const float* f, double* AL) {
-  // ---- per lobe: integral threshold, peak revision, peak threshold (fmls.cpp:254-292) ----
-  int run = 0;
-  for (int l = 0; l < nl; ++l) {
-    const int np = T.np[l];
-    const uint32_t r0 = 3 * (uint32_t)run;
-    int s = T.head[l];
-    for (int p = 0; p < np && s >= 0; ++p) {
-      T.rev[r0 + 3 * p] = S.unit[3 * s];
-      T.rev[r0 + 3 * p + 1] = S.unit[3 * s + 1];
-      T.rev[r0 + 3 * p + 2] = S.unit[3 * s + 2];
-      s = T.nxt[s];
-    }
-    bool keep = !(T.integ[l] < S.integral_threshold);
-    double maxp = T.peak[l];
-    if (keep) {
-      for (int p = 0; p < np; ++p) {
-        double nd[3] = {T.rev[r0 + 3 * p], T.rev[r0 + 3 * p + 1], T.rev[r0 + 3 * p + 2]};
-        const double nvl = fmls_get_peak<LMAX>(f, nd, S, AL);
-        if (isfinite(nvl) && isfinite(nd[0]) && isfinite(nd[1]) && isfinite(nd[2])) {
-          double max_dp = 0.0;
-          int nearest = np;
-          for (int q = 0; q < np; ++q) {
-            const double dp =
-                fabs(nd[0] * T.rev[r0 + 3 * q] + nd[1] * T.rev[r0 + 3 * q + 1] + nd[2] * T.rev[r0 + 3 * q + 2]);
-            if (dp > max_dp) {
-              max_dp = dp;
-              nearest = q;
-            }
-          }
-          if (nearest == p && T.lab[fmls_select(S, nd)] == l) {
-            T.rev[r0 + 3 * p] = nd[0];
-            T.rev[r0 + 3 * p + 1] = nd[1];
-            T.rev[r0 + 3 * p + 2] = nd[2];
-            if (p == 0) maxp = nvl;
+__device__ void fmls_revise_lobe(const FmlsDeviceSet& S, const Tabs& T, int l, const float* f, double* AL) {
+  const int np = T.np[l];
+  const int head = T.head[l];
+  for (int p = 0, s = head; p < np && s >= 0; ++p, s = T.nxt[s]) {
+    T.rev[3 * s] = S.unit[3 * s];
+    T.rev[3 * s + 1] = S.unit[3 * s + 1];
+    T.rev[3 * s + 2] = S.unit[3 * s + 2];
+  }
+  bool keep = !(T.integ[l] < S.integral_threshold);
+  double maxp = T.peak[l];
+  if (keep) {
+    for (int p = 0, sp = head; p < np && sp >= 0; ++p, sp = T.nxt[sp]) {
+      double nd[3] = {T.rev[3 * sp], T.rev[3 * sp + 1], T.rev[3 * sp + 2]};
+      const double nvl = fmls_get_peak<LMAX>(f, nd, S, AL);
+      if (isfinite(nvl) && isfinite(nd[0]) && isfinite(nd[1]) && isfinite(nd[2])) {
+        double max_dp = 0.0;
+        int nearest = np;
+        for (int q = 0, sq = head; q < np && sq >= 0; ++q, sq = T.nxt[sq]) {
+          const double dp = fabs(nd[0] * T.rev[3 * sq] + nd[1] * T.rev[3 * sq + 1] + nd[2] * T.rev[3 * sq + 2]);
+          if (dp > max_dp) {
+            max_dp = dp;
+            nearest = q;
           }
         }
+        if (nearest == p && T.lab[fmls_select(S, nd)] == l) {
+          T.rev[3 * sp] = nd[0];
+          T.rev[3 * sp + 1] = nd[1];
+          T.rev[3 * sp + 2] = nd[2];
+          if (p == 0) maxp = nvl;
+        }
       }
-      keep = !(maxp < S.peak_threshold);
     }
-    if (keep) {
-      if (S.dir_from_peak) {
-        T.mean[3 * l] = T.rev[r0];
-        T.mean[3 * l + 1] = T.rev[r0 + 1];
-        T.mean[3 * l + 2] = T.rev[r0 + 2];
-      } else {
-        const double m0 = T.mean[3 * l], m1 = T.mean[3 * l + 1], m2 = T.mean[3 * l + 2];
-        const double nrm = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
-        T.mean[3 * l] = m0 / nrm;
-        T.mean[3 * l + 1] = m1 / nrm;
-        T.mean[3 * l + 2] = m2 / nrm;
-      }
-      T.peak[l] = maxp;
+    keep = !(maxp < S.peak_threshold);
+  }
+  if (keep) {
+    if (S.dir_from_peak) {
+      T.mean[3 * l] = T.rev[3 * head];
+      T.mean[3 * l + 1] = T.rev[3 * head + 1];
+      T.mean[3 * l + 2] = T.rev[3 * head + 2];
     } else {
-      T.np[l] = -1;
+      const double m0 = T.mean[3 * l], m1 = T.mean[3 * l + 1], m2 = T.mean[3 * l + 2];
+      const double nrm = sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+      T.mean[3 * l] = m0 / nrm;
+      T.mean[3 * l + 1] = m1 / nrm;
+      T.mean[3 * l + 2] = m2 / nrm;
     }
-    run += np;
+    T.peak[l] = maxp;
+  } else {
+    T.np[l] = -1;
   }
 }
 
+// The chunk's lobes as work items, voxel by voxel: items[o + l] = slot << 11 | l for the voxel's
+// lobes l, o its first item (the block offsets of its lobe counts in sums, then a workgroup scan).
+__global__ __launch_bounds__(kFmlsBlock) void fmls_lobe_items(FmlsLaunch L, size_t nv) {
+  __shared__ unsigned long long s[kFmlsBlock];
+  const size_t slot = (size_t)blockIdx.x * kFmlsBlock + threadIdx.x;
+  const unsigned long long c = slot < nv ? (unsigned long long)L.work.nlobes[slot] : 0ull;
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 1; o < kFmlsBlock; o <<= 1) {
+    const unsigned long long x = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0ull;
+    __syncthreads();
+    s[threadIdx.x] += x;
+    __syncthreads();
+  }
+  if (slot >= nv) return;
+  const unsigned long long off = L.sums[blockIdx.x] + s[threadIdx.x] - c;
+  for (unsigned long long l = 0; l < c; ++l) L.work.items[off + l] = (uint32_t)((slot << 11) | l);
+}
+
+// Lobe revision over the chunk's lobe items, one lane per lobe (so a voxel of many lobes spreads
+// over many lanes instead of holding back its wavefront), the lanes of the grid striding over the
+// items.
 template <int LMAX>
-__global__ __launch_bounds__(kFmlsBlock) void fmls_segment_kernel(FmlsDeviceSet S, FmlsLaunch L, size_t v0, size_t nv) {
+__global__ __launch_bounds__(kFmlsBlock) void fmls_revise_kernel(FmlsDeviceSet S, FmlsLaunch L, size_t v0) {
   // per-lane Legendre values of the Newton search: lane t's column at s_al[t + kFmlsBlock k]
   __shared__ double s_al[(LMAX / 2 + 1) * (LMAX / 2 + 1) * kFmlsBlock];
+  const unsigned long long n = *L.work.nitems;
+  const size_t stride = (size_t)gridDim.x * kFmlsBlock;
+  for (size_t t = (size_t)blockIdx.x * kFmlsBlock + threadIdx.x; t < n; t += stride) {
+    const uint32_t it = L.work.items[t];
+    const size_t slot = it >> 11;
+    const Tabs T = tabs_of(L.work, slot, L.chunk);
+    fmls_revise_lobe<LMAX>(S, T, (int)(it & 0x7ffu), L.fod + (v0 + slot) * (size_t)L.ld_fod, s_al + threadIdx.x);
+  }
+}
+
+// Each voxel's kept lobes in order, at most maxnum of them (fod2fixel.cpp:139-150): its fixel
+// count and summary.
+__global__ __launch_bounds__(kFmlsBlock) void fmls_finalize_kernel(FmlsDeviceSet S, FmlsLaunch L, size_t v0,
+                                                                    size_t nv) {
   const size_t slot = (size_t)blockIdx.x * kFmlsBlock + threadIdx.x;
   if (slot >= nv) return;
   const size_t vox = v0 + slot;
   const Tabs T = tabs_of(L.work, slot, L.chunk);
-  const float* f = L.fod + vox * (size_t)L.ld_fod;
   const int nl = L.work.nlobes[slot];
-  if (nl > 0) fmls_revise_voxel<LMAX>(S, T, nl, f, s_al + threadIdx.x);
-  // the kept lobes in order, at most maxnum of them (fod2fixel.cpp:139-150)
   int nk = 0;
   double sp0 = 0.0, sp1 = 0.0, sx = NAN, sy = NAN, sz = NAN;
   for (int l = 0; l < nl; ++l) {
@@ -1028,10 +1059,15 @@ __global__ void fmls_reset_total(unsigned long long* total) {
   if (threadIdx.x == 0) *total = 0ull;
 }
 
+// the lobe revision's grid: workgroups striding over the items, a few per SIMD of the largest
+// MI355X part, fewer when the pass is small
+constexpr unsigned kReviseGrid = 8192;
+
 template <int LMAX>
-hipError_t launch_segment(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0, size_t nv, hipStream_t stream) {
-  const unsigned nb = (unsigned)((nv + kFmlsBlock - 1) / kFmlsBlock);
-  fmls_segment_kernel<LMAX><<<dim3(nb), dim3(kFmlsBlock), 0, stream>>>(S, L, v0, nv);
+hipError_t launch_revise(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0, size_t nv, hipStream_t stream) {
+  const size_t nb = (nv * 8 + kFmlsBlock - 1) / kFmlsBlock;   // about 8 lobes per voxel and lane
+  const unsigned grid = (unsigned)(nb < kReviseGrid ? (nb ? nb : 1) : kReviseGrid);
+  fmls_revise_kernel<LMAX><<<dim3(grid), dim3(kFmlsBlock), 0, stream>>>(S, L, v0);
   return hipGetLastError();
 }
 
@@ -1061,16 +1097,28 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t 
     fmls_grow_kernel<Lab16><<<dim3((unsigned)((nv + S.lanes16 - 1) / S.lanes16)), dim3(64),
                               (size_t)S.ndir * S.lanes16 * 2, stream>>>(S, L, nv, S.lanes16, 1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the lobes as items: block sums of the lobe counts, their scan, the items
+    fmls_block_sums<<<dim3(nb), dim3(kFmlsBlock), 0, stream>>>(reinterpret_cast<const uint32_t*>(L.work.nlobes), 0,
+                                                               nv, L.sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    fmls_reset_total<<<1, 64, 0, stream>>>(L.work.nitems);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    fmls_scan_sums<<<1, dim3(kFmlsBlock), 0, stream>>>(L.sums, nb, L.work.nitems);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    fmls_lobe_items<<<dim3(nb), dim3(kFmlsBlock), 0, stream>>>(L, nv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     switch (S.lmax) {
-      case 2: e = launch_segment<2>(S, L, v0, nv, stream); break;
-      case 4: e = launch_segment<4>(S, L, v0, nv, stream); break;
-      case 6: e = launch_segment<6>(S, L, v0, nv, stream); break;
-      case 8: e = launch_segment<8>(S, L, v0, nv, stream); break;
-      case 10: e = launch_segment<10>(S, L, v0, nv, stream); break;
-      case 12: e = launch_segment<12>(S, L, v0, nv, stream); break;
+      case 2: e = launch_revise<2>(S, L, v0, nv, stream); break;
+      case 4: e = launch_revise<4>(S, L, v0, nv, stream); break;
+      case 6: e = launch_revise<6>(S, L, v0, nv, stream); break;
+      case 8: e = launch_revise<8>(S, L, v0, nv, stream); break;
+      case 10: e = launch_revise<10>(S, L, v0, nv, stream); break;
+      case 12: e = launch_revise<12>(S, L, v0, nv, stream); break;
       default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
+    fmls_finalize_kernel<<<dim3(nb), dim3(kFmlsBlock), 0, stream>>>(S, L, v0, nv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     fmls_block_sums<<<dim3(nb), dim3(kFmlsBlock), 0, stream>>>(L.count, v0, nv, L.sums);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     fmls_scan_sums<<<1, dim3(kFmlsBlock), 0, stream>>>(L.sums, nb, L.total);

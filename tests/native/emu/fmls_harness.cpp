@@ -1,5 +1,5 @@
 // Runs mrfixel on a case written by tests/test_fmls_kernel_emulated.py through the C ABI, with the
-// FMLS sort, segmentation, offset and emit kernels compiled for the CPU against emu_runtime.h
+// FMLS sort, growth, revision, offset and emit kernels compiled for the CPU against emu_runtime.h
 // (KERNEL_SOURCE: the kernel file without its launchers) and run one workgroup at a time.  The
 // amplitude GEMM (FMA in ascending coefficient order, as the GPU kernel accumulates, into the same
 // voxel-major layout) is restated here on the host.
@@ -23,9 +23,10 @@ void run_grid(unsigned grid, F&& body) {
   for (unsigned b = 0; b < grid; ++b) emu::run_workgroup([&]() { body(); }, b, grid);
 }
 
+// the revision over a grid of three workgroups, so that lanes stride over several items
 template <int LMAX>
-void run_segment(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0, size_t nv, unsigned nb) {
-  run_grid(nb, [&]() { fmls_segment_kernel<LMAX>(S, L, v0, nv); });
+void run_revise(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t v0) {
+  run_grid(3, [&]() { fmls_revise_kernel<LMAX>(S, L, v0); });
 }
 
 // the launch sequence of fmls_launch, each kernel run one workgroup at a time
@@ -49,15 +50,20 @@ hipError_t fmls_launch(const FmlsDeviceSet& S, const FmlsLaunch& L, hipStream_t)
     run_grid(unsigned((nv + S.lanes8 - 1) / S.lanes8), [&]() { fmls_grow_kernel<Lab8>(S, L, nv, S.lanes8, 0); });
     emu_grow_lds = lds16.data();
     run_grid(unsigned((nv + S.lanes16 - 1) / S.lanes16), [&]() { fmls_grow_kernel<Lab16>(S, L, nv, S.lanes16, 1); });
+    run_grid(nb, [&]() { fmls_block_sums(reinterpret_cast<const uint32_t*>(L.work.nlobes), 0, nv, L.sums); });
+    *L.work.nitems = 0ull;
+    run_grid(1, [&]() { fmls_scan_sums(L.sums, nb, L.work.nitems); });
+    run_grid(nb, [&]() { fmls_lobe_items(L, nv); });
     switch (S.lmax) {
-      case 2: run_segment<2>(S, L, v0, nv, nb); break;
-      case 4: run_segment<4>(S, L, v0, nv, nb); break;
-      case 6: run_segment<6>(S, L, v0, nv, nb); break;
-      case 8: run_segment<8>(S, L, v0, nv, nb); break;
-      case 10: run_segment<10>(S, L, v0, nv, nb); break;
-      case 12: run_segment<12>(S, L, v0, nv, nb); break;
+      case 2: run_revise<2>(S, L, v0); break;
+      case 4: run_revise<4>(S, L, v0); break;
+      case 6: run_revise<6>(S, L, v0); break;
+      case 8: run_revise<8>(S, L, v0); break;
+      case 10: run_revise<10>(S, L, v0); break;
+      case 12: run_revise<12>(S, L, v0); break;
       default: return hipErrorInvalidValue;
     }
+    run_grid(nb, [&]() { fmls_finalize_kernel(S, L, v0, nv); });
     run_grid(nb, [&]() { fmls_block_sums(L.count, v0, nv, L.sums); });
     run_grid(1, [&]() { fmls_scan_sums(L.sums, nb, L.total); });
     run_grid(nb, [&]() { fmls_emit(L, v0, nv); });
