@@ -1,10 +1,12 @@
 #include "fmls.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 #include <utility>
 
 #include "dense.h"
@@ -160,6 +162,10 @@ std::vector<int> hull_adjacency(const std::vector<double>& unit, int n) {
 std::vector<double> integration_weights(const std::vector<double>& unit, int n) {
   // A w = e: one row per even SH basis function up to lmax, one column per direction, solved in
   // the least-squares sense by Householder QR (the reference uses householderQr, fmls.cpp:141).
+  // The reflector of step k is applied to the columns after k by up to 16 host threads, each
+  // owning the columns j = t (mod threads) and the right-hand side thread 0; a spin barrier
+  // separates the steps.  Every column sees the same operations in the same order as a single
+  // thread would apply them, so the weights do not depend on the thread count.
   const int lmax = sh::LforN(n) + 2;
   Matrix dirs(n, 2);
   for (int i = 0; i < n; ++i) {
@@ -172,32 +178,58 @@ std::vector<double> integration_weights(const std::vector<double>& unit, int n) 
   std::vector<double> A(S.a);                   // column-major A: column d at A[d m .. d m + m)
   std::vector<double> b(size_t(m), 0.0);
   b[0] = 2.0 * std::sqrt(M_PI);
-  std::vector<double> v(static_cast<size_t>(m));
-  for (int k = 0; k < n; ++k) {
-    double* ck = &A[size_t(k) * m];
-    double nrm2 = 0.0;
-    for (int r = k; r < m; ++r) nrm2 += ck[r] * ck[r];
-    const double nrm = std::sqrt(nrm2);
-    if (nrm == 0.0) throw std::runtime_error("integration weights: rank-deficient direction set");
-    const double alpha = ck[k] > 0.0 ? -nrm : nrm;
-    for (int r = k; r < m; ++r) v[size_t(r)] = ck[r];
-    v[size_t(k)] -= alpha;
-    double vv = 0.0;
-    for (int r = k; r < m; ++r) vv += v[size_t(r)] * v[size_t(r)];
-    const double beta = 2.0 / vv;
-    ck[k] = alpha;
-    for (int j = k + 1; j < n; ++j) {
-      double* cj = &A[size_t(j) * m];
-      double s = 0.0;
-      for (int r = k; r < m; ++r) s += v[size_t(r)] * cj[r];
-      s *= beta;
-      for (int r = k; r < m; ++r) cj[r] -= s * v[size_t(r)];
+  const int nt = std::max(1, std::min({16, int(std::thread::hardware_concurrency()), n / 64}));
+  std::atomic<int> arrived{0}, phase{0};
+  std::atomic<bool> failed{false};
+  auto barrier = [&]() {
+    const int ph = phase.load(std::memory_order_acquire);
+    if (arrived.fetch_add(1, std::memory_order_acq_rel) == nt - 1) {
+      arrived.store(0, std::memory_order_relaxed);
+      phase.store(ph + 1, std::memory_order_release);
+    } else {
+      while (phase.load(std::memory_order_acquire) == ph) std::this_thread::yield();
     }
-    double s = 0.0;
-    for (int r = k; r < m; ++r) s += v[size_t(r)] * b[size_t(r)];
-    s *= beta;
-    for (int r = k; r < m; ++r) b[size_t(r)] -= s * v[size_t(r)];
-  }
+  };
+  auto worker = [&](int t) {
+    std::vector<double> v(static_cast<size_t>(m));
+    for (int k = 0; k < n; ++k) {
+      // the reflector of column k (final since step k - 1), formed by every thread alike
+      const double* ck = &A[size_t(k) * m];
+      double nrm2 = 0.0;
+      for (int r = k; r < m; ++r) nrm2 += ck[r] * ck[r];
+      const double nrm = std::sqrt(nrm2);
+      if (nrm == 0.0) failed.store(true);
+      const double alpha = ck[k] > 0.0 ? -nrm : nrm;
+      for (int r = k; r < m; ++r) v[size_t(r)] = ck[r];
+      v[size_t(k)] -= alpha;
+      double vv = 0.0;
+      for (int r = k; r < m; ++r) vv += v[size_t(r)] * v[size_t(r)];
+      const double beta = 2.0 / vv;
+      barrier();   // every thread has read column k
+      if (k % nt == t) A[size_t(k) * m + k] = alpha;
+      if (nrm != 0.0) {
+        for (int j = k + 1 + ((t - (k + 1) % nt + nt) % nt); j < n; j += nt) {
+          double* cj = &A[size_t(j) * m];
+          double s = 0.0;
+          for (int r = k; r < m; ++r) s += v[size_t(r)] * cj[r];
+          s *= beta;
+          for (int r = k; r < m; ++r) cj[r] -= s * v[size_t(r)];
+        }
+        if (t == 0) {
+          double s = 0.0;
+          for (int r = k; r < m; ++r) s += v[size_t(r)] * b[size_t(r)];
+          s *= beta;
+          for (int r = k; r < m; ++r) b[size_t(r)] -= s * v[size_t(r)];
+        }
+      }
+      barrier();   // columns after k and b updated
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(worker, t);
+  worker(0);
+  for (auto& x : th) x.join();
+  if (failed.load()) throw std::runtime_error("integration weights: rank-deficient direction set");
   std::vector<double> w(static_cast<size_t>(n));
   for (int k = n - 1; k >= 0; --k) {
     double s = b[size_t(k)];
