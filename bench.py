@@ -776,13 +776,13 @@ def bench_fixel(args, torch, ph):
                                "directions, default thresholds; FODs from the plain GPU CSD, resident in HBM"},
         "roofline": {"bound": "hbm", "achieved": hbm / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hbm / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fmls_amp_kernel + fmls_sort_kernel + fmls_grow_kernel + fmls_segment_kernel<%d> "
-                               "+ offset and emit kernels" % shared.lmax,
+                     "kernel": "fmls_amp_kernel + fmls_sort_kernel + fmls_grow_kernel + fmls_revise_kernel<%d> "
+                               "+ item, finalize, offset and emit kernels" % shared.lmax,
                      "kernel_ms": kern_ms, "algorithmic_bytes": hbm,
                      "note": "sequential per voxel (lobe growth in visiting order): a wavefront per voxel "
-                             "sorts, a lane per voxel grows lobes (labels in LDS) and revises peaks; the "
-                             "amplitudes and per-voxel work arrays in HBM are extra traffic; the amplitude "
-                             "GEMM is 2 x 1281 x n_sh FP64 FLOPs per voxel"},
+                             "sorts, a lane per voxel grows lobes (labels in LDS), a lane per lobe revises "
+                             "peaks; the amplitudes and per-voxel work arrays in HBM are extra traffic; the "
+                             "amplitude GEMM is 2 x 1281 x n_sh FP64 FLOPs per voxel"},
         "fixels": {"total": nfix, "per_voxel": nfix / ph.nvox},
     }
     if not args.no_cpu:
@@ -875,10 +875,10 @@ def bench_tournier(args, ph):
                                    "8 b=0 + 64 b=3000), -number 300 -iter_voxels 3000 -dilate 1 -max_iters 10; "
                                    "the DWI read once, resident on the GPU across iterations"},
             "roofline": {"bound": "latency", "achieved": None, "peak": None, "unit": None, "frac": None,
-                         "traffic": None, "kernel": "fmls_segment_kernel<4> (iteration 0, the whole mask)",
+                         "traffic": None, "kernel": "FMLS segmentation kernels (iteration 0, the whole mask)",
                          "kernel_ms": it0["fmls_ms"],
                          "note": "iteration 0 segments every masked voxel's lmax-4 FOD and dominates; its roofline is "
-                                 "the fod2fixel workload's (latency- and divergence-bound, one lane per voxel); the "
+                                 "the fod2fixel workload's (latency-bound, sequential lobe growth per voxel); the "
                                  "later iterations see the dilated top 3000 voxels only"},
             "iterations": timed[-1]["iterations"],
             "converged": timed[-1]["converged"],
