@@ -109,48 +109,61 @@ std::vector<int> hull_adjacency(const std::vector<double>& unit, int n) {
   // point lies above its plane.  Facets are small on a well-spread set, so candidates come from
   // the nearest neighbours of each vertex and the test runs over the nearest 48 points; the
   // result is checked against an independent convex hull in the tests.
+  // The points are split into ranges over up to 16 host threads, each collecting its own edges;
+  // the per-direction neighbour lists are sorted and made unique after, so the result does not
+  // depend on the split.
   const int N = 2 * n;
   auto P = [&](int i, int c) { return i < n ? unit[size_t(3 * i + c)] : -unit[size_t(3 * (i - n) + c)]; };
   const int K = std::min(N - 1, 12), Q = std::min(N - 1, 48);
-  std::vector<std::vector<int>> nb(static_cast<size_t>(n));
-  std::vector<std::pair<double, int>> near(static_cast<size_t>(N));
-  for (int i = 0; i < N; ++i) {
-    const double pi[3] = {P(i, 0), P(i, 1), P(i, 2)};
-    for (int j = 0; j < N; ++j) near[size_t(j)] = {j == i ? -2.0 : pi[0] * P(j, 0) + pi[1] * P(j, 1) + pi[2] * P(j, 2), j};
-    std::partial_sort(near.begin(), near.begin() + Q, near.end(),
-                      [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
-                        return x.first > y.first || (x.first == y.first && x.second < y.second);
-                      });
-    for (int ka = 0; ka < K; ++ka)
-      for (int kb = ka + 1; kb < K; ++kb) {
-        const int a = near[size_t(ka)].second, b = near[size_t(kb)].second;
-        const double ea[3] = {P(a, 0) - pi[0], P(a, 1) - pi[1], P(a, 2) - pi[2]};
-        const double eb[3] = {P(b, 0) - pi[0], P(b, 1) - pi[1], P(b, 2) - pi[2]};
-        double nr[3] = {ea[1] * eb[2] - ea[2] * eb[1], ea[2] * eb[0] - ea[0] * eb[2], ea[0] * eb[1] - ea[1] * eb[0]};
-        const double len = std::sqrt(dot3(nr, nr));
-        if (!(len > 0.0)) continue;
-        for (double& v : nr) v /= len;
-        double d = dot3(nr, pi);
-        if (d < 0.0) {
-          for (double& v : nr) v = -v;
-          d = -d;
+  const int nt = std::max(1, std::min({16, int(std::thread::hardware_concurrency()), N / 128}));
+  std::vector<std::vector<std::vector<int>>> nbt(static_cast<size_t>(nt),
+                                                 std::vector<std::vector<int>>(static_cast<size_t>(n)));
+  auto range = [&](int t) {
+    std::vector<std::vector<int>>& nb = nbt[size_t(t)];
+    std::vector<std::pair<double, int>> near(static_cast<size_t>(N));
+    for (int i = int(int64_t(N) * t / nt); i < int(int64_t(N) * (t + 1) / nt); ++i) {
+      const double pi[3] = {P(i, 0), P(i, 1), P(i, 2)};
+      for (int j = 0; j < N; ++j) near[size_t(j)] = {j == i ? -2.0 : pi[0] * P(j, 0) + pi[1] * P(j, 1) + pi[2] * P(j, 2), j};
+      std::partial_sort(near.begin(), near.begin() + Q, near.end(),
+                        [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+                          return x.first > y.first || (x.first == y.first && x.second < y.second);
+                        });
+      for (int ka = 0; ka < K; ++ka)
+        for (int kb = ka + 1; kb < K; ++kb) {
+          const int a = near[size_t(ka)].second, b = near[size_t(kb)].second;
+          const double ea[3] = {P(a, 0) - pi[0], P(a, 1) - pi[1], P(a, 2) - pi[2]};
+          const double eb[3] = {P(b, 0) - pi[0], P(b, 1) - pi[1], P(b, 2) - pi[2]};
+          double nr[3] = {ea[1] * eb[2] - ea[2] * eb[1], ea[2] * eb[0] - ea[0] * eb[2], ea[0] * eb[1] - ea[1] * eb[0]};
+          const double len = std::sqrt(dot3(nr, nr));
+          if (!(len > 0.0)) continue;
+          for (double& v : nr) v /= len;
+          double d = dot3(nr, pi);
+          if (d < 0.0) {
+            for (double& v : nr) v = -v;
+            d = -d;
+          }
+          bool facet = true;
+          for (int kq = 0; kq < Q && facet; ++kq) {
+            const int q = near[size_t(kq)].second;
+            if (q == a || q == b) continue;
+            if (nr[0] * P(q, 0) + nr[1] * P(q, 1) + nr[2] * P(q, 2) > d + 1e-12) facet = false;
+          }
+          if (!facet) continue;
+          const int v[3] = {i % n, a % n, b % n};
+          for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y)
+              if (v[x] != v[y]) nb[size_t(v[x])].push_back(v[y]);
         }
-        bool facet = true;
-        for (int kq = 0; kq < Q && facet; ++kq) {
-          const int q = near[size_t(kq)].second;
-          if (q == a || q == b) continue;
-          if (nr[0] * P(q, 0) + nr[1] * P(q, 1) + nr[2] * P(q, 2) > d + 1e-12) facet = false;
-        }
-        if (!facet) continue;
-        const int v[3] = {i % n, a % n, b % n};
-        for (int x = 0; x < 3; ++x)
-          for (int y = 0; y < 3; ++y)
-            if (v[x] != v[y]) nb[size_t(v[x])].push_back(v[y]);
-      }
-  }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(range, t);
+  range(0);
+  for (auto& x : th) x.join();
   std::vector<int> adj(size_t(n) * kMaxAdj, -1);
   for (int i = 0; i < n; ++i) {
-    auto& v = nb[size_t(i)];
+    std::vector<int> v;
+    for (int t = 0; t < nt; ++t) v.insert(v.end(), nbt[size_t(t)][size_t(i)].begin(), nbt[size_t(t)][size_t(i)].end());
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
     if (int(v.size()) > kMaxAdj) throw std::runtime_error("direction set: more than 8 neighbours for a direction");
