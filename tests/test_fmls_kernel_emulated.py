@@ -56,8 +56,9 @@ def stick_fod(dirs, weights, lmax=8):
 
 
 def fod_batch(seed=11, n=10):
-    """n single, crossing and three-fibre FODs with SH noise, plus the empty cases and the constant
-    FOD (rounding noise splits the sphere into over 300 lobes)."""
+    """n single, crossing and three-fibre FODs with SH noise, plus the empty cases, the constant
+    FOD (rounding noise splits the sphere into over 300 lobes) and axis-aligned sticks (amplitudes
+    equal up to rounding around each elevation ring, a peak on a pole)."""
     rng = np.random.default_rng(seed)
     fods, mask = [], []
     for k in range(n):
@@ -79,6 +80,8 @@ def fod_batch(seed=11, n=10):
     n = stick_fod([[0, 1, 0]], [0.5]); n[0] = np.nan
     fods.append(n); mask.append(1)                          # c0 NaN
     fods.append(stick_fod([[0.2, 0.1, 1]], [0.02])); mask.append(1)   # below the peak threshold
+    fods.append(stick_fod([[0, 0, 1]], [0.5])); mask.append(1)   # axially symmetric: near-equal amplitudes
+    fods.append(stick_fod([[1, 0, 0]], [0.5])); mask.append(1)
     return np.array(fods, dtype=np.float32), np.array(mask, dtype=np.uint8)
 
 

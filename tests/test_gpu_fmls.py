@@ -59,7 +59,9 @@ def edge_cases():
     z = stick_fod([[0, 1, 0]], [0.5]); z[0] = 0.0             # c0 = 0
     nan = stick_fod([[0, 1, 0]], [0.5]); nan[0] = np.nan      # c0 NaN
     weak = stick_fod([[0.2, 0.1, 1]], [0.02])                # below the peak threshold
-    return np.array([const, neg, z, nan, weak], dtype=np.float32)
+    zs = stick_fod([[0, 0, 1]], [0.5])                        # axially symmetric: near-equal amplitudes
+    xs = stick_fod([[1, 0, 0]], [0.5])
+    return np.array([const, neg, z, nan, weak, zs, xs], dtype=np.float32)
 
 
 def expected(o, fods, mask=None, maxnum=0, dirpeak=False):
