@@ -858,12 +858,12 @@ __device__ __forceinline__ void fmls_sorted_out(const FmlsDeviceSet& S, const Fm
   for (int k = 0; k < PER; ++k) {
     const int e = lane * PER + k;
     if (e < cnt) {
-      const int d = idx[k];
+      const AdjRow ar = adj_row(S, idx[k]);   // one 16-byte load per direction
       unsigned m = 0;
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
-        const int j = S.adj[d * 8 + r];
-        m |= (j >= 0 && (int)s_pos[j] < e) ? 1u << r : 0u;
+        const int j = (int16_t)((ar.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
+        m |= (j >= 0 && (int)s_pos[j < 0 ? 0 : j] < e) ? 1u << r : 0u;
       }
       L.work.nbr[(size_t)e * cs + slot] = (uint8_t)m;
     }
