@@ -55,10 +55,10 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 size_t fmls_work_bytes(int ndir, size_t chunk) {
   const size_t n = size_t(ndir) * chunk;
   // doubles: amps (fmls_amp_ld(ndir) per slot), peak, integ (n each), mean, rev (3 n each); int16:
-  // lab, ord, nxt, head, tail, np; uint8: nbr; int32: npos, nlobes (one per slot); uint32: items
+  // lab, nxt, head, tail, np; uint32: ordm (n); int32: npos, nlobes (one per slot); uint32: items
   // (n); the item count -- the pieces fmls_work_carve takes, in its order
   return align256(8 * size_t(fmls_amp_ld(ndir)) * chunk) + 2 * align256(8 * n) + 2 * align256(8 * 3 * n) +
-         6 * align256(2 * n) + align256(n) + 2 * align256(4 * chunk) + align256(4 * n) + align256(8);
+         5 * align256(2 * n) + align256(4 * n) + 2 * align256(4 * chunk) + align256(4 * n) + align256(8);
 }
 
 FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
@@ -76,12 +76,11 @@ FmlsWork fmls_work_carve(void* base, int ndir, size_t chunk) {
   W.mean = reinterpret_cast<double*>(take(8 * 3 * n));
   W.rev = reinterpret_cast<double*>(take(8 * 3 * n));
   W.lab = reinterpret_cast<int16_t*>(take(2 * n));
-  W.ord = reinterpret_cast<uint16_t*>(take(2 * n));
   W.nxt = reinterpret_cast<int16_t*>(take(2 * n));
   W.head = reinterpret_cast<int16_t*>(take(2 * n));
   W.tail = reinterpret_cast<int16_t*>(take(2 * n));
   W.np = reinterpret_cast<int16_t*>(take(2 * n));
-  W.nbr = reinterpret_cast<uint8_t*>(take(n));
+  W.ordm = reinterpret_cast<uint32_t*>(take(4 * n));
   W.npos = reinterpret_cast<int32_t*>(take(4 * chunk));
   W.nlobes = reinterpret_cast<int32_t*>(take(4 * chunk));
   W.items = reinterpret_cast<uint32_t*>(take(4 * n));

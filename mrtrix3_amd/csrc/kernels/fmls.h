@@ -52,8 +52,8 @@ inline int fmls_grow_lanes(int ndir, int bytes, int lds_cu, int lds_block) {
 struct FmlsWork {
   double* amps;              // [chunk][fmls_amp_ld(ndir)] amplitudes on the directions
   int16_t* lab;              // [ndir] lobe of each direction; -1 none; -2 - l: waits for lobe l
-  uint16_t* ord;             // [ndir] positive directions in visiting order
-  uint8_t* nbr;              // [ndir] bit r set: neighbour r of ord[i] is visited before it
+  uint32_t* ordm;            // [ndir] positive directions in visiting order (bits 0-15) and which of
+                             // their neighbours are visited before them (bit 16 + r: neighbour r)
   int16_t* nxt;              // [ndir] peak lists through the seed directions
   double* peak;              // [ndir] lobe records: maximal peak amplitude
   double* mean;              // [3 ndir] amplitude-weighted direction sum, then the fixel direction

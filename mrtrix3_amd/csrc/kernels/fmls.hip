@@ -235,8 +235,7 @@ __device__ int fmls_select(const FmlsDeviceSet& S, const double* p) {
 // reference's order.
 struct Tabs {
   Lane<int16_t> lab;
-  Lane<uint16_t> ord;
-  Lane<uint8_t> nbr;
+  Lane<uint32_t> ordm;
   Lane<int16_t> nxt;
   Lane<double> peak, mean, integ;
   Lane<int16_t> head, tail, np;
@@ -245,7 +244,7 @@ struct Tabs {
 
 __device__ Tabs tabs_of(const FmlsWork& W, size_t slot, size_t cs) {
   const uint32_t o = (uint32_t)slot, c = (uint32_t)cs;
-  return Tabs{{W.lab, c, o},  {W.ord, c, o},   {W.nbr, c, o},  {W.nxt, c, o}, {W.peak, c, o}, {W.mean, c, o},
+  return Tabs{{W.lab, c, o},  {W.ordm, c, o},  {W.nxt, c, o}, {W.peak, c, o}, {W.mean, c, o},
               {W.integ, c, o}, {W.head, c, o}, {W.tail, c, o}, {W.np, c, o},  {W.rev, c, o}};
 }
 
@@ -419,11 +418,6 @@ struct LdsLab {
 using Lab8 = LdsLab<int8_t, 127>;        // waiting labels down to -128: lobes 0..126
 using Lab16 = LdsLab<int16_t, 32767>;    // any direction count
 
-// Lobe growth and the retrospective assignments of one voxel (fmls.cpp:188-252) from its P > 0
-// positive directions in visiting order (ord, with their amplitudes in rev and the
-// visited-before neighbour masks in nbr), its labels in `lab`; returns the number of positive
-// lobes, or -1 when the voxel needs more lobes than `lab` can number (nothing in ord, rev or nbr
-// is changed, so the voxel can be grown again with wider labels).
 // The ascending distinct lobes among d's neighbour labels (INT_MAX pads), each lobe merged into
 // the lowest-numbered one in ascending order, d added to it, and the labels and lobe records
 // renumbered over the erased records (fmls.cpp:203-237); only lobe merge ratios below 1 can
@@ -488,10 +482,10 @@ __device__ __forceinline__ int grow_merge(const FmlsDeviceSet& S, const Tabs& T,
 }
 
 // Lobe growth and the retrospective assignments of one voxel (fmls.cpp:188-252) from its P > 0
-// positive directions in visiting order (ord, with their amplitudes in rev and the
-// visited-before neighbour masks in nbr), its labels in `lab`; returns the number of positive
-// lobes, or -1 when the voxel needs more lobes than `lab` can number (nothing in ord, rev or nbr
-// is changed, so the voxel can be grown again with wider labels).
+// positive directions in visiting order (ordm, with their visited-before neighbour masks, and
+// their amplitudes in rev), its labels in `lab`; returns the number of positive lobes, or -1 when
+// the voxel needs more lobes than `lab` can number (nothing in ordm or rev is changed, so the
+// voxel can be grown again with wider labels).
 //
 // The global loads of a block of eight steps (visiting order, neighbour masks, amplitudes, then
 // the directions' unit vectors, weights and adjacency rows) are issued before its first step.
@@ -511,13 +505,13 @@ __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab&
   // carried over would make the compiler wait for everything at the loop head).  Indices past the
   // end are clamped to the last entry: in bounds, never used.
   for (int i0 = 0; i0 < P; i0 += KA) {
-    int ra[KA];      // ord | nbr << 16
+    int ra[KA];      // direction | visited-before neighbour mask << 16
     double rv[KA];   // amplitude
     DirInfo rb[KA];
 #pragma unroll
     for (int j = 0; j < KA; ++j) {
       const int ia = i0 + j < P ? i0 + j : P - 1;
-      ra[j] = (int)T.ord[ia] | ((int)T.nbr[ia] << 16);
+      ra[j] = (int)T.ordm[ia];
       rv[j] = T.rev[ia];
     }
 #pragma unroll
@@ -572,7 +566,7 @@ __device__ int fmls_grow_voxel(const FmlsDeviceSet& S, const Tabs& T, const Lab&
   // retrospective assignments in visiting order (fmls.cpp:251-252): the waiting directions, whose
   // labels still name the lobe they wait for (renumbered by later merges, never assigned)
   for (int i = 0; i < P; ++i) {
-    const int d = T.ord[i];
+    const int d = (int)(T.ordm[i] & 0xffffu);
     const int lb = lab.get(d);
     if (lb <= -2) {
       const int l = -2 - lb;
@@ -835,8 +829,8 @@ __device__ __forceinline__ void fmls_bitonic(double (&key)[PER], int (&idx)[PER]
   }
 }
 
-// The sorted entries [0, cnt) to ord and rev, their positions to s_pos (LDS, ndir entries), then
-// each one's mask of neighbours visited before it to nbr.
+// The sorted entries [0, cnt): their amplitudes to rev and their positions to s_pos (LDS, ndir
+// entries), then each direction with its mask of neighbours visited before it to ordm.
 template <int PER>
 __device__ __forceinline__ void fmls_sorted_out(const FmlsDeviceSet& S, const FmlsLaunch& L, size_t slot, int lane,
                                                 int cnt, const double (&key)[PER], const int (&idx)[PER],
@@ -849,7 +843,6 @@ __device__ __forceinline__ void fmls_sorted_out(const FmlsDeviceSet& S, const Fm
     const int e = lane * PER + k;
     if (e < cnt) {
       s_pos[idx[k]] = (uint16_t)e;
-      L.work.ord[(size_t)e * cs + slot] = (uint16_t)idx[k];
       L.work.rev[(size_t)e * cs + slot] = key[k];
     }
   }
@@ -865,7 +858,7 @@ __device__ __forceinline__ void fmls_sorted_out(const FmlsDeviceSet& S, const Fm
         const int j = (int16_t)((ar.w[r / 2] >> (16 * (r % 2))) & 0xffffu);
         m |= (j >= 0 && (int)s_pos[j < 0 ? 0 : j] < e) ? 1u << r : 0u;
       }
-      L.work.nbr[(size_t)e * cs + slot] = (uint8_t)m;
+      L.work.ordm[(size_t)e * cs + slot] = (uint32_t)idx[k] | (m << 16);
     }
   }
 }
@@ -885,9 +878,9 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned n) {
 // among equal ones) is positive (fmls.cpp:171-184), then the positive directions in visiting order
 // by a bitonic sort of (amplitude, direction) entries held in registers, padded with entries
 // visited after every positive one: with at most 1024 positive directions (nearly every voxel)
-// they are first packed through LDS and sorted as 64 x 16 entries, otherwise as 64 x 32.  Writes npos (0: nothing to segment), ord,
-// rev (the amplitudes) and nbr (which neighbours of each direction are visited before it, from
-// the directions' positions in an LDS table) for the segmentation kernel.
+// they are first packed through LDS and sorted as 64 x 16 entries, otherwise as 64 x 32.  Writes
+// npos (0: nothing to segment), ordm (the directions and which of their neighbours are visited
+// before them, from the directions' positions in an LDS table) and rev (the amplitudes).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void fmls_sort_kernel(
     FmlsDeviceSet S, FmlsLaunch L, size_t v0, size_t nv) {
   // the small sort's staging area (keys, then directions), then the visiting position of each
